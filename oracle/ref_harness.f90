@@ -1,0 +1,98 @@
+! C-callable entry points onto the reference Fortran solvers -- TEST INFRASTRUCTURE ONLY.
+!
+! Linked by oracle/Makefile together with the reference's own, unmodified sources
+! (src/modules.f90 and the six solver files, compiled where they lie), into
+! oracle/_ref/libpsdm_ref.so.  Used by tests/golden/make_golden.py to write golden
+! vectors and by bench.py's cpu_baseline leg ("kind": "reference").
+!
+! Each wrapper sets the two module switches the solvers read (mod_parfile: dPML, fs;
+! solver_ac_forward_ram.f90:8) and calls the solver with the reference's own
+! argument list (solver_ac_forward_ram.f90:4-6, solver_ac_backward_ram.f90:4-6,
+! solver_ac_forward_disk.f90:4-7, solver_ac_backward_disk.f90:4-7).
+module ref_harness
+  use iso_c_binding
+  use mod_parfile, only: dPML, fs
+  implicit none
+contains
+
+  subroutine ref_forward_ram(dpml_in, fs_in, NumSou, NumRec, nxSou, nySou, nxRec, nyRec, S, dmodel, dt, &
+                             ny, nx, ctmp, store_snap, nt_snap, nt, Fields, Rec_point) bind(C, name="ref_forward_ram")
+    integer(c_int), value :: dpml_in, fs_in, NumSou, NumRec, ny, nx, store_snap, nt_snap, nt
+    integer(c_int) :: nxSou(NumSou), nySou(NumSou), nxRec(NumRec), nyRec(NumRec)
+    real(c_float), value :: dmodel, dt
+    real(c_float) :: S(nt, NumSou), ctmp(ny, nx), Fields(ny, nx, nt_snap), Rec_point(nt, NumRec)
+    integer :: rank, ns, nr, ny_, nx_, ss, nts, nt_
+    real :: dm, dtt
+    rank = 0; ns = NumSou; nr = NumRec; ny_ = ny; nx_ = nx; ss = store_snap; nts = nt_snap; nt_ = nt
+    dm = dmodel; dtt = dt
+    dPML = dpml_in; fs = fs_in
+    call solver_ac_forward_ram(rank, ns, nr, nxSou, nySou, nxRec, nyRec, S, dm, dtt, ny_, nx_, ctmp, &
+                               ss, nts, nt_, Fields, Rec_point)
+  end subroutine
+
+  subroutine ref_backward_ram(dpml_in, fs_in, NumSou, NumRec, nxSou, nySou, nxRec, nyRec, Sadj, dmodel, dt, &
+                              ny, nx, ctmp, store_snap, nt_snap, nt, Fields, kernel) bind(C, name="ref_backward_ram")
+    integer(c_int), value :: dpml_in, fs_in, NumSou, NumRec, ny, nx, store_snap, nt_snap, nt
+    integer(c_int) :: nxSou(NumSou), nySou(NumSou), nxRec(NumRec), nyRec(NumRec)
+    real(c_float), value :: dmodel, dt
+    real(c_float) :: Sadj(nt, NumRec), ctmp(ny, nx), Fields(ny, nx, nt_snap), kernel(ny, nx)
+    integer :: rank, ns, nr, ny_, nx_, ss, nts, nt_
+    real :: dm, dtt
+    rank = 0; ns = NumSou; nr = NumRec; ny_ = ny; nx_ = nx; ss = store_snap; nts = nt_snap; nt_ = nt
+    dm = dmodel; dtt = dt
+    dPML = dpml_in; fs = fs_in
+    call solver_ac_backward_ram(rank, ns, nr, nxSou, nySou, nxRec, nyRec, Sadj, dm, dtt, ny_, nx_, ctmp, &
+                                ss, nts, nt_, Fields, kernel)
+  end subroutine
+
+  subroutine c_to_folder(cstr, folder)
+    character(kind=c_char) :: cstr(*)
+    character(len=500) :: folder
+    integer :: n
+    folder = ' '
+    n = 0
+    do while (cstr(n + 1) /= c_null_char .and. n < 499)
+      n = n + 1
+      folder(n:n) = cstr(n)
+    enddo
+  end subroutine
+
+  subroutine ref_forward_disk(dpml_in, fs_in, NumSou, NumRec, nxSou, nySou, nxRec, nyRec, S, dmodel, dt, &
+                              ny, nx, ctmp, shotid, folder_c, store_snap, nt_snap, nt, Rec_point) &
+                              bind(C, name="ref_forward_disk")
+    integer(c_int), value :: dpml_in, fs_in, NumSou, NumRec, ny, nx, store_snap, nt_snap, nt, shotid
+    integer(c_int) :: nxSou(NumSou), nySou(NumSou), nxRec(NumRec), nyRec(NumRec)
+    real(c_float), value :: dmodel, dt
+    real(c_float) :: S(nt, NumSou), ctmp(ny, nx), Rec_point(nt, NumRec)
+    character(kind=c_char) :: folder_c(*)
+    character(len=500) :: folder
+    integer :: rank, ns, nr, ny_, nx_, ss, nts, nt_, sid, unit_f
+    real :: dm, dtt
+    call c_to_folder(folder_c, folder)
+    rank = 0; ns = NumSou; nr = NumRec; ny_ = ny; nx_ = nx; ss = store_snap; nts = nt_snap; nt_ = nt
+    sid = shotid; unit_f = 77; dm = dmodel; dtt = dt
+    dPML = dpml_in; fs = fs_in
+    call solver_ac_forward_disk(rank, ns, nr, nxSou, nySou, nxRec, nyRec, S, dm, dtt, ny_, nx_, ctmp, &
+                                sid, unit_f, folder, ss, nts, nt_, Rec_point)
+  end subroutine
+
+  subroutine ref_backward_disk(dpml_in, fs_in, NumSou, NumRec, nxSou, nySou, nxRec, nyRec, Sadj, dmodel, dt, &
+                               ny, nx, ctmp, shotid, folder_c, store_snap, nt_snap, nt, kernel) &
+                               bind(C, name="ref_backward_disk")
+    integer(c_int), value :: dpml_in, fs_in, NumSou, NumRec, ny, nx, store_snap, nt_snap, nt, shotid
+    integer(c_int) :: nxSou(NumSou), nySou(NumSou), nxRec(NumRec), nyRec(NumRec)
+    real(c_float), value :: dmodel, dt
+    real(c_float) :: Sadj(nt, NumRec), ctmp(ny, nx), kernel(ny, nx)
+    character(kind=c_char) :: folder_c(*)
+    character(len=500) :: folder
+    integer :: rank, ns, nr, ny_, nx_, ss, nts, nt_, sid, unit_f
+    real :: dm, dtt
+    call c_to_folder(folder_c, folder)
+    rank = 0; ns = NumSou; nr = NumRec; ny_ = ny; nx_ = nx; ss = store_snap; nts = nt_snap; nt_ = nt
+    sid = shotid; unit_f = 77; dm = dmodel; dtt = dt
+    dPML = dpml_in; fs = fs_in
+    call solver_ac_backward_disk(rank, ns, nr, nxSou, nySou, nxRec, nyRec, Sadj, dm, dtt, ny_, nx_, ctmp, &
+                                 sid, unit_f, folder, ss, nts, nt_, kernel)
+  end subroutine
+
+end module ref_harness

@@ -1,0 +1,180 @@
+/*
+ * Oracle for PSDM's RTM hot path -- TEST INFRASTRUCTURE ONLY (see rtm_oracle_impl.h).
+ *
+ * Exports the reference-faithful FP32 solver and its FP64 twin through a plain C ABI
+ * so that tests/ (via ctypes) and bench.py's cpu_baseline leg can run it.  Nothing
+ * in psdm_amd/ links or loads this library.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* finite_difference_coef, solver_FD_coeffs.f90:1-73, accuracy 3 (the only order the
+ * solver uses, solver_ac_forward_ram.f90:95).  num=1: first derivative
+ * (:14-19), num=2: second derivative with coef(1) halved (:43-50, :68). */
+static void fd_coef_f32(int num, float *c) {
+    if (num == 1) {
+        c[0] = 0.f;
+        c[1] = 3.f / 4.f;
+        c[2] = -3.f / 20.f;
+        c[3] = 1.f / 60.f;
+    } else {
+        c[0] = -49.f / 18.f;
+        c[1] = 3.f / 2.f;
+        c[2] = -3.f / 20.f;
+        c[3] = 1.f / 90.f;
+        c[0] = c[0] / 2.f;
+    }
+}
+static void fd_coef_f64(int num, double *c) {
+    if (num == 1) {
+        c[0] = 0.;
+        c[1] = 3. / 4.;
+        c[2] = -3. / 20.;
+        c[3] = 1. / 60.;
+    } else {
+        c[0] = -49. / 18.;
+        c[1] = 3. / 2.;
+        c[2] = -3. / 20.;
+        c[3] = 1. / 90.;
+        c[0] = c[0] / 2.;
+    }
+}
+
+/* 1D damping profile of PML_coeffs, solver_PML_coeffs.f90:16-34.
+ * mode 1: FP32 arithmetic in the reference's expression order;
+ * mode 0: FP64 arithmetic.  Result in double either way. */
+static void pml_profile(int mode, int acc, int dPML, float dt_f, float d_f, int dn, double *A, double *B) {
+    for (int i = 0; i < dn; i++) A[i] = B[i] = 0.;
+    if (mode == 1) {
+        float pi = 3.14159265f, Refl = 0.00001f, d = d_f, dt = dt_f;
+        float wPML = (float)(dPML - 1) * d;
+        float Af[1024], Bf[1024];
+        for (int i = acc; i < dPML + acc; i++) { /* 1-based i = 1+acc .. dPML+acc */
+            float y = (float)(i - acc) * d;
+            float alpha = 30.f * pi * (1.f - (y / wPML));
+            float sigma = (-((float)((2 + 1) * 2000) * logf(Refl)) / (2.f * wPML)) * ((y / wPML) * (y / wPML));
+            Af[i] = expf(-(sigma + alpha) * dt);
+            Bf[i] = (Af[i] - 1.f) * sigma / (sigma + alpha);
+        }
+        for (int i = acc; i < dPML + acc; i++) {
+            A[i] = Af[i];
+            B[i] = Bf[i];
+        }
+    } else {
+        double pi = 3.14159265, Refl = 0.00001, d = d_f, dt = dt_f;
+        double wPML = (double)(dPML - 1) * d;
+        for (int i = acc; i < dPML + acc; i++) {
+            double y = (double)(i - acc) * d;
+            double alpha = 30. * pi * (1. - (y / wPML));
+            double sigma = (-((double)((2 + 1) * 2000) * log(Refl)) / (2. * wPML)) * ((y / wPML) * (y / wPML));
+            A[i] = exp(-(sigma + alpha) * dt);
+            B[i] = (A[i] - 1.) * sigma / (sigma + alpha);
+        }
+    }
+    for (int i = dPML + acc; i < dn; i++) { /* :283-286 */
+        A[i] = A[dPML + acc - 1];
+        B[i] = B[dPML + acc - 1];
+    }
+}
+
+/* Corner tables, built with the reference's own loops (solver_PML_coeffs.f90:288-322).
+ * Order of the four outputs: ld, rd, lu, ru; each (dn,dn) column-major. */
+#define CORNERS(R)                                                                                  \
+    static void corners_##R(int dn, const R *f, R *ld, R *rd, R *lu, R *ru) {                      \
+        for (int i = 1; i <= dn; i++)                                                               \
+            for (int j = 1; j <= i; j++) rd[(i - 1) * dn + (j - 1)] = f[i - 1];                    \
+        for (int i = 1; i <= dn; i++)                                                               \
+            for (int j = i + 1; j <= dn; j++) rd[(i - 1) * dn + (j - 1)] = rd[(j - 1) * dn + (i - 1)]; \
+        for (int i = 1; i <= dn; i++)                                                               \
+            for (int j = 1; j <= dn - i + 1; j++) ld[(i - 1) * dn + (j - 1)] = f[dn - i];          \
+        for (int ix = 1; ix <= dn - 1; ix++) {                                                      \
+            int j = 2 + (ix - 1);                                                                   \
+            for (int i = dn - (ix - 1); i <= dn; i++) ld[(i - 1) * dn + (j - 1)] = f[j - 1];        \
+        }                                                                                           \
+        for (int j = 1; j <= dn; j++)                                                               \
+            for (int i = 1; i <= dn; i++) {                                                         \
+                lu[(i - 1) * dn + (j - 1)] = ld[(i - 1) * dn + (dn - j)];                           \
+                ru[(i - 1) * dn + (j - 1)] = rd[(i - 1) * dn + (dn - j)];                           \
+            }                                                                                       \
+    }
+CORNERS(float)
+CORNERS(double)
+
+static void pml_tables_f32(int mode, int acc, int dPML, float dt, float d, int dn, float *fA, float *fB,
+                           float *ldA, float *rdA, float *luA, float *ruA, float *ldB, float *rdB, float *luB,
+                           float *ruB) {
+    double A[1024], B[1024];
+    pml_profile(mode, acc, dPML, dt, d, dn, A, B);
+    for (int i = 0; i < dn; i++) {
+        fA[i] = (float)A[i];
+        fB[i] = (float)B[i];
+    }
+    corners_float(dn, fA, ldA, rdA, luA, ruA);
+    corners_float(dn, fB, ldB, rdB, luB, ruB);
+}
+static void pml_tables_f64(int mode, int acc, int dPML, float dt, float d, int dn, double *fA, double *fB,
+                           double *ldA, double *rdA, double *luA, double *ruA, double *ldB, double *rdB,
+                           double *luB, double *ruB) {
+    pml_profile(mode, acc, dPML, dt, d, dn, fA, fB);
+    corners_double(dn, fA, ldA, rdA, luA, ruA);
+    corners_double(dn, fB, ldB, rdB, luB, ruB);
+}
+
+#define R float
+#define SUF _f32
+#include "rtm_oracle_impl.h"
+#undef R
+#undef SUF
+
+#define R double
+#define SUF _f64
+#include "rtm_oracle_impl.h"
+#undef R
+#undef SUF
+
+/* ------------------------------------------------------------------ exports */
+
+int rtm_oracle_solver_f32(int iprop, int variant, int table_mode, int dPML, int fs, int NumSou, int NumRec,
+                          const int *nxSou, const int *nySou, const int *nxRec, const int *nyRec, const float *S,
+                          float dmodel, float dt, int ny, int nx, const float *ctmp, int store_snap, int nt_snap,
+                          int nt, float *Fields, float *Rec_point, float *kernel) {
+    return solver_f32(iprop, variant, table_mode, dPML, fs, NumSou, NumRec, nxSou, nySou, nxRec, nyRec, S, dmodel,
+                      dt, ny, nx, ctmp, store_snap, nt_snap, nt, Fields, Rec_point, kernel);
+}
+
+int rtm_oracle_solver_f64(int iprop, int variant, int table_mode, int dPML, int fs, int NumSou, int NumRec,
+                          const int *nxSou, const int *nySou, const int *nxRec, const int *nyRec, const float *S,
+                          float dmodel, float dt, int ny, int nx, const float *ctmp, int store_snap, int nt_snap,
+                          int nt, double *Fields, double *Rec_point, double *kernel) {
+    return solver_f64(iprop, variant, table_mode, dPML, fs, NumSou, NumRec, nxSou, nySou, nxRec, nyRec, S, dmodel,
+                      dt, ny, nx, ctmp, store_snap, nt_snap, nt, Fields, Rec_point, kernel);
+}
+
+/* Tables as the oracle builds them: fA,fB (dn); corners (4,dn,dn) in order ld, rd, lu, ru. */
+void rtm_oracle_pml_tables_f32(int mode, int dPML, float dt, float d, float *fA, float *fB, float *cA, float *cB) {
+    int dn = dPML + 6;
+    size_t q = (size_t)dn * dn;
+    pml_tables_f32(mode, 3, dPML, dt, d, dn, fA, fB, cA, cA + q, cA + 2 * q, cA + 3 * q, cB, cB + q, cB + 2 * q,
+                   cB + 3 * q);
+}
+void rtm_oracle_pml_tables_f64(int mode, int dPML, float dt, float d, double *fA, double *fB, double *cA,
+                               double *cB) {
+    int dn = dPML + 6;
+    size_t q = (size_t)dn * dn;
+    pml_tables_f64(mode, 3, dPML, dt, d, dn, fA, fB, cA, cA + q, cA + 2 * q, cA + 3 * q, cB, cB + q, cB + 2 * q,
+                   cB + 3 * q);
+}
+void rtm_oracle_fd_coef_f64(int num, double *c) { fd_coef_f64(num, c); }
+void rtm_oracle_fd_coef_f32(int num, float *c) { fd_coef_f32(num, c); }
+
+/* One full-grid derivative pass: which = 0 d2/dx2, 1 d2/dy2, 2 d/dx, 3 d/dy (grid (nyt,nxt)). */
+void rtm_oracle_deriv_f64(int which, int nyt, int nxt, const double *p, double *d) {
+    double c1[4], c2[4];
+    fd_coef_f64(1, c1);
+    fd_coef_f64(2, c2);
+    if (which == 0) deriv2x_f64(3, c2, nyt, nxt, p, d);
+    else if (which == 1) deriv2y_f64(3, c2, nyt, nxt, p, d);
+    else if (which == 2) derivx_f64(3, c1, nyt, nxt, p, d);
+    else derivy_f64(3, c1, nyt, nxt, p, d);
+}
