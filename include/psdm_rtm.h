@@ -1,0 +1,129 @@
+/*
+ * psdm_rtm.h -- C ABI of the MI355X (gfx950) RTM solver library libpsdm_rtm.so.
+ *
+ * Drop-in for the four time-stepping solvers of PSDM's RTM path.  The reference calls
+ * them from dynamic_window.f90:90-92,134-137,177-179,221-224; the Fortran iso_c_binding
+ * shim psdm_amd/csrc/psdm_rtm_fortran.f90 defines subroutines with the reference's
+ * names and argument lists and forwards to the entry points below, so
+ * calculate_PSDM_RTM.f90 and dynamic_window.f90 link unchanged.
+ *
+ * Conventions (as in the reference): arrays are column-major with depth fastest;
+ * geometry indices are 1-based grid indices inside the shot window; scalars are
+ * int32 / float32.  Every entry point returns 0 on success and a negative code on
+ * error; psdm_rtm_last_error() then describes the failure.  A context is not
+ * re-entrant: use one per host thread.
+ */
+#ifndef PSDM_RTM_H
+#define PSDM_RTM_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct psdm_rtm_ctx psdm_rtm_ctx;
+
+enum {
+    PSDM_RTM_OK = 0,
+    PSDM_RTM_EINVAL = -1,  /* argument outside the solver's defined range */
+    PSDM_RTM_EDEVICE = -2, /* HIP runtime error (allocation, launch) */
+    PSDM_RTM_ENOENT = -3,  /* no device-resident snapshots under that key */
+};
+
+/* Context on HIP device `device`; owns the device buffers and one stream. */
+int psdm_rtm_create(psdm_rtm_ctx **ctx, int device);
+int psdm_rtm_destroy(psdm_rtm_ctx *ctx);
+const char *psdm_rtm_last_error(const psdm_rtm_ctx *ctx);
+
+/*
+ * Replaces solver_ac_forward_ram (solver_ac_forward_ram.f90:4-6).
+ * dPML, fs: the mod_parfile values the reference reads (solver_ac_forward_ram.f90:8).
+ * S(nt,NumSou); ctmp(ny,nx); Rec_point(nt,NumRec) is overwritten.
+ * Fields(ny,nx,nt_snap): host_fields != 0 -> snapshots are copied into Fields (needed
+ * when the caller tapers them, calculate_PSDM_RTM.f90:421-427); host_fields == 0 ->
+ * snapshots stay in HBM, keyed by the Fields pointer, for the matching backward call.
+ */
+int psdm_rtm_forward_ram(psdm_rtm_ctx *ctx, int dPML, int fs, int NumSou, int NumRec, const int *nxSou,
+                         const int *nySou, const int *nxRec, const int *nyRec, const float *S, float dmodel,
+                         float dt, int ny, int nx, const float *ctmp, int store_snap, int nt_snap, int nt,
+                         float *Fields, int host_fields, float *Rec_point);
+
+/*
+ * Replaces solver_ac_backward_ram (solver_ac_backward_ram.f90:4-6).
+ * Sadj(nt,NumRec); kernel(ny,nx) is accumulated (+=) as in the reference.
+ * host_fields != 0: snapshots are read from Fields; == 0: from HBM under the key
+ * Fields left by psdm_rtm_forward_ram, which are released afterwards.
+ */
+int psdm_rtm_backward_ram(psdm_rtm_ctx *ctx, int dPML, int fs, int NumSou, int NumRec, const int *nxSou,
+                          const int *nySou, const int *nxRec, const int *nyRec, const float *Sadj, float dmodel,
+                          float dt, int ny, int nx, const float *ctmp, int store_snap, int nt_snap, int nt,
+                          const float *Fields, int host_fields, float *kernel);
+
+/*
+ * Replace solver_ac_forward_disk / solver_ac_backward_disk (solver_ac_forward_disk.f90:4-7,
+ * solver_ac_backward_disk.f90:4-7).  The per-step snapshot files of the reference
+ * become device-resident snapshots keyed by shotid; the backward call consumes and
+ * frees them.  The _disk stepping order (strips scattered after the free-surface row
+ * is zeroed, full-width velocity pads) is kept.
+ */
+int psdm_rtm_forward_disk(psdm_rtm_ctx *ctx, int dPML, int fs, int NumSou, int NumRec, const int *nxSou,
+                          const int *nySou, const int *nxRec, const int *nyRec, const float *S, float dmodel,
+                          float dt, int ny, int nx, const float *ctmp, int shotid, int store_snap, int nt_snap,
+                          int nt, float *Rec_point);
+int psdm_rtm_backward_disk(psdm_rtm_ctx *ctx, int dPML, int fs, int NumSou, int NumRec, const int *nxSou,
+                           const int *nySou, const int *nxRec, const int *nyRec, const float *Sadj, float dmodel,
+                           float dt, int ny, int nx, const float *ctmp, int shotid, int store_snap, int nt_snap,
+                           int nt, float *kernel);
+
+/* ------------------------------------------------------------------------------
+ * Device-resident form (no counterpart in the reference: the one-shot drivers above
+ * and the batched engine are built on it).  All float pointers are device pointers
+ * on the context's device; geometry arrays are host pointers.
+ * ------------------------------------------------------------------------------ */
+typedef struct {
+    int ny, nx, nt;          /* shot window (depth samples, columns), time steps */
+    int dPML, fs;            /* CPML width, free surface flag (mod_parfile) */
+    int variant;             /* 0 = _ram stepping semantics, 1 = _disk */
+    int store_snap, nt_snap; /* snapshot interval and count, as in the reference */
+    float dmodel, dt;
+    int NumSou, NumRec;
+    const int *nxSou, *nySou; /* forward injection points (1-based) */
+    const int *nxRec, *nyRec; /* receivers (1-based) */
+} psdm_rtm_geom;
+
+/* Forward on device: d_ctmp(ny,nx), d_S(nt,NumSou) -> d_rec(nt,NumRec); snapshots kept
+ * in HBM under `key`.  Launches are ordered on the context stream; the call returns
+ * once they are queued (psdm_rtm_sync waits). */
+int psdm_rtm_forward_device(psdm_rtm_ctx *ctx, const psdm_rtm_geom *g, const float *d_ctmp, const float *d_S,
+                            float *d_rec, long long key);
+/* Backward on device: d_Sadj(nt,NumRec), snapshots under `key` (released afterwards),
+ * d_kernel(ny,nx) accumulated. */
+int psdm_rtm_backward_device(psdm_rtm_ctx *ctx, const psdm_rtm_geom *g, const float *d_ctmp, const float *d_Sadj,
+                             float *d_kernel, long long key);
+int psdm_rtm_sync(psdm_rtm_ctx *ctx);
+
+/* Cross-shot stack, replaces calculate_PSDM_RTM.f90:471-497 (Cov_w = 1 on the window,
+ * embedding at x_i:x_f, PSDM_sum += / Cov_sum +=; the MPI_ALLREDUCE pair :493-494 has
+ * no counterpart on one GPU).  d_kernel(ny,nx_w) is added into d_psdm_sum(ny,nx_prop)
+ * (FP64) at columns x_i..x_i+nx_w-1 (1-based), d_cov_sum(ny,nx_prop) (int32 fold) gains
+ * 1 there, and d_kernel is cleared for the next shot.  Ordered on the context stream. */
+int psdm_rtm_stack_image(psdm_rtm_ctx *ctx, float *d_kernel, int ny, int nx_w, int x_i, double *d_psdm_sum,
+                         int *d_cov_sum, int nx_prop);
+/* normalize_psdm (auxiliar_PSDM.f90:1-20): d_psdm = d_psdm_sum / d_cov_sum where the
+ * fold is positive, else 0. */
+int psdm_rtm_normalize(psdm_rtm_ctx *ctx, const double *d_psdm_sum, const int *d_cov_sum, int ny, int nx,
+                       float *d_psdm);
+/* Bytes of device memory the context holds (wavefields, CPML strips, snapshots). */
+long long psdm_rtm_device_bytes(const psdm_rtm_ctx *ctx);
+
+/* Launch-shape and timing controls for the device form.
+ * tile_x: columns per workgroup tile (8 or 16).  use_graph: capture the time loop in
+ * hipGraphs of `graph_steps` steps.  timing != 0 records HIP events around every step
+ * kernel; psdm_rtm_step_stats returns their count and mean duration (ms). */
+int psdm_rtm_set_launch(psdm_rtm_ctx *ctx, int tile_x, int use_graph, int graph_steps);
+int psdm_rtm_set_timing(psdm_rtm_ctx *ctx, int timing);
+int psdm_rtm_step_stats(psdm_rtm_ctx *ctx, long long *n_steps, double *mean_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PSDM_RTM_H */

@@ -75,7 +75,11 @@ def ref_lib():
         disk = [_I] * 4 + [_P] * 5 + [_F, _F, _I, _I, _P, _I, C.c_char_p, _I, _I, _I, _P]
         L.ref_forward_disk.argtypes = disk
         L.ref_backward_disk.argtypes = disk
-        for f in (L.ref_forward_ram, L.ref_backward_ram, L.ref_forward_disk, L.ref_backward_disk):
+        L.ref_pml_coeffs.argtypes = [_I, _F, _F, _P, _P, _P, _P]
+        L.ref_fd_coef.argtypes = [_I, _P]
+        L.ref_deriv.argtypes = [_I, _I, _I, _P, _P]
+        for f in (L.ref_forward_ram, L.ref_backward_ram, L.ref_forward_disk, L.ref_backward_disk, L.ref_pml_coeffs,
+                  L.ref_fd_coef, L.ref_deriv):
             f.restype = None
         _ref = L
     return _ref
@@ -106,6 +110,31 @@ def pml_tables(dPML: int, dt: float, d: float, mode: int = 0, precision: str = "
     fn(mode, dPML, dt, d, _ptr(fA), _ptr(fB), _ptr(cA), _ptr(cB))
     # each (dn,dn) block was written column-major: transpose to [row, col]
     return fA, fB, cA.transpose(0, 2, 1).copy(), cB.transpose(0, 2, 1).copy()
+
+
+def ref_pml_tables(dPML: int, dt: float, d: float):
+    """PML_coeffs of the reference itself (oracle/_ref); same layout as pml_tables."""
+    dn = dPML + 6
+    fA = np.zeros(dn, np.float32)
+    fB = np.zeros(dn, np.float32)
+    cA = np.zeros((4, dn, dn), np.float32)
+    cB = np.zeros((4, dn, dn), np.float32)
+    ref_lib().ref_pml_coeffs(dPML, dt, d, _ptr(fA), _ptr(fB), _ptr(cA), _ptr(cB))
+    return fA, fB, cA.transpose(0, 2, 1).copy(), cB.transpose(0, 2, 1).copy()
+
+
+def ref_fd_coef(num: int):
+    c = np.zeros(4, np.float32)
+    ref_lib().ref_fd_coef(num, _ptr(c))
+    return c
+
+
+def ref_deriv(which: int, p: np.ndarray) -> np.ndarray:
+    """A derivative pass of the reference itself (FP32), p indexed [row, col]."""
+    pf = np.asfortranarray(p, dtype=np.float32)
+    d = np.zeros_like(pf, order="F")
+    ref_lib().ref_deriv(which, pf.shape[0], pf.shape[1], _ptr(pf), _ptr(d))
+    return d
 
 
 def fd_coef(num: int, precision: str = "f64"):

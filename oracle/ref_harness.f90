@@ -95,4 +95,48 @@ contains
                                  sid, unit_f, folder, ss, nts, nt_, kernel)
   end subroutine
 
+  ! PML_coeffs (solver_PML_coeffs.f90:1-75): fA,fB(dn), corners ld,rd,lu,ru (dn,dn) each
+  subroutine ref_pml_coeffs(dpml_in, dt, d, fA, fB, cA, cB) bind(C, name="ref_pml_coeffs")
+    integer(c_int), value :: dpml_in
+    real(c_float), value :: dt, d
+    real(c_float) :: fA(dpml_in + 6), fB(dpml_in + 6), cA(dpml_in + 6, dpml_in + 6, 4), cB(dpml_in + 6, dpml_in + 6, 4)
+    integer :: acc, dp, dn
+    real :: dtt, dd
+    acc = 3; dp = dpml_in; dn = dp + 2 * acc; dtt = dt; dd = d
+    fA = 0; fB = 0; cA = 0; cB = 0
+    call PML_coeffs(acc, dp, dtt, dd, dn, fA, fB, cA(:, :, 1), cA(:, :, 2), cA(:, :, 3), cA(:, :, 4), &
+                    cB(:, :, 1), cB(:, :, 2), cB(:, :, 3), cB(:, :, 4))
+  end subroutine
+
+  ! finite_difference_coef (solver_FD_coeffs.f90:1-73), accuracy 3
+  subroutine ref_fd_coef(num, coef) bind(C, name="ref_fd_coef")
+    integer(c_int), value :: num
+    real(c_float) :: coef(4)
+    integer :: n, acc
+    n = num; acc = 3
+    coef = 0
+    call finite_difference_coef(n, acc, coef)
+  end subroutine
+
+  ! space_deriv2x / 2y / derivx / derivy (solver_space_derivs_clean.f90:1-140), acc = 3
+  subroutine ref_deriv(which, nyt, nxt, p, d) bind(C, name="ref_deriv")
+    integer(c_int), value :: which, nyt, nxt
+    real(c_float) :: p(nyt, nxt), d(nyt, nxt)
+    real :: c1(4), c2(4)
+    integer :: acc, ny_, nx_
+    acc = 3; ny_ = nyt; nx_ = nxt
+    call finite_difference_coef(1, acc, c1)
+    call finite_difference_coef(2, acc, c2)
+    select case (which)
+    case (0)
+      call space_deriv2x(acc, c2, ny_, nx_, p, d)
+    case (1)
+      call space_deriv2y(acc, c2, ny_, nx_, p, d)
+    case (2)
+      call space_derivx(acc, c1, ny_, nx_, p, d)
+    case default
+      call space_derivy(acc, c1, ny_, nx_, p, d)
+    end select
+  end subroutine
+
 end module ref_harness

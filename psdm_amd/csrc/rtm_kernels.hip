@@ -1,0 +1,329 @@
+// gfx950 kernels of the RTM time step: 6th-order acoustic stencil, CPML strip
+// recursions, source injection, snapshot store and zero-lag imaging fused in one launch.
+//
+// Semantics follow the reference solver step by step (solver_ac_forward_ram.f90:344-577,
+// solver_ac_backward_ram.f90:329-545): derivatives with clamped edges
+// (solver_space_derivs_clean.f90), CPML memory recursions (solver_PML_matrices.f90),
+// interior update (:443-444), free surface (:463-473), snapshot (:476-500) and imaging
+// (solver_ac_backward_ram.f90:455-481).  Every product and sum is rounded separately in
+// the reference's order (built with -ffp-contract=off), so one step reproduces the
+// reference FP32 arithmetic.
+#include "rtm_core.hpp"
+
+namespace psdm {
+
+namespace {
+
+struct Cell {
+    int i, j;
+    size_t idx;  // i*ld + j
+};
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// c2 of padded cell (j,i): the reference fills the pads with the nearest edge
+// velocity of ctmp (solver_ac_forward_ram.f90:282-309), except that its _ram loops
+// fill the top and bottom pads only over the first dn interior columns
+// (`do j=1,dnx`, :288-302); the rest of those pads keeps c = 0.
+__device__ __forceinline__ float c2_at(const StepArgs &a, int j, int i, int xs, int ys) {
+    const int iw = i - a.dn;
+    if (a.ram && ys != 0 && xs == 0 && iw >= a.dn) return 0.f;
+    const int jw = clampi(j - a.dn, 0, a.ny - 1);
+    return a.c2[(size_t)clampi(iw, 0, a.nx - 1) * a.ldc + jw];
+}
+
+// Source term at an interior cell (injection :348-358; duplicates resolved on the host
+// to the last writer).
+__device__ __forceinline__ bool source_at(const StepArgs &a, int ev_s, int ev_e, int cell, float &src) {
+    bool has = false;
+    for (int e = ev_s; e < ev_e; ++e)
+        if (a.ev_cell[e] == cell) {
+            src = a.inj_val[(size_t)(a.k - 2) * a.ninj + a.ev_inj[e]];
+            has = true;
+        }
+    return has;
+}
+
+// Interior update, free surface, snapshot / imaging for an interior cell.
+__device__ __forceinline__ void interior_update(const StepArgs &a, const Cell &c, float p, float p1, float d2x,
+                                                float d2y, float c2, int ev_s, int ev_e) {
+    float lap = d2x + d2y;
+    float src;
+    if (ev_s < ev_e && source_at(a, ev_s, ev_e, c.j + c.i * a.nyt, src)) lap = lap + src;
+    float v = (-p1 + 2.f * p) + lap * c2;
+    if (a.fs && c.j == a.dn) v = 0.f;
+    a.pn[c.idx] = v;
+    if (a.snap_slot >= 0) {
+        const size_t o = (size_t)(c.i - a.dn) * a.ny + (c.j - a.dn);
+        const size_t so = (size_t)a.snap_slot * a.snap_stride + o;
+        if (a.mode == 0) {
+            a.snaps[so] = ((v - p1) * 0.5f) / a.dt;
+        } else {
+            a.image[o] = a.image[o] + ((0.5f * (v - p1)) * a.snaps[so]) * a.ss_f;
+        }
+    }
+}
+
+template <int TX>
+struct Tile {
+    static constexpr int HY = kTileY + 6, HX = TX + 6;
+};
+
+}  // namespace
+
+template <int TX>
+__global__ __launch_bounds__(kBlock) void rtm_step_kernel(StepArgs a) {
+    constexpr int TY = kTileY, HY = Tile<TX>::HY, HX = Tile<TX>::HX, CPW = TX / 4;
+    __shared__ float tile[HX * HY];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int b = blockIdx.x;
+
+    // traces of the previous step (:567-571), read from p(k-1) before it is overwritten
+    if (a.rec_row >= 0)
+        for (int r = b * kBlock + threadIdx.x; r < a.nrec; r += gridDim.x * kBlock) {
+            const int c = a.rec_cell[r];
+            a.rec[(size_t)r * a.rec_ld + a.rec_row] = c >= 0 ? a.pc[c] : 0.f;
+        }
+
+    // XCD-aware order: the 8 XCDs take blocks round-robin, so XCD x gets the
+    // contiguous tile range [x*per_xcd, (x+1)*per_xcd) and shares halos in its own L2.
+    const int t = (b & 7) * a.per_xcd + (b >> 3);
+    if (t >= a.ntiles) return;
+    const int tj = t % a.ntile_y, ti = t / a.ntile_y;
+    const int j0 = tj * TY, i0 = ti * TX;
+
+    // stage p(k-1) rows j0-3..j0+66, columns i0-3..i0+TX+2 (zero outside the grid)
+    for (int li = w; li < HX; li += 4) {
+        const int gi = i0 - 3 + li;
+        const bool colok = gi >= 0 && gi < a.nxt;
+        const float *col = a.pc + (size_t)(colok ? gi : 0) * a.ld;
+        int gj = j0 - 3 + lane;
+        tile[li * HY + lane] = (colok && gj >= 0 && gj < a.nyt) ? col[gj] : 0.f;
+        if (lane < 6) {
+            gj = j0 + 61 + lane;
+            tile[li * HY + 64 + lane] = (colok && gj < a.nyt) ? col[gj] : 0.f;
+        }
+    }
+    __syncthreads();
+
+    const int j = j0 + lane;
+    if (j >= a.nyt) return;
+    const int ev_s = a.ev_ptr[t], ev_e = a.ev_ptr[t + 1];
+    const int dn = a.dn;
+    const bool interior_tile = j0 >= dn && j0 + TY <= dn + a.ny && i0 >= dn && i0 + TX <= dn + a.nx;
+
+#pragma unroll
+    for (int q = 0; q < CPW; ++q) {
+        const int i = i0 + w * CPW + q;
+        if (i >= a.nxt) break;
+        const Cell c{i, j, (size_t)i * a.ld + j};
+        const float *T = tile + (i - i0 + 3) * HY + (lane + 3);  // T[di*HY + dj]
+        if (interior_tile) {
+            const float p = T[0];
+            float d2x = (p + p) * kC2_0;
+            d2x = d2x + (T[HY] + T[-HY]) * kC2_1;
+            d2x = d2x + (T[2 * HY] + T[-2 * HY]) * kC2_2;
+            d2x = d2x + (T[3 * HY] + T[-3 * HY]) * kC2_3;
+            float d2y = (p + p) * kC2_0;
+            d2y = d2y + (T[1] + T[-1]) * kC2_1;
+            d2y = d2y + (T[2] + T[-2]) * kC2_2;
+            d2y = d2y + (T[3] + T[-3]) * kC2_3;
+            const float p1 = a.pn[c.idx];
+            const float c2 = a.c2[(size_t)(i - dn) * a.ldc + (j - dn)];
+            interior_update(a, c, p, p1, d2x, d2y, c2, ev_s, ev_e);
+            continue;
+        }
+
+        // ---- general cell: CPML strips, corners, clamped derivative edges ----
+        const int xs = i < dn ? 1 : (i >= dn + a.nx ? 2 : 0);
+        const int ys = j < dn ? 1 : (j >= dn + a.ny ? 2 : 0);
+        if (a.fs && ys == 1) {  // with a free surface the top CPML is never updated (:391)
+            a.pn[c.idx] = 0.f;
+            continue;
+        }
+        auto P = [&](int gj, int gi) -> float {
+            const int lj = gj - j0 + 3, li = gi - i0 + 3;
+            if (lj >= 0 && lj < HY && li >= 0 && li < HX) return tile[li * HY + lj];
+            return a.pc[(size_t)gi * a.ld + gj];
+        };
+        // derivatives at the clamped column/row (solver_space_derivs_clean.f90:24-34)
+        const int ic = clampi(i, 3, a.nxt - 4), jc = clampi(j, 3, a.nyt - 4);
+        const float px = P(j, ic), py = P(jc, i);
+        const float xm1 = P(j, ic - 1), xp1 = P(j, ic + 1), xm2 = P(j, ic - 2), xp2 = P(j, ic + 2);
+        const float xm3 = P(j, ic - 3), xp3 = P(j, ic + 3);
+        const float ym1 = P(jc - 1, i), yp1 = P(jc + 1, i), ym2 = P(jc - 2, i), yp2 = P(jc + 2, i);
+        const float ym3 = P(jc - 3, i), yp3 = P(jc + 3, i);
+        float d2x = (px + px) * kC2_0;
+        d2x = d2x + (xp1 + xm1) * kC2_1;
+        d2x = d2x + (xp2 + xm2) * kC2_2;
+        d2x = d2x + (xp3 + xm3) * kC2_3;
+        float d2y = (py + py) * kC2_0;
+        d2y = d2y + (yp1 + ym1) * kC2_1;
+        d2y = d2y + (yp2 + ym2) * kC2_2;
+        d2y = d2y + (yp3 + ym3) * kC2_3;
+        const float p = T[0];
+        const float p1 = a.pn[c.idx];
+        const float c2 = c2_at(a, j, i, xs, ys);
+        if (xs == 0 && ys == 0) {
+            interior_update(a, c, p, p1, d2x, d2y, c2, ev_s, ev_e);
+            continue;
+        }
+        const size_t ld = (size_t)a.ld;
+        const int dn2 = 2 * dn;
+        // x-direction memory (strips xl/xr and the corners), pml_matrices_x/_xycorner(1)
+        float propx = 0.f;
+        int distx = 0;
+        if (xs != 0) {
+            const int il = xs == 1 ? i : i - (dn + a.nx);  // column inside the strip
+            distx = xs == 1 ? dn - i : il + 1;             // 1 next to the interior
+        }
+        int disty = 0;
+        if (ys != 0) {
+            const int jl = ys == 1 ? j : j - (dn + a.ny);
+            disty = ys == 1 ? dn - j : jl + 1;
+        }
+        // corners use F(max(dist_x, dist_y)) for both directions (solver_PML_coeffs.f90:288-322)
+        const int dist = distx > disty ? distx : disty;
+        const float A = a.fA[dist - 1], B = a.fB[dist - 1];
+        if (xs != 0) {
+            const int il = xs == 1 ? i : i - (dn + a.nx);
+            const int xc = xs == 1 ? i : i - a.nx;  // column of the x memory arrays
+            const int xcc = xc - il + clampi(il, 3, dn - 4);
+            const float *ph = a.phx_o + j;
+            float dphi = (ph[(xcc + 1) * ld] - ph[(xcc - 1) * ld]) * kC1_1;
+            dphi = dphi + (ph[(xcc + 2) * ld] - ph[(xcc - 2) * ld]) * kC1_2;
+            dphi = dphi + (ph[(xcc + 3) * ld] - ph[(xcc - 3) * ld]) * kC1_3;
+            float d1x = (xp1 - xm1) * kC1_1;
+            d1x = d1x + (xp2 - xm2) * kC1_2;
+            d1x = d1x + (xp3 - xm3) * kC1_3;
+            const size_t xo = (size_t)xc * ld + j;
+            const float phn = A * a.phx_o[xo] + B * d1x;
+            const float chn = A * a.chx[xo] + B * (d2x + dphi);
+            a.phx_n[xo] = phn;
+            a.chx[xo] = chn;
+            propx = dphi + chn;
+        }
+        float propy = 0.f;
+        if (ys != 0) {
+            const int jl = ys == 1 ? j : j - (dn + a.ny);
+            const int yr = ys == 1 ? j : j - a.ny;  // row of the y memory arrays
+            const int yrc = yr - jl + clampi(jl, 3, dn - 4);
+            const float *ph = a.phy_o + (size_t)i * dn2;
+            float dphi = (ph[yrc + 1] - ph[yrc - 1]) * kC1_1;
+            dphi = dphi + (ph[yrc + 2] - ph[yrc - 2]) * kC1_2;
+            dphi = dphi + (ph[yrc + 3] - ph[yrc - 3]) * kC1_3;
+            float d1y = (yp1 - ym1) * kC1_1;
+            d1y = d1y + (yp2 - ym2) * kC1_2;
+            d1y = d1y + (yp3 - ym3) * kC1_3;
+            const size_t yo = (size_t)i * dn2 + yr;
+            const float phn = A * a.phy_o[yo] + B * d1y;
+            const float chn = A * a.chy[yo] + B * (d2y + dphi);
+            a.phy_n[yo] = phn;
+            a.chy[yo] = chn;
+            propy = dphi + chn;
+        }
+        float v;
+        if (ys == 0) {
+            // x strip, interior rows (:376-382).  With a free surface the _ram form
+            // zeroes row dn+1 of the grid but the strip keeps its own history there.
+            const bool own = a.ram && a.fs && j == dn;
+            const int hsi = xs == 1 ? i : i - a.nx;
+            const float ps = own ? a.hs_c[hsi] : p;
+            const float p1s = own ? a.hs_n[hsi] : p1;
+            v = (-p1s + 2.f * ps) + c2 * ((d2x + d2y) + propx);
+            if (own) {
+                a.hs_n[hsi] = v;
+                v = 0.f;
+            }
+        } else if (xs == 0) {
+            v = (-p1 + 2.f * p) + c2 * ((d2x + d2y) + propy);  // y strip (:388-396)
+        } else {
+            v = (-p1 + 2.f * p) + c2 * ((d2x + d2y) + propx);  // corner x call (:402-403)
+            v = v + c2 * propy;                                 // corner y call (:407)
+        }
+        a.pn[c.idx] = v;
+    }
+}
+
+__global__ void rtm_record_kernel(const float *p, const int *rec_cell, float *rec, int rec_ld, int row, int nrec) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < nrec) {
+        const int c = rec_cell[r];
+        rec[(size_t)r * rec_ld + row] = c >= 0 ? p[c] : 0.f;
+    }
+}
+
+// c2 = c**2*dt**2/dmodel**2 (solver_ac_forward_ram.f90:319), FP32 in the same order.
+__global__ void rtm_c2_kernel(const float *vel, float *c2, int ny, int nx, int ldc, float dt, float dx) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y;
+    if (j < ny && i < nx) {
+        const float c = vel[(size_t)i * ny + j];
+        c2[(size_t)i * ldc + j] = ((c * c) * (dt * dt)) / (dx * dx);
+    }
+}
+
+// out[r*ncols + e] = traces[cols[e]*nt + r], r in [0, nt-1): the value injected at step r+2
+__global__ void rtm_gather_kernel(const float *traces, int nt, const int *cols, int ncols, float *out) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.y;
+    if (e < ncols && r < nt - 1) out[(size_t)r * ncols + e] = traces[(size_t)cols[e] * nt + r];
+}
+
+// Cross-shot stack (calculate_PSDM_RTM.f90:471-497): the shot image kernel(ny,nx_w)
+// is embedded at columns x0..x0+nx_w-1 of PSDM_sum (FP64) and the fold Cov_sum gains 1
+// there (Cov_w = 1 on the window).  The shot image is cleared for the next shot
+// (the driver's PSDM_w = 0, :384).
+__global__ void rtm_stack_kernel(float *img, int ny, int nx_w, int x0, double *sum, int ld_sum, int *fold) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y;
+    if (j < ny && i < nx_w) {
+        const size_t o = (size_t)i * ny + j;
+        const size_t g = (size_t)(x0 + i) * ld_sum + j;
+        sum[g] += (double)img[o];
+        fold[g] += 1;
+        img[o] = 0.f;
+    }
+}
+
+// normalize_psdm (auxiliar_PSDM.f90:1-20): PSDM = PSDM_sum / Cov_sum where the fold is
+// positive, 0 elsewhere.
+__global__ void rtm_normalize_kernel(const double *sum, const int *fold, float *out, size_t n) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n) out[e] = fold[e] > 0 ? (float)(sum[e] / (double)fold[e]) : 0.f;
+}
+
+void launch_stack(float *img, int ny, int nx_w, int x0, double *sum, int ld_sum, int *fold, hipStream_t s) {
+    hipLaunchKernelGGL(rtm_stack_kernel, dim3((ny + 255) / 256, nx_w), dim3(256), 0, s, img, ny, nx_w, x0, sum,
+                       ld_sum, fold);
+}
+
+void launch_normalize(const double *sum, const int *fold, float *out, size_t n, hipStream_t s) {
+    hipLaunchKernelGGL(rtm_normalize_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, sum, fold, out, n);
+}
+
+void launch_step(const StepArgs &a, int tile_x, hipStream_t s) {
+    const int grid = 8 * a.per_xcd;
+    if (tile_x == 16)
+        hipLaunchKernelGGL(rtm_step_kernel<16>, dim3(grid), dim3(kBlock), 0, s, a);
+    else
+        hipLaunchKernelGGL(rtm_step_kernel<8>, dim3(grid), dim3(kBlock), 0, s, a);
+}
+
+void launch_record(const float *p, const int *rec_cell, float *rec, int rec_ld, int row, int nrec, hipStream_t s) {
+    if (nrec <= 0) return;
+    hipLaunchKernelGGL(rtm_record_kernel, dim3((nrec + 255) / 256), dim3(256), 0, s, p, rec_cell, rec, rec_ld, row,
+                       nrec);
+}
+
+void launch_c2(const float *vel, float *c2, int ny, int nx, int ldc, float dt, float dx, hipStream_t s) {
+    hipLaunchKernelGGL(rtm_c2_kernel, dim3((ny + 255) / 256, nx), dim3(256), 0, s, vel, c2, ny, nx, ldc, dt, dx);
+}
+
+void launch_gather(const float *traces, int nt, const int *cols, int ncols, float *out, hipStream_t s) {
+    if (ncols <= 0 || nt < 2) return;
+    hipLaunchKernelGGL(rtm_gather_kernel, dim3((ncols + 63) / 64, nt - 1), dim3(64), 0, s, traces, nt, cols, ncols,
+                       out);
+}
+
+}  // namespace psdm
