@@ -1,0 +1,239 @@
+"""RTM line migration benchmark on one MI355X.
+
+Workload (default): the 2001 x 601 @ 10 m marine line -- 64 shots at 25 m spacing,
+480 channels at 12.5 m from 150 m offset, dt 0.8 ms, nt 6000, CPML 40 cells, each shot
+migrated in its 715-column aperture window (forward with FP32 snapshots every step in
+HBM, back-propagation of the time-reversed gather, zero-lag imaging, FP64 cross-shot
+stack).  One "step" = one migrated shot; the timed region migrates K shots with the
+velocity model, the gathers and the image already resident in HBM.
+
+Prints one JSON line: shots/s (value), Gpt-updates/s, the step kernel's HBM roofline
+(algorithmic bytes / mean kernel duration from HIP events on the library stream) and
+the CPU baseline (reference Fortran solvers, N processes) on a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_leg(ny, nx_w, dPML, dx, dt, nprocs, nt_sample, nt_full):
+    from oracle import cpu_baseline
+
+    r = cpu_baseline.run(ny, nx_w, nt_sample, dPML, dx, dt, nprocs)
+    dn = dPML + 6
+    upd_shot = 2 * (ny + 2 * dn) * (nx_w + 2 * dn) * (nt_full - 1)
+    return {
+        "value": round(r["gupd_per_s"], 4),
+        "unit": "Gpt-updates/s",
+        "shots_per_s": round(r["gupd_per_s"] * 1e9 / upd_shot, 6),
+        "cores": r["cores"],
+        "kind": r["kind"],
+        "sample": (f"{r['cores']} independent processes, each forward+backward of one {ny}x{nx_w} window "
+                   f"(dPML {dPML}) over {nt_sample} of the {nt_full} steps; rate = updates / slowest rank time "
+                   f"({r['rank_s']:.1f} s)"),
+    }
+
+
+def accuracy_check(ctx):
+    """Single-shot 401 x 201 @ 12.5 m workload cut to 600 steps: GPU image and traces vs
+    the FP64 oracle (and the FP32 oracle, which the GPU reproduces bit for bit)."""
+    import numpy as np
+
+    from oracle import pyoracle as O
+    from psdm_amd import solvers as S
+    from psdm_amd.workloads import line_workload
+    from tests.cases import rel_l2
+
+    W = line_workload("single401")
+    m, sh = W["model"], W["shots"][0]
+    nt, dt, dPML = 600, W["dt"], W["dPML"]
+    ctmp = np.asfortranarray(m.migration[:, sh.x_i - 1:sh.x_f])
+    ctrue = np.asfortranarray(m.true[:, sh.x_i - 1:sh.x_f])
+    src = S.ricker_mcs(nt, dt, W["f0"], 1.2 / W["f0"]).reshape(nt, 1)
+    nxS, nyS = [sh.nxSou - sh.x_i + 1], [sh.nySou]
+    nxR, nyR = sh.nxRec - sh.x_i + 1, sh.nyRec
+    ny, nx = ctmp.shape
+    par = S.ParFile(dPML=dPML, fs=0)
+    NumRec = nxR.size
+    d, _ = O.forward(nxS, nyS, nxR, nyR, src, m.dx, dt, ctrue, 1, nt, nt, dPML, 0)
+    Sadj = np.asfortranarray(d[::-1, :])
+    Rec = np.zeros((nt, NumRec), np.float32, order="F")
+    F = np.zeros((ny, nx, nt), np.float32, order="F")
+    k = np.zeros((ny, nx), np.float32, order="F")
+    S.solver_ac_forward_ram(0, 1, NumRec, nxS, nyS, nxR, nyR, src, m.dx, dt, ny, nx, ctmp, 1, nt, nt, F, Rec,
+                            par=par, host_fields=False, ctx=ctx)
+    S.solver_ac_backward_ram(0, 1, NumRec, nxS, nyS, nxR, nyR, Sadj, m.dx, dt, ny, nx, ctmp, 1, nt, nt, F, k,
+                             par=par, host_fields=False, ctx=ctx)
+    R64, F64 = O.forward(nxS, nyS, nxR, nyR, src, m.dx, dt, ctmp, 1, nt, nt, dPML, 0, precision="f64")
+    k64 = O.backward(nxS, nyS, nxR, nyR, Sadj, m.dx, dt, ctmp, 1, nt, nt, dPML, 0, Fields=F64, precision="f64")
+    return {"workload": f"single401 shot, {ny}x{nx} window, {nt} steps", "image_rel_l2_vs_fp64": rel_l2(k, k64),
+            "traces_rel_l2_vs_fp64": rel_l2(Rec, R64)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=64, help="shots timed")
+    ap.add_argument("--warmup", type=int, default=1, help="untimed shots before the timed region")
+    ap.add_argument("--tile-x", type=int, default=8)
+    ap.add_argument("--timing-every", type=int, default=16, help="sample every n-th step kernel with HIP events")
+    ap.add_argument("--cpu-procs", type=int, default=16)
+    ap.add_argument("--cpu-steps", type=int, default=120)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-accuracy", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build runs the RTM path on a single MI355X "
+              f"(--gpus {args.gpus} is not supported; shots shard across GPUs with no per-step "
+              f"communication, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from psdm_amd.workloads import line_workload, step_bytes, updates_per_shot
+
+    W = line_workload("mcs2001")
+    m, shots, nt, dt, dPML = W["model"], W["shots"], W["nt"], W["dt"], W["dPML"]
+    ny = m.nz
+    nxw = shots[0].nx_w
+
+    cpu = None
+    if not args.no_cpu:
+        log(f"[bench] cpu baseline: {args.cpu_procs} procs x {args.cpu_steps} steps")
+        cpu = cpu_leg(ny, nxw, dPML, m.dx, dt, args.cpu_procs, args.cpu_steps, nt)
+        log(f"[bench] cpu baseline {cpu['value']} Gupd/s ({cpu['kind']})")
+
+    import numpy as np
+    import torch
+
+    from psdm_amd._native import Context
+    from psdm_amd.solvers import DeviceShot, backward_device, forward_device, ricker_mcs
+
+    dev = torch.device("cuda:0")
+    ctx = Context(0)
+    ctx.set_launch(tile_x=args.tile_x)
+    d_true = torch.from_numpy(np.ascontiguousarray(m.true.T)).to(dev)        # (nx, ny) = Fortran (ny, nx)
+    d_mig = torch.from_numpy(np.ascontiguousarray(m.migration.T)).to(dev)
+    d_wat = torch.from_numpy(np.ascontiguousarray(m.water.T)).to(dev)
+    src = ricker_mcs(nt, dt, W["f0"], 1.2 / W["f0"])
+    d_src = torch.from_numpy(src.reshape(1, nt)).to(dev)
+    nrec = shots[0].nxRec.size
+    torch.cuda.synchronize()
+
+    nshots = max(args.steps + args.warmup, 1)
+    geos, geos_model = [], []
+    for k in range(nshots):
+        sh = shots[k % len(shots)]
+        args_geo = (ny, sh.nx_w, nt, dPML, W["fs"], m.dx, dt, [sh.nxSou - sh.x_i + 1], [sh.nySou],
+                    sh.nxRec - sh.x_i + 1, sh.nyRec)
+        geos.append(DeviceShot(*args_geo))               # migration: snapshots every step
+        geos_model.append(DeviceShot(*args_geo, nt_snap=0))  # data modelling: no snapshots
+
+    # observed gathers: model on the true and the water-only model, subtract (direct wave
+    # removed), reverse in time (BackSource, calculate_PSDM_RTM.f90:448-450)
+    log(f"[bench] modelling {min(nshots, len(shots))} gathers")
+    gathers = {}
+    d_a = torch.empty((nrec, nt), device=dev)
+    d_b = torch.empty((nrec, nt), device=dev)
+    for k in range(min(nshots, len(shots))):
+        sh = shots[k]
+        forward_device(ctx, geos_model[k], d_true[sh.x_i - 1:sh.x_f], d_src, d_a, key=-1)
+        forward_device(ctx, geos_model[k], d_wat[sh.x_i - 1:sh.x_f], d_src, d_b, key=-1)
+        ctx.sync()
+        gathers[k] = (d_a - d_b).flip(1).contiguous()
+        torch.cuda.synchronize()
+
+    d_kernel = torch.zeros((max(s.nx_w for s in shots), ny), device=dev)
+    psdm_sum = torch.zeros((m.nx, ny), dtype=torch.float64, device=dev)
+    fold = torch.zeros((m.nx, ny), dtype=torch.int32, device=dev)
+    d_rec = torch.empty((nrec, nt), device=dev)
+    torch.cuda.synchronize()
+
+    def migrate(k):
+        sh = shots[k % len(shots)]
+        win = d_mig[sh.x_i - 1:sh.x_f]
+        forward_device(ctx, geos[k], win, d_src, d_rec, key=k)
+        backward_device(ctx, geos[k], win, gathers[k % len(shots)], d_kernel, key=k)
+        ctx.check(ctx._L.psdm_rtm_stack_image(ctx.h, d_kernel.data_ptr(), ny, sh.nx_w, sh.x_i,
+                                              psdm_sum.data_ptr(), fold.data_ptr(), m.nx), "stack_image")
+
+    log(f"[bench] warmup {args.warmup} shot(s)")
+    for k in range(args.warmup):
+        migrate(k)
+    ctx.sync()
+    torch.cuda.synchronize()
+    psdm_sum.zero_()
+    fold.zero_()
+    torch.cuda.synchronize()
+
+    ctx.set_timing(args.timing_every)
+    log(f"[bench] timing {args.steps} shots")
+    t0 = time.perf_counter()
+    for k in range(args.warmup, args.warmup + args.steps):
+        migrate(k)
+    ctx.sync()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    n_ev, mean_ms = ctx.step_stats()
+    elapsed = t1 - t0
+
+    upd = sum(updates_per_shot(ny, shots[k % len(shots)].nx_w, nt, dPML)
+              for k in range(args.warmup, args.warmup + args.steps))
+    shots_per_s = args.steps / elapsed
+    gupd = upd / elapsed / 1e9
+    bfw = step_bytes(ny, nxw, dPML, 4.0, 1, backward=False)
+    bbw = step_bytes(ny, nxw, dPML, 4.0, 1, backward=True)
+    step_b = 0.5 * (bfw + bbw)
+    achieved = step_b / (mean_ms * 1e-3) / 1e9 if mean_ms > 0 else None
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "step_kernel_mcs2001_traffic.json")
+    if os.path.exists(tpath):
+        traffic = json.load(open(tpath)).get("bytes_per_launch")
+
+    acc = None
+    if not args.no_accuracy:
+        log("[bench] accuracy check vs FP64 oracle")
+        acc = accuracy_check(ctx)
+
+    out = {
+        "metric": "migrated shots/s + Gpt-updates/s, batched vs plain, at rel-L2<=1e-4; p99 latency",
+        "value": round(shots_per_s, 4),
+        "unit": "shots/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * elapsed / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded 2001x601 model; gathers modelled on device, direct wave removed)",
+        "config": {"workload": "mcs2001 line: 2001x601 @10 m, 64 shots @25 m, 480 ch @12.5 m, near offset 150 m, "
+                               "dt 0.8 ms, nt 6000, dPML 40, plain one-shot-per-call form, FP32 snapshots every step",
+                   "form": "plain", "window": f"{ny}x{nxw}", "tile_x": args.tile_x},
+        "gpt_updates_per_s": round(gupd, 3),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1) if achieved else None, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
+                     "traffic": traffic, "kernel": "rtm_step_kernel", "mean_kernel_ms": round(mean_ms, 5),
+                     "sampled_launches": n_ev, "algorithmic_bytes_per_launch": round(step_b)},
+        "cpu_baseline": cpu,
+        "accuracy": acc,
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,54 @@
+"""CPU baseline timing for bench.py -- TEST INFRASTRUCTURE ONLY (the checker's side).
+
+Times the reference Fortran solvers (oracle/_ref/libpsdm_ref.so, kind "reference") or,
+where that library was not built, the C restatement (oracle/liboracle.so, kind "port")
+on a bounded sample of a shot: forward + backward over `nt_sample` time steps of the
+shot's full padded window.  N independent processes each migrate their own copy, the
+way `mpirun -np N psdm_run` gives each rank its own shot (calculate_PSDM_RTM.f90:151-168).
+"""
+from __future__ import annotations
+
+import multiprocessing as mp
+import os
+import time
+
+import numpy as np
+
+
+def _one_rank(args):
+    ny, nx, nt, dPML, dx, dt, kind, seed = args
+    import sys
+
+    here = os.path.dirname(os.path.abspath(__file__))
+    sys.path.insert(0, os.path.dirname(here))
+    from oracle import pyoracle as O
+
+    rng = np.random.default_rng(seed)
+    ctmp = np.asfortranarray((1500 + 2000 * rng.random((ny, nx))).astype(np.float32))
+    t = np.arange(nt) * dt
+    a = (np.pi * 15.0 * (t - 0.08)) ** 2
+    S = ((1 - 2 * a) * np.exp(-a)).astype(np.float32).reshape(nt, 1)
+    nxR = np.arange(10, nx - 10, 2, dtype=np.int32)
+    nyR = np.ones_like(nxR)
+    impl = "ref" if kind == "reference" else "oracle"
+    t0 = time.perf_counter()
+    Rec, Fields = O.forward([nx // 2], [1], nxR, nyR, S, dx, dt, ctmp, 1, nt, nt, dPML, 0, impl=impl)
+    Sadj = np.asfortranarray(Rec[::-1, :])
+    O.backward([nx // 2], [1], nxR, nyR, Sadj, dx, dt, ctmp, 1, nt, nt, dPML, 0, Fields=Fields, impl=impl)
+    return time.perf_counter() - t0
+
+
+def run(ny: int, nx: int, nt_sample: int, dPML: int, dx: float, dt: float, nprocs: int):
+    from oracle import pyoracle as O
+
+    kind = "reference" if O.ref_available() else "port"
+    dn = dPML + 6
+    updates = 2 * (ny + 2 * dn) * (nx + 2 * dn) * (nt_sample - 1)
+    ctx = mp.get_context("spawn")
+    t0 = time.perf_counter()
+    with ctx.Pool(nprocs) as pool:
+        per = pool.map(_one_rank, [(ny, nx, nt_sample, dPML, dx, dt, kind, s) for s in range(nprocs)])
+    wall = time.perf_counter() - t0
+    busy = max(per)
+    return dict(kind=kind, cores=nprocs, updates_per_rank=updates, wall_s=wall, rank_s=busy,
+                gupd_per_s=nprocs * updates / busy / 1e9)

@@ -1,0 +1,172 @@
+! Fortran side of the drop-in: the four solver subroutines of PSDM's RTM path with the
+! reference's own names and argument lists, forwarding to libpsdm_rtm.so through
+! iso_c_binding (include/psdm_rtm.h).  Linking this file (with -lpsdm_rtm) instead of
+! solver_ac_forward_ram.f90, solver_ac_backward_ram.f90, solver_ac_forward_disk.f90
+! and solver_ac_backward_disk.f90 leaves their callers (dynamic_window.f90:90-92,
+! 134-137, 177-179, 221-224) and calculate_PSDM_RTM.f90 unchanged.
+!
+! Module inputs are read exactly where the reference reads them: dPML and fs from
+! mod_parfile (solver_ac_forward_ram.f90:8); taper_option decides whether the forward
+! snapshots must be materialised in the caller's Fields (the driver tapers them in
+! place, calculate_PSDM_RTM.f90:421-427) or can stay in HBM until the backward call.
+! Errors keep the reference's fail-fast behaviour: a message, then error stop.
+
+module psdm_rtm_iface
+  use iso_c_binding
+  implicit none
+  type(c_ptr), save :: psdm_ctx = c_null_ptr
+
+  interface
+    integer(c_int) function psdm_rtm_create(ctx, device) bind(C, name="psdm_rtm_create")
+      import :: c_ptr, c_int
+      type(c_ptr) :: ctx
+      integer(c_int), value :: device
+    end function
+    type(c_ptr) function psdm_rtm_last_error(ctx) bind(C, name="psdm_rtm_last_error")
+      import :: c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function psdm_rtm_forward_ram(ctx, dPML, fs, NumSou, NumRec, nxSou, nySou, nxRec, nyRec, S, &
+        dmodel, dt, ny, nx, ctmp, store_snap, nt_snap, nt, Fields, host_fields, Rec_point) &
+        bind(C, name="psdm_rtm_forward_ram")
+      import :: c_ptr, c_int, c_float
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: dPML, fs, NumSou, NumRec, ny, nx, store_snap, nt_snap, nt, host_fields
+      integer(c_int) :: nxSou(*), nySou(*), nxRec(*), nyRec(*)
+      real(c_float), value :: dmodel, dt
+      real(c_float) :: S(*), ctmp(*), Fields(*), Rec_point(*)
+    end function
+    integer(c_int) function psdm_rtm_backward_ram(ctx, dPML, fs, NumSou, NumRec, nxSou, nySou, nxRec, nyRec, Sadj, &
+        dmodel, dt, ny, nx, ctmp, store_snap, nt_snap, nt, Fields, host_fields, kernel) &
+        bind(C, name="psdm_rtm_backward_ram")
+      import :: c_ptr, c_int, c_float
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: dPML, fs, NumSou, NumRec, ny, nx, store_snap, nt_snap, nt, host_fields
+      integer(c_int) :: nxSou(*), nySou(*), nxRec(*), nyRec(*)
+      real(c_float), value :: dmodel, dt
+      real(c_float) :: Sadj(*), ctmp(*), Fields(*), kernel(*)
+    end function
+    integer(c_int) function psdm_rtm_forward_disk(ctx, dPML, fs, NumSou, NumRec, nxSou, nySou, nxRec, nyRec, S, &
+        dmodel, dt, ny, nx, ctmp, shotid, store_snap, nt_snap, nt, Rec_point) bind(C, name="psdm_rtm_forward_disk")
+      import :: c_ptr, c_int, c_float
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: dPML, fs, NumSou, NumRec, ny, nx, shotid, store_snap, nt_snap, nt
+      integer(c_int) :: nxSou(*), nySou(*), nxRec(*), nyRec(*)
+      real(c_float), value :: dmodel, dt
+      real(c_float) :: S(*), ctmp(*), Rec_point(*)
+    end function
+    integer(c_int) function psdm_rtm_backward_disk(ctx, dPML, fs, NumSou, NumRec, nxSou, nySou, nxRec, nyRec, Sadj, &
+        dmodel, dt, ny, nx, ctmp, shotid, store_snap, nt_snap, nt, kernel) bind(C, name="psdm_rtm_backward_disk")
+      import :: c_ptr, c_int, c_float
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: dPML, fs, NumSou, NumRec, ny, nx, shotid, store_snap, nt_snap, nt
+      integer(c_int) :: nxSou(*), nySou(*), nxRec(*), nyRec(*)
+      real(c_float), value :: dmodel, dt
+      real(c_float) :: Sadj(*), ctmp(*), kernel(*)
+    end function
+  end interface
+
+contains
+
+  ! Context of this process (one rank per GPU: the launcher selects the device through
+  ! HIP_VISIBLE_DEVICES, so the library always opens device 0).
+  type(c_ptr) function psdm_context()
+    integer(c_int) :: rc
+    if (.not. c_associated(psdm_ctx)) then
+      rc = psdm_rtm_create(psdm_ctx, 0_c_int)
+      if (rc /= 0) then
+        write(*,*) 'psdm_rtm: no usable HIP device (psdm_rtm_create returned ', rc, ')'
+        error stop
+      endif
+    endif
+    psdm_context = psdm_ctx
+  end function
+
+  subroutine psdm_check(rc, where)
+    integer(c_int), intent(in) :: rc
+    character(len=*), intent(in) :: where
+    character(kind=c_char), pointer :: msg(:)
+    integer :: n
+    if (rc == 0) return
+    call c_f_pointer(psdm_rtm_last_error(psdm_ctx), msg, [512])
+    n = 0
+    do while (n < 512)
+      if (msg(n + 1) == c_null_char) exit
+      n = n + 1
+    enddo
+    write(*,*) 'psdm_rtm: ', where, ' failed (', rc, '): ', msg(1:n)
+    error stop
+  end subroutine
+
+end module psdm_rtm_iface
+
+
+subroutine solver_ac_forward_ram(rank,NumSou,NumRec,nxSou,nySou,nxRec,nyRec,&
+S,dmodel,dt,ny,nx,ctmp,&
+store_snap,nt_snap,nt,Fields,Rec_point)
+  use mod_parfile, only: dPML, fs, taper_option
+  use psdm_rtm_iface
+  implicit none
+  integer :: rank, NumSou, NumRec, ny, nx, store_snap, nt_snap, nt
+  integer :: nxSou(NumSou), nySou(NumSou), nxRec(NumRec), nyRec(NumRec)
+  real :: dmodel, dt
+  real :: S(nt,NumSou), ctmp(ny,nx), Fields(ny,nx,nt_snap), Rec_point(nt,NumRec)
+  integer(c_int) :: host
+  host = 0
+  if (taper_option /= 0) host = 1
+  call psdm_check(psdm_rtm_forward_ram(psdm_context(), dPML, fs, NumSou, NumRec, nxSou, nySou, nxRec, nyRec, S, &
+                  dmodel, dt, ny, nx, ctmp, store_snap, nt_snap, nt, Fields, host, Rec_point), 'solver_ac_forward_ram')
+end subroutine
+
+
+subroutine solver_ac_backward_ram(rank,NumSou,NumRec,nxSou,nySou,nxRec,nyRec,&
+Sadj,dmodel,dt,ny,nx,ctmp,&
+store_snap,nt_snap,nt,Fields,kernel)
+  use mod_parfile, only: dPML, fs, taper_option
+  use psdm_rtm_iface
+  implicit none
+  integer :: rank, NumSou, NumRec, ny, nx, store_snap, nt_snap, nt
+  integer :: nxSou(NumSou), nySou(NumSou), nxRec(NumRec), nyRec(NumRec)
+  real :: dmodel, dt
+  real :: Sadj(nt,NumRec), ctmp(ny,nx), Fields(ny,nx,nt_snap), kernel(ny,nx)
+  integer(c_int) :: host
+  host = 0
+  if (taper_option /= 0) host = 1
+  call psdm_check(psdm_rtm_backward_ram(psdm_context(), dPML, fs, NumSou, NumRec, nxSou, nySou, nxRec, nyRec, Sadj, &
+                  dmodel, dt, ny, nx, ctmp, store_snap, nt_snap, nt, Fields, host, kernel), 'solver_ac_backward_ram')
+end subroutine
+
+
+subroutine solver_ac_forward_disk(rank,NumSou,NumRec,nxSou,nySou,nxRec,nyRec,&
+S,dmodel,dt,ny,nx,ctmp,&
+shotid,unit_fields,folder_fields,&
+store_snap,nt_snap,nt,Rec_point)
+  use mod_parfile, only: dPML, fs
+  use psdm_rtm_iface
+  implicit none
+  integer :: rank, NumSou, NumRec, ny, nx, store_snap, nt_snap, nt, shotid, unit_fields
+  integer :: nxSou(NumSou), nySou(NumSou), nxRec(NumRec), nyRec(NumRec)
+  real :: dmodel, dt
+  real :: S(nt,NumSou), ctmp(ny,nx), Rec_point(nt,NumRec)
+  character(len=500) :: folder_fields
+  ! the reference writes one file per snapshot under folder_fields; here they stay in HBM
+  call psdm_check(psdm_rtm_forward_disk(psdm_context(), dPML, fs, NumSou, NumRec, nxSou, nySou, nxRec, nyRec, S, &
+                  dmodel, dt, ny, nx, ctmp, shotid, store_snap, nt_snap, nt, Rec_point), 'solver_ac_forward_disk')
+end subroutine
+
+
+subroutine solver_ac_backward_disk(rank,NumSou,NumRec,nxSou,nySou,nxRec,nyRec,&
+Sadj,dmodel,dt,ny,nx,ctmp,&
+shotid,unit_fields,folder_fields,&
+store_snap,nt_snap,nt,kernel)
+  use mod_parfile, only: dPML, fs
+  use psdm_rtm_iface
+  implicit none
+  integer :: rank, NumSou, NumRec, ny, nx, store_snap, nt_snap, nt, shotid, unit_fields
+  integer :: nxSou(NumSou), nySou(NumSou), nxRec(NumRec), nyRec(NumRec)
+  real :: dmodel, dt
+  real :: Sadj(nt,NumRec), ctmp(ny,nx), kernel(ny,nx)
+  character(len=500) :: folder_fields
+  call psdm_check(psdm_rtm_backward_disk(psdm_context(), dPML, fs, NumSou, NumRec, nxSou, nySou, nxRec, nyRec, Sadj, &
+                  dmodel, dt, ny, nx, ctmp, shotid, store_snap, nt_snap, nt, kernel), 'solver_ac_backward_disk')
+end subroutine
