@@ -91,7 +91,7 @@ def main():
     ap.add_argument("--tile-x", type=int, default=8)
     ap.add_argument("--timing-every", type=int, default=16, help="sample every n-th step kernel with HIP events")
     ap.add_argument("--cpu-procs", type=int, default=16)
-    ap.add_argument("--cpu-steps", type=int, default=120)
+    ap.add_argument("--cpu-steps", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
     args = ap.parse_args()

@@ -362,13 +362,9 @@ int run_loop(psdm_rtm_ctx *c, const Layout &L, Prepared &P, int mode, int nslots
         if (t_this) {
             if (int rc = take_event(c, &e0)) return rc;
             if (int rc = take_event(c, &e1)) return rc;
-            HIP_TRY(c, hipEventRecord(e0, c->stream));
-        }
-        psdm::launch_step(a, L.tile_x, c->stream);
-        if (t_this) {
-            HIP_TRY(c, hipEventRecord(e1, c->stream));
             c->ev_pending.emplace_back(e0, e1);
         }
+        psdm::launch_step(a, L.tile_x, c->stream, e0, e1);
         std::swap(pc, pn);
         std::swap(phxo, phxn);
         std::swap(phyo, phyn);

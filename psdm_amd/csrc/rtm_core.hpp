@@ -60,7 +60,7 @@ struct StepArgs {
     int rec_row;           // row of rec that p_c (step k-1) fills, -1 = none
 };
 
-void launch_step(const StepArgs &a, int tile_x, hipStream_t s);
+void launch_step(const StepArgs &a, int tile_x, hipStream_t s, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 void launch_record(const float *p, const int *rec_cell, float *rec, int rec_ld, int row, int nrec, hipStream_t s);
 void launch_c2(const float *vel, float *c2, int ny, int nx, int ldc, float dt, float dx, hipStream_t s);
 void launch_stack(float *img, int ny, int nx_w, int x0, double *sum, int ld_sum, int *fold, hipStream_t s);

@@ -8,6 +8,8 @@
 // (solver_ac_backward_ram.f90:455-481).  Every product and sum is rounded separately in
 // the reference's order (built with -ffp-contract=off), so one step reproduces the
 // reference FP32 arithmetic.
+#include <hip/hip_ext.h>
+
 #include "rtm_core.hpp"
 
 namespace psdm {
@@ -302,12 +304,14 @@ void launch_normalize(const double *sum, const int *fold, float *out, size_t n, 
     hipLaunchKernelGGL(rtm_normalize_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, sum, fold, out, n);
 }
 
-void launch_step(const StepArgs &a, int tile_x, hipStream_t s) {
+// hipExtLaunchKernelGGL stamps start/stop on the dispatch packet itself, so a sampled
+// step reports the kernel's own duration (as a profiler's kernel trace does).
+void launch_step(const StepArgs &a, int tile_x, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
     const int grid = 8 * a.per_xcd;
     if (tile_x == 16)
-        hipLaunchKernelGGL(rtm_step_kernel<16>, dim3(grid), dim3(kBlock), 0, s, a);
+        hipExtLaunchKernelGGL(rtm_step_kernel<16>, dim3(grid), dim3(kBlock), 0, s, e0, e1, 0, a);
     else
-        hipLaunchKernelGGL(rtm_step_kernel<8>, dim3(grid), dim3(kBlock), 0, s, a);
+        hipExtLaunchKernelGGL(rtm_step_kernel<8>, dim3(grid), dim3(kBlock), 0, s, e0, e1, 0, a);
 }
 
 void launch_record(const float *p, const int *rec_cell, float *rec, int rec_ld, int row, int nrec, hipStream_t s) {
