@@ -123,6 +123,66 @@ int psdm_rtm_set_launch(psdm_rtm_ctx *ctx, int tile_x, int use_graph, int graph_
 int psdm_rtm_set_timing(psdm_rtm_ctx *ctx, int timing);
 int psdm_rtm_step_stats(psdm_rtm_ctx *ctx, long long *n_steps, double *mean_ms);
 
+/* ------------------------------------------------------------------------------
+ * Batched / served engine (no counterpart in the reference, whose ranks migrate one
+ * shot at a time, calculate_PSDM_RTM.f90:166-541).  `nslots` shots stay resident and
+ * advance together, one launch per time step over all of them, with the steps
+ * captured in hipGraphs of `chunk_steps` launches.  Shots are admitted into free slots
+ * at chunk boundaries when their snapshot store fits the HBM budget, and each finished
+ * shot is stacked into the engine's FP64 image (PSDM_sum) and int32 fold (Cov_sum).
+ * Per shot the algorithm is the one-shot solver's, except that with store_snap > 1
+ * backward step k images with the snapshot taken at forward step nt + 2 - k (the
+ * reference's own bookkeeping pairs unfilled slots then, see the _ram entry points).
+ * ------------------------------------------------------------------------------ */
+typedef struct psdm_rtm_engine psdm_rtm_engine;
+
+typedef struct {
+    int nslots;              /* shots resident at once (1..256) */
+    int ny, nx_prop;         /* migration model Vpmodel(ny, nx_prop) */
+    const float *d_vel;      /* device pointer to Vpmodel, column-major */
+    int max_nx, max_nt;      /* slot capacity: widest window, longest record */
+    int dPML, fs, variant;   /* as the one-shot solver (variant 0 = _ram, 1 = _disk) */
+    float dmodel, dt;
+    int store_snap;          /* snapshot interval */
+    int snap_format;         /* 0 FP32, 1 FP16, 2 BF16 (power-of-two scale per snapshot tile) */
+    int kahan;               /* compensated FP32 image accumulation */
+    int chunk_steps;         /* time steps per captured graph */
+    int tile_x;              /* 8 or 16 columns per tile */
+    int grid;                /* persistent grid size in workgroups (0 = one per work item) */
+    int use_graph;           /* replay captured graphs (1) or launch each step (0) */
+    int max_inflight;        /* chunks queued ahead of the host (>= 1) */
+    long long hbm_budget;    /* bytes for snapshot stores (0 = nslots x the largest shot) */
+    double *d_psdm_sum;      /* optional caller-owned stack (ny, nx_prop) FP64, zeroed; NULL = engine-owned */
+    int *d_cov_sum;          /* optional caller-owned fold (ny, nx_prop) int32, zeroed */
+} psdm_rtm_engine_cfg;
+
+typedef struct {
+    int x_i, nx_w, nt;         /* window: first model column (1-based), width; record length */
+    int NumRec;
+    int nxSou, nySou;          /* source, 1-based inside the window */
+    const int *nxRec, *nyRec;  /* host arrays, 1-based inside the window */
+    const float *d_wavelet;    /* device, S(nt) */
+    const float *d_data;       /* device, observed gather Data(nt, NumRec) in natural time order */
+    float *d_image_out;        /* optional device (ny, nx_w): receives the shot's image */
+} psdm_rtm_job;
+
+int psdm_rtm_engine_create(psdm_rtm_ctx *ctx, const psdm_rtm_engine_cfg *cfg, psdm_rtm_engine **engine);
+int psdm_rtm_engine_destroy(psdm_rtm_engine *engine);
+/* Queues one shot; device buffers it names must stay valid until it completes. */
+int psdm_rtm_engine_submit(psdm_rtm_engine *engine, const psdm_rtm_job *job, long long *id);
+/* Issues up to max_chunks chunks (retiring and admitting shots at each boundary);
+ * returns early when nothing is resident. */
+int psdm_rtm_engine_run(psdm_rtm_engine *engine, int max_chunks, int *n_active, int *n_pending);
+/* Shots completed since the last poll: ids and device completion times on the host
+ * steady clock (seconds). */
+int psdm_rtm_engine_poll(psdm_rtm_engine *engine, long long *ids, double *t_done, int cap, int *n);
+/* Runs until every submitted shot is complete and stacked. */
+int psdm_rtm_engine_drain(psdm_rtm_engine *engine);
+/* Device pointers of the engine's stack: PSDM_sum (ny, nx_prop) FP64, Cov_sum int32. */
+int psdm_rtm_engine_image(psdm_rtm_engine *engine, double **d_psdm_sum, int **d_cov_sum);
+int psdm_rtm_engine_stats(psdm_rtm_engine *engine, long long *steps_issued, long long *slot_steps,
+                          long long *hbm_used, long long *hbm_budget);
+
 #ifdef __cplusplus
 }
 #endif

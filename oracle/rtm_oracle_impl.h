@@ -205,6 +205,12 @@ static int FN(solver)(int iprop, int variant, int table_mode, int dPML, int fs, 
                       int store_snap, int nt_snap, int nt, R *Fields, R *Rec_point, R *kernel) {
     const int acc = 3;
     const R dmodel = (R)dmodel_f, dt = (R)dt_f;
+    /* variant bit 2: "aligned" snapshot pairing for store_snap > 1 -- backward step k
+     * images with the snapshot the forward took at step nt + 2 - k.  This is not the
+     * reference's bookkeeping (which, with the drivers' nt_snap = nt, reads unfilled
+     * slots); it is the pairing the batched engine uses, checked here in FP64. */
+    const int aligned = (variant & 2) != 0;
+    variant &= 1;
     int dn = dPML + 2 * acc;
     int nxt = nx + 2 * dn, nyt = dn + ny + dn;
     size_t ng = (size_t)nyt * nxt;
@@ -353,7 +359,18 @@ static int FN(solver)(int iprop, int variant, int table_mode, int dPML, int fs, 
         if (fs != 0)
             for (int i = 0; i < nxt; i++) AT(p3, dn, i, nyt) = 0;
 
-        if (l == store_snap) {
+        if (aligned && iprop == 2) {
+            if ((nt + 1 - k) % store_snap == 0) {
+                int s = (nt + 1 - k) / store_snap;
+                if (kernel && Fields && s >= 1 && s <= nt_snap)
+                    for (int i = 0; i < nx; i++)
+                        for (int j = 0; j < ny; j++)
+                            AT(kernel, j, i, ny) =
+                                AT(kernel, j, i, ny) + (R)0.5 * (AT(p3, dn + j, dn + i, nyt) - AT(p1, dn + j, dn + i, nyt)) *
+                                                           Fields[((size_t)(s - 1) * nx + i) * ny + j] * (R)store_snap;
+            }
+            l = 0;
+        } else if (l == store_snap) {
             if (iprop == 1) {
                 snap++;
                 if (Fields && snap >= 1 && snap <= nt_snap)

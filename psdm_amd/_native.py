@@ -33,6 +33,14 @@ SIGNATURES = {
     "psdm_rtm_set_launch": ([_P, _I, _I, _I], _I),
     "psdm_rtm_set_timing": ([_P, _I], _I),
     "psdm_rtm_step_stats": ([_P, C.POINTER(_LL), C.POINTER(C.c_double)], _I),
+    "psdm_rtm_engine_create": ([_P, _P, C.POINTER(_P)], _I),
+    "psdm_rtm_engine_destroy": ([_P], _I),
+    "psdm_rtm_engine_submit": ([_P, _P, C.POINTER(_LL)], _I),
+    "psdm_rtm_engine_run": ([_P, _I, C.POINTER(_I), C.POINTER(_I)], _I),
+    "psdm_rtm_engine_poll": ([_P, C.POINTER(_LL), C.POINTER(C.c_double), _I, C.POINTER(_I)], _I),
+    "psdm_rtm_engine_drain": ([_P], _I),
+    "psdm_rtm_engine_image": ([_P, C.POINTER(_P), C.POINTER(_P)], _I),
+    "psdm_rtm_engine_stats": ([_P, C.POINTER(_LL), C.POINTER(_LL), C.POINTER(_LL), C.POINTER(_LL)], _I),
 }
 
 
@@ -55,6 +63,13 @@ def lib() -> C.CDLL:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(
                 f"{LIB_PATH} is missing: build it with `python -m psdm_amd.build` (there is no CPU fallback)")
+        # torch ships its own libamdhip64.so.7 (same soname as /opt/rocm's): whichever
+        # loads first serves the whole process, and torch cannot initialise on the other
+        # one.  Load torch's first so that torch tensors and this library share it.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         for name, (args, res) in SIGNATURES.items():
             f = getattr(L, name)

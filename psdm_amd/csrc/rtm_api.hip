@@ -10,23 +10,12 @@
 #include <string>
 #include <vector>
 
-#include "../../include/psdm_rtm.h"
 #include "rtm_core.hpp"
+#include "rtm_ctx.hpp"
 
 using psdm::StepArgs;
 
 namespace {
-
-struct DevBuf {
-    void *p = nullptr;
-    size_t n = 0;
-};
-
-struct SnapStore {
-    float *d = nullptr;
-    size_t bytes = 0;
-    int ny = 0, nx = 0, nslots = 0;
-};
 
 // Geometry of one call, 0-based padded-grid quantities.
 struct Layout {
@@ -37,37 +26,14 @@ struct Layout {
 
 }  // namespace
 
-struct psdm_rtm_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    std::string err;
-    std::map<std::string, DevBuf> bufs;  // named workspace buffers, grown on demand
-    std::map<long long, SnapStore> snaps;
-    std::vector<std::pair<void *, size_t>> spare;  // released snapshot buffers
-    int tile_x = 8;
-    int use_graph = 0, graph_steps = 64;
-    int timing = 0;                 // sample every `timing`-th step kernel (0 = off)
-    std::vector<hipEvent_t> ev_free;  // recycled events
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pending;
-    long long n_timed = 0;
-    double sum_ms = 0.0;
-};
-
-namespace {
+namespace psdm {
+namespace host {
 
 int fail(psdm_rtm_ctx *c, int code, const std::string &msg) {
     if (c) c->err = msg;
     return code;
 }
 
-#define HIP_TRY(ctx, expr)                                                                  \
-    do {                                                                                    \
-        hipError_t e_ = (expr);                                                             \
-        if (e_ != hipSuccess)                                                               \
-            return fail(ctx, PSDM_RTM_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
-    } while (0)
-
-// Grow-only named device buffer.
 int get_buf(psdm_rtm_ctx *c, const char *name, size_t bytes, void **out) {
     DevBuf &b = c->bufs[name];
     if (b.n < bytes) {
@@ -83,6 +49,44 @@ int get_buf(psdm_rtm_ctx *c, const char *name, size_t bytes, void **out) {
     *out = b.p;
     return 0;
 }
+
+int set_device(psdm_rtm_ctx *c) {
+    HIP_TRY(c, hipSetDevice(c->device));
+    return 0;
+}
+
+// 1D CPML damping profile, PML_coeffs (solver_PML_coeffs.f90:16-34), evaluated in FP64
+// and rounded to FP32.  Index = distance into the CPML - 1 (interior side first);
+// the first acc=3 cells keep A = B = 0 (:272 starts at i = 1+acc).
+void pml_profile(int dPML, double dt, double d, std::vector<float> &A, std::vector<float> &B) {
+    const int acc = 3, dn = dPML + 2 * acc;
+    std::vector<double> a(dn, 0.0), b(dn, 0.0);
+    const double pi = 3.14159265, Refl = 0.00001, w = (double)(dPML - 1) * d;
+    for (int i = acc; i < dPML + acc; i++) {
+        const double y = (double)(i - acc) * d;
+        const double alpha = 30. * pi * (1. - (y / w));
+        const double sigma = (-(6000. * std::log(Refl)) / (2. * w)) * ((y / w) * (y / w));
+        a[i] = std::exp(-(sigma + alpha) * dt);
+        b[i] = (a[i] - 1.) * sigma / (sigma + alpha);
+    }
+    for (int i = dPML + acc; i < dn; i++) {
+        a[i] = a[dPML + acc - 1];
+        b[i] = b[dPML + acc - 1];
+    }
+    A.resize(dn);
+    B.resize(dn);
+    for (int i = 0; i < dn; i++) {
+        A[i] = (float)a[i];
+        B[i] = (float)b[i];
+    }
+}
+
+}  // namespace host
+}  // namespace psdm
+
+using namespace psdm::host;
+
+namespace {
 
 // Snapshot buffers are recycled: a released store goes to the spare list (kept in
 // stream order, so a later forward may reuse it without waiting) instead of hipFree,
@@ -114,32 +118,6 @@ int alloc_store(psdm_rtm_ctx *c, size_t bytes, float **out) {
     HIP_TRY(c, hipMalloc(&p, bytes));
     *out = (float *)p;
     return 0;
-}
-
-// 1D CPML damping profile, PML_coeffs (solver_PML_coeffs.f90:16-34), evaluated in FP64
-// and rounded to FP32.  Index = distance into the CPML - 1 (interior side first);
-// the first acc=3 cells keep A = B = 0 (:272 starts at i = 1+acc).
-void pml_profile(int dPML, double dt, double d, std::vector<float> &A, std::vector<float> &B) {
-    const int acc = 3, dn = dPML + 2 * acc;
-    std::vector<double> a(dn, 0.0), b(dn, 0.0);
-    const double pi = 3.14159265, Refl = 0.00001, w = (double)(dPML - 1) * d;
-    for (int i = acc; i < dPML + acc; i++) {
-        const double y = (double)(i - acc) * d;
-        const double alpha = 30. * pi * (1. - (y / w));
-        const double sigma = (-(6000. * std::log(Refl)) / (2. * w)) * ((y / w) * (y / w));
-        a[i] = std::exp(-(sigma + alpha) * dt);
-        b[i] = (a[i] - 1.) * sigma / (sigma + alpha);
-    }
-    for (int i = dPML + acc; i < dn; i++) {
-        a[i] = a[dPML + acc - 1];
-        b[i] = b[dPML + acc - 1];
-    }
-    A.resize(dn);
-    B.resize(dn);
-    for (int i = 0; i < dn; i++) {
-        A[i] = (float)a[i];
-        B[i] = (float)b[i];
-    }
 }
 
 int check_geom(psdm_rtm_ctx *c, const psdm_rtm_geom *g) {
@@ -277,7 +255,7 @@ int prepare(psdm_rtm_ctx *c, const Layout &L, const float *d_ctmp, const float *
     if (ninj > 0) {
         if (int rc = get_buf(c, "inj", (size_t)(L.nt - 1) * ninj * sizeof(float), &v)) return rc;
         a.inj_val = (float *)v;
-        psdm::launch_gather(d_traces, L.nt, di + o_cols, ninj, (float *)v, c->stream);
+        psdm::launch_gather(d_traces, L.nt, di + o_cols, ninj, (float *)v, 0, c->stream);
     }
     // host vectors die here: wait for the copies before returning
     HIP_TRY(c, hipStreamSynchronize(c->stream));
@@ -442,11 +420,6 @@ psdm_rtm_geom host_geom(int dPML, int fs, int variant, int NumSou, int NumRec, c
     g.nxRec = nxRec;
     g.nyRec = nyRec;
     return g;
-}
-
-int set_device(psdm_rtm_ctx *c) {
-    HIP_TRY(c, hipSetDevice(c->device));
-    return 0;
 }
 
 // Host-array forward: upload, run, download traces (and snapshots when asked).
@@ -629,7 +602,7 @@ int psdm_rtm_stack_image(psdm_rtm_ctx *c, float *d_kernel, int ny, int nx_w, int
     if (ny < 1 || nx_w < 1 || x_i < 1 || x_i + nx_w - 1 > nx_prop)
         return fail(c, PSDM_RTM_EINVAL, "window [x_i, x_i+nx_w-1] outside 1..nx_prop");
     if (int rc = set_device(c)) return rc;
-    psdm::launch_stack(d_kernel, ny, nx_w, x_i - 1, d_psdm_sum, ny, d_cov_sum, c->stream);
+    psdm::launch_stack(d_kernel, nullptr, ny, nx_w, x_i - 1, d_psdm_sum, ny, d_cov_sum, c->stream);
     HIP_TRY(c, hipGetLastError());
     return 0;
 }

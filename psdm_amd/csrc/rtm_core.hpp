@@ -17,6 +17,11 @@ constexpr int kTileY = 64;        // rows per tile: one wavefront, one row per l
 constexpr int kBlock = 256;       // 4 wavefronts per workgroup
 constexpr int kMaxTables = 1024;  // max CPML cells per side (dPML + 6)
 
+// Snapshot storage formats.  FP16/BF16 snapshots carry one power-of-two scale per
+// (snapshot, tile): the tile's max-abs value is mapped to 2^14 (FP16) or 1 (BF16), so
+// the only error is the 16-bit rounding of the scaled value.
+enum SnapFmt { kSnapF32 = 0, kSnapF16 = 1, kSnapBF16 = 2 };
+
 // Everything one time step of one shot needs.  Layout in HBM:
 //   wavefields  p[i*ld + j], i in [0,nxt), j in [0,nyt), ld = nyt rounded up to 64
 //   x-CPML mem  x[c*ld + j], c in [0,2dn): c < dn left strip column c, else right
@@ -33,6 +38,9 @@ struct StepArgs {
     int k;            // time step index of the reference loop, 2..nt
     int mode;         // 0 forward (snapshots), 1 backward (imaging)
     int snap_slot;    // snapshot slot this step writes/reads, -1 = none
+    int zc, zp, zm;   // state not yet written this phase reads as zero: p(k-1), p(k-2), CPML memory
+    int snap_fmt;     // SnapFmt
+    int kahan;        // compensated image accumulation
     float dt, ss_f;   // dt; float(store_snap)
     const float *pc;  // p at step k-1
     float *pn;        // in: p at step k-2; out: p at step k
@@ -50,9 +58,11 @@ struct StepArgs {
     const int *ev_inj;     // column of inj_val
     const float *inj_val;  // (nt-1) x ninj, row k-2 holds the values injected at step k
     int ninj;
-    float *snaps;          // forward: snapshot base; backward: read
+    void *snaps;           // forward: snapshot base (format snap_fmt); backward: read
+    float *snap_scale;     // FP16/BF16: scale[slot * ntiles + tile]
     float *image;          // backward: kernel(ny,nx), ld = ny
-    size_t snap_stride;    // floats per snapshot (ny*nx)
+    float *image_comp;     // Kahan compensation (kahan != 0)
+    size_t snap_stride;    // elements per snapshot (ny*nx)
     const int *rec_cell;   // receivers: cell j + i*ld, or -1 outside the grid
     float *rec;            // Rec_point(nt, NumRec) column-major, leading dimension rec_ld = nt
     int rec_ld;
@@ -60,11 +70,43 @@ struct StepArgs {
     int rec_row;           // row of rec that p_c (step k-1) fills, -1 = none
 };
 
+// One slot of the batched engine: a shot resident on the device for its forward and
+// backward phases.  `a` holds the static fields; the per-step pointers and flags are
+// resolved on the device from the global step (rtm_batch_kernel).
+struct SlotDesc {
+    StepArgs a;
+    float *pA, *pB, *phxA, *phxB, *phyA, *phyB, *hsA, *hsB;
+    long long g0;  // global step of the slot's first forward step (k = 2); < 0: slot idle
+    int nt, ss, nsnap;
+    const int *ev_ptr_f, *ev_cell_f, *ev_inj_f;
+    const float *inj_f;
+    int ninj_f;
+    const int *ev_ptr_b, *ev_cell_b, *ev_inj_b;
+    const float *inj_b;
+    int ninj_b;
+};
+
+struct BatchArgs {
+    const SlotDesc *slots;
+    int nslots;
+    int cap_tiles;           // work items per slot (tiles of the slot capacity)
+    int nwork;               // nslots * cap_tiles
+    int per_xcd;             // work items per XCD
+    const long long *gbase;  // global step of the chunk's first step
+    int goff;                // this launch's offset inside the chunk
+};
+
 void launch_step(const StepArgs &a, int tile_x, hipStream_t s, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
+void launch_batch(const BatchArgs &b, StepArgs *cur, int tile_x, int grid, hipStream_t s, hipEvent_t e0 = nullptr,
+                  hipEvent_t e1 = nullptr);
+void launch_resolve(const SlotDesc *slots, int nslots, const long long *gbase, StepArgs *cur, hipStream_t s);
+void launch_advance(long long *gbase, long long by, hipStream_t s);
 void launch_record(const float *p, const int *rec_cell, float *rec, int rec_ld, int row, int nrec, hipStream_t s);
 void launch_c2(const float *vel, float *c2, int ny, int nx, int ldc, float dt, float dx, hipStream_t s);
-void launch_stack(float *img, int ny, int nx_w, int x0, double *sum, int ld_sum, int *fold, hipStream_t s);
+void launch_stack(float *img, float *comp, int ny, int nx_w, int x0, double *sum, int ld_sum, int *fold,
+                  hipStream_t s);
 void launch_normalize(const double *sum, const int *fold, float *out, size_t n, hipStream_t s);
-void launch_gather(const float *traces, int nt, const int *cols, int ncols, float *out, hipStream_t s);
+// out[r*ncols + e] = traces[cols[e]*nt + (reverse ? nt-1-r : r)], r in [0, nt-1)
+void launch_gather(const float *traces, int nt, const int *cols, int ncols, float *out, int reverse, hipStream_t s);
 
 }  // namespace psdm

@@ -1,0 +1,64 @@
+// Context internals shared by the one-shot runtime (rtm_api.hip) and the batched
+// engine (rtm_engine.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/psdm_rtm.h"
+
+namespace psdm {
+namespace host {
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t n = 0;
+};
+
+struct SnapStore {
+    float *d = nullptr;
+    size_t bytes = 0;
+    int ny = 0, nx = 0, nslots = 0;
+};
+
+}  // namespace host
+}  // namespace psdm
+
+struct psdm_rtm_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    std::map<std::string, psdm::host::DevBuf> bufs;  // named workspace buffers, grown on demand
+    std::map<long long, psdm::host::SnapStore> snaps;
+    std::vector<std::pair<void *, size_t>> spare;  // released snapshot buffers
+    int tile_x = 8;
+    int use_graph = 0, graph_steps = 64;
+    int timing = 0;                                 // sample every `timing`-th step kernel (0 = off)
+    std::vector<hipEvent_t> ev_free;                // recycled events
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pending;
+    long long n_timed = 0;
+    double sum_ms = 0.0;
+};
+
+namespace psdm {
+namespace host {
+
+int fail(psdm_rtm_ctx *c, int code, const std::string &msg);
+// Grow-only named device buffer of the context.
+int get_buf(psdm_rtm_ctx *c, const char *name, size_t bytes, void **out);
+int set_device(psdm_rtm_ctx *c);
+// 1D CPML damping profile (FP64 evaluation rounded to FP32), index = distance - 1.
+void pml_profile(int dPML, double dt, double d, std::vector<float> &A, std::vector<float> &B);
+
+}  // namespace host
+}  // namespace psdm
+
+#define HIP_TRY(ctx, expr)                                                                                   \
+    do {                                                                                                     \
+        hipError_t e_ = (expr);                                                                              \
+        if (e_ != hipSuccess)                                                                                \
+            return psdm::host::fail(ctx, PSDM_RTM_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)

@@ -1,0 +1,615 @@
+// Batched / served RTM engine.  S shot slots stay resident on the device and advance
+// together: one launch per time step walks the slot x tile work list (rtm_batch_kernel),
+// and the time steps are captured in a hipGraph of `chunk_steps` launches that is
+// replayed.  Between replays the host retires finished shots (image copied out and
+// stacked in FP64) and admits queued shots into free slots when their snapshot store
+// fits the HBM budget.  Each slot runs the same per-shot algorithm as the one-shot
+// path; slots never read each other's memory, so a shot's result does not depend on
+// what shares the device with it.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <vector>
+
+#include "rtm_core.hpp"
+#include "rtm_ctx.hpp"
+
+using namespace psdm;
+using namespace psdm::host;
+
+namespace {
+
+// First-fit allocator over one device arena (the snapshot stores).
+struct Arena {
+    size_t size = 0;
+    std::map<size_t, size_t> free_, used_;  // offset -> length
+    void init(size_t n) {
+        size = n;
+        free_.clear();
+        used_.clear();
+        if (n) free_[0] = n;
+    }
+    bool alloc(size_t n, size_t *off) {
+        n = (n + 255) & ~(size_t)255;
+        for (auto it = free_.begin(); it != free_.end(); ++it)
+            if (it->second >= n) {
+                *off = it->first;
+                const size_t rem = it->second - n, o2 = it->first + n;
+                free_.erase(it);
+                if (rem) free_[o2] = rem;
+                used_[*off] = n;
+                return true;
+            }
+        return false;
+    }
+    void release(size_t off) {
+        auto u = used_.find(off);
+        if (u == used_.end()) return;
+        const size_t n = u->second;
+        used_.erase(u);
+        auto it = free_.emplace(off, n).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+    size_t in_use() const {
+        size_t t = 0;
+        for (auto &kv : used_) t += kv.second;
+        return t;
+    }
+};
+
+struct Job {
+    long long id = 0;
+    psdm_rtm_job j{};
+    std::vector<int> nxRec, nyRec;
+    size_t snap_bytes = 0;
+};
+
+struct Slot {
+    bool busy = false;
+    Job job;
+    long long g0 = 0, g_end = 0;
+    size_t arena_off = 0;
+    int *d_ints = nullptr;
+    size_t n_ints = 0;
+    float *d_inj = nullptr;
+    size_t n_inj = 0;
+};
+
+struct DoneRec {
+    long long id, chunk;
+};
+
+double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+struct psdm_rtm_engine {
+    psdm_rtm_ctx *c = nullptr;
+    psdm_rtm_engine_cfg cfg{};
+    int dn = 0, ld = 0, nyt = 0, nxt_max = 0, ntile_y = 0, cap_tiles = 0, nwork = 0, per_xcd = 0, grid = 0;
+    size_t n_p = 0, n_x = 0, n_y = 0, state_stride = 0, esize = 4;
+    // device
+    SlotDesc *d_slots = nullptr;
+    StepArgs *d_cur = nullptr;
+    long long *d_gbase = nullptr;
+    float *d_tabs = nullptr, *d_state = nullptr, *d_c2 = nullptr, *d_img = nullptr, *d_comp = nullptr;
+    char *d_arena = nullptr;
+    double *d_sum = nullptr;
+    int *d_fold = nullptr;
+    bool own_image = true;
+    // host
+    SlotDesc *h_slots = nullptr;  // pinned, two copies (ring) for asynchronous upload
+    int h_ring = 0;
+    hipEvent_t h_ring_ev[2] = {nullptr, nullptr};
+    bool slots_dirty = true;
+    std::vector<Slot> slots;
+    std::deque<Job> pending;
+    Arena arena;
+    long long G = 0, next_id = 1, chunks = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    static constexpr int kRing = 256;
+    hipEvent_t ev_chunk[kRing] = {};
+    hipEvent_t ev_origin = nullptr;
+    double t_origin = 0.0;
+    std::vector<DoneRec> done_wait;  // retired, waiting for their chunk to complete
+    std::vector<std::pair<long long, double>> done_ready;
+    double chunk_ms_sum = 0.0;
+    long long chunk_ms_n = 0;
+    long long active_slot_steps = 0;  // sum over issued steps of busy slots
+};
+
+namespace {
+
+int efail(psdm_rtm_engine *e, int code, const std::string &m) { return fail(e->c, code, m); }
+
+// Launches one chunk of time steps (graph replay or eager launches).
+int issue_chunk(psdm_rtm_engine *e) {
+    psdm_rtm_ctx *c = e->c;
+    const int C = e->cfg.chunk_steps;
+    if (e->exec) {
+        HIP_TRY(c, hipGraphLaunch(e->exec, c->stream));
+    } else {
+        for (int g = 0; g < C; g++) {
+            BatchArgs b{e->d_slots, e->cfg.nslots, e->cap_tiles, e->nwork, e->per_xcd, e->d_gbase, g};
+            launch_batch(b, e->d_cur, e->cfg.tile_x, e->grid, c->stream);
+        }
+        launch_advance(e->d_gbase, C, c->stream);
+    }
+    HIP_TRY(c, hipGetLastError());
+    return 0;
+}
+
+int capture_graph(psdm_rtm_engine *e) {
+    psdm_rtm_ctx *c = e->c;
+    HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+    for (int g = 0; g < e->cfg.chunk_steps; g++) {
+        BatchArgs b{e->d_slots, e->cfg.nslots, e->cap_tiles, e->nwork, e->per_xcd, e->d_gbase, g};
+        launch_batch(b, e->d_cur, e->cfg.tile_x, e->grid, c->stream);
+    }
+    launch_advance(e->d_gbase, e->cfg.chunk_steps, c->stream);
+    HIP_TRY(c, hipStreamEndCapture(c->stream, &e->graph));
+    HIP_TRY(c, hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0));
+    return 0;
+}
+
+size_t snap_bytes_of(const psdm_rtm_engine *e, int nx_w, int nt) {
+    const int nsnap = (nt - 1) / e->cfg.store_snap;
+    const int ntiles = e->ntile_y * ((nx_w + 2 * e->dn + e->cfg.tile_x - 1) / e->cfg.tile_x);
+    size_t b = (size_t)nsnap * e->cfg.ny * nx_w * e->esize;
+    b = (b + 255) & ~(size_t)255;
+    if (e->cfg.snap_format != kSnapF32) b += (size_t)nsnap * ntiles * sizeof(float);
+    return b;
+}
+
+// Builds one slot's device view for `job` starting at global step G.
+int admit(psdm_rtm_engine *e, int s, Job &job, size_t arena_off) {
+    psdm_rtm_ctx *c = e->c;
+    const psdm_rtm_job &J = job.j;
+    const int ny = e->cfg.ny, nx = J.nx_w, nt = J.nt, dn = e->dn, TX = e->cfg.tile_x;
+    const int nxt = nx + 2 * dn;
+    const int ntiles = e->ntile_y * ((nxt + TX - 1) / TX);
+    Slot &sl = e->slots[s];
+
+    // injection events: forward source (1 point), backward receivers (last writer wins)
+    auto build = [&](const int *px, const int *py, int n, std::vector<int> &ptr, std::vector<int> &cell,
+                     std::vector<int> &inj, std::vector<int> &cols) {
+        std::map<int, int> last;
+        for (int q = 0; q < n; q++) {
+            const int jw = py[q] - 1, iw = px[q] - 1;
+            if (jw < 0 || jw >= ny || iw < 0 || iw >= nx) continue;
+            last[(dn + jw) + (dn + iw) * e->nyt] = q;
+        }
+        std::vector<std::vector<std::pair<int, int>>> per(ntiles);
+        for (auto &kv : last) {
+            const int j = kv.first % e->nyt, i = kv.first / e->nyt;
+            per[(i / TX) * e->ntile_y + j / kTileY].push_back(kv);
+        }
+        ptr.assign(ntiles + 1, 0);
+        for (int t = 0; t < ntiles; t++) {
+            for (auto &kv : per[t]) {
+                cell.push_back(kv.first);
+                inj.push_back((int)cols.size());
+                cols.push_back(kv.second);
+            }
+            ptr[t + 1] = (int)cell.size();
+        }
+    };
+    const int sx = J.nxSou, sy = J.nySou;
+    std::vector<int> fp, fc, fi, fcol, bp, bc, bi, bcol;
+    build(&sx, &sy, 1, fp, fc, fi, fcol);
+    build(job.nxRec.data(), job.nyRec.data(), J.NumRec, bp, bc, bi, bcol);
+    std::vector<int> ints;
+    auto put = [&](const std::vector<int> &v) {
+        const size_t o = ints.size();
+        ints.insert(ints.end(), v.begin(), v.end());
+        return o;
+    };
+    const size_t ofp = put(fp), ofc = put(fc), ofi = put(fi), ofcol = put(fcol);
+    const size_t obp = put(bp), obc = put(bc), obi = put(bi), obcol = put(bcol);
+    if (sl.n_ints < ints.size()) {
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        if (sl.d_ints) HIP_TRY(c, hipFree(sl.d_ints));
+        HIP_TRY(c, hipMalloc(&sl.d_ints, ints.size() * sizeof(int)));
+        sl.n_ints = ints.size();
+    }
+    HIP_TRY(c, hipMemcpyAsync(sl.d_ints, ints.data(), ints.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    const int ninj_f = (int)fcol.size(), ninj_b = (int)bcol.size();
+    const size_t n_inj = (size_t)(nt - 1) * (ninj_f + ninj_b);
+    if (sl.n_inj < n_inj) {
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        if (sl.d_inj) HIP_TRY(c, hipFree(sl.d_inj));
+        HIP_TRY(c, hipMalloc(&sl.d_inj, std::max<size_t>(n_inj, 1) * sizeof(float)));
+        sl.n_inj = std::max<size_t>(n_inj, 1);
+    }
+    float *inj_f = sl.d_inj, *inj_b = sl.d_inj + (size_t)(nt - 1) * ninj_f;
+    launch_gather(J.d_wavelet, nt, sl.d_ints + ofcol, ninj_f, inj_f, 0, c->stream);
+    launch_gather(J.d_data, nt, sl.d_ints + obcol, ninj_b, inj_b, 1, c->stream);
+    float *c2 = e->d_c2 + (size_t)s * ny * e->cfg.max_nx;
+    launch_c2(e->cfg.d_vel + (size_t)(J.x_i - 1) * ny, c2, ny, nx, ny, e->cfg.dt, e->cfg.dmodel, c->stream);
+    // the host vectors above are read by the queued copy: wait before they go away
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+
+    SlotDesc &d = e->h_slots[e->h_ring * e->cfg.nslots + s];
+    std::memset(&d, 0, sizeof(d));
+    StepArgs &a = d.a;
+    a.ny = ny;
+    a.nx = nx;
+    a.dn = dn;
+    a.nyt = e->nyt;
+    a.nxt = nxt;
+    a.ld = e->ld;
+    a.ntile_y = e->ntile_y;
+    a.ntiles = ntiles;
+    a.per_xcd = 0;
+    a.fs = e->cfg.fs != 0;
+    a.ram = e->cfg.variant == 0;
+    a.snap_fmt = e->cfg.snap_format;
+    a.kahan = e->cfg.kahan;
+    a.dt = e->cfg.dt;
+    a.ss_f = (float)e->cfg.store_snap;
+    a.c2 = c2;
+    a.ldc = ny;
+    a.fA = e->d_tabs;
+    a.fB = e->d_tabs + dn;
+    a.snaps = e->d_arena + arena_off;
+    const int nsnap = (nt - 1) / e->cfg.store_snap;
+    const size_t data_b = ((size_t)nsnap * ny * nx * e->esize + 255) & ~(size_t)255;
+    a.snap_scale = e->cfg.snap_format != kSnapF32 ? (float *)(e->d_arena + arena_off + data_b) : nullptr;
+    a.image = e->d_img + (size_t)s * ny * e->cfg.max_nx;
+    a.image_comp = e->d_comp + (size_t)s * ny * e->cfg.max_nx;
+    a.snap_stride = (size_t)ny * nx;
+    a.rec_row = -1;
+    float *st = e->d_state + (size_t)s * e->state_stride;
+    d.pA = st;
+    d.pB = st + e->n_p;
+    d.phxA = st + 2 * e->n_p;
+    d.phxB = d.phxA + e->n_x;
+    a.chx = d.phxB + e->n_x;
+    d.phyA = a.chx + e->n_x;
+    d.phyB = d.phyA + e->n_y;
+    a.chy = d.phyB + e->n_y;
+    d.hsA = a.chy + e->n_y;
+    d.hsB = d.hsA + 2 * dn;
+    d.g0 = e->G;
+    d.nt = nt;
+    d.ss = e->cfg.store_snap;
+    d.nsnap = nsnap;
+    d.ev_ptr_f = sl.d_ints + ofp;
+    d.ev_cell_f = sl.d_ints + ofc;
+    d.ev_inj_f = sl.d_ints + ofi;
+    d.inj_f = inj_f;
+    d.ninj_f = ninj_f;
+    d.ev_ptr_b = sl.d_ints + obp;
+    d.ev_cell_b = sl.d_ints + obc;
+    d.ev_inj_b = sl.d_ints + obi;
+    d.inj_b = inj_b;
+    d.ninj_b = ninj_b;
+    sl.busy = true;
+    sl.g0 = e->G;
+    sl.g_end = e->G + 2LL * (nt - 1);
+    sl.arena_off = arena_off;
+    sl.job = std::move(job);
+    e->slots_dirty = true;
+    return 0;
+}
+
+int retire(psdm_rtm_engine *e, int s) {
+    psdm_rtm_ctx *c = e->c;
+    Slot &sl = e->slots[s];
+    const psdm_rtm_job &J = sl.job.j;
+    const int ny = e->cfg.ny;
+    float *img = e->d_img + (size_t)s * ny * e->cfg.max_nx;
+    float *comp = e->d_comp + (size_t)s * ny * e->cfg.max_nx;
+    if (J.d_image_out)
+        HIP_TRY(c, hipMemcpyAsync(J.d_image_out, img, (size_t)ny * J.nx_w * sizeof(float), hipMemcpyDeviceToDevice,
+                                  c->stream));
+    launch_stack(img, e->cfg.kahan ? comp : nullptr, ny, J.nx_w, J.x_i - 1, e->d_sum, ny, e->d_fold, c->stream);
+    e->arena.release(sl.arena_off);
+    const int C = e->cfg.chunk_steps;
+    e->done_wait.push_back({sl.job.id, (sl.g_end - 1) / C});
+    sl.busy = false;
+    SlotDesc &d = e->h_slots[e->h_ring * e->cfg.nslots + s];
+    d.g0 = -1;
+    e->slots_dirty = true;
+    return 0;
+}
+
+int upload_slots(psdm_rtm_engine *e) {
+    psdm_rtm_ctx *c = e->c;
+    if (!e->slots_dirty) return 0;
+    const int S = e->cfg.nslots;
+    SlotDesc *src = e->h_slots + e->h_ring * S;
+    HIP_TRY(c, hipMemcpyAsync(e->d_slots, src, S * sizeof(SlotDesc), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipEventRecord(e->h_ring_ev[e->h_ring], c->stream));
+    // continue editing in the other copy once its previous upload has been consumed
+    const int nxt = e->h_ring ^ 1;
+    HIP_TRY(c, hipEventSynchronize(e->h_ring_ev[nxt]));
+    std::memcpy(e->h_slots + nxt * S, src, S * sizeof(SlotDesc));
+    e->h_ring = nxt;
+    e->slots_dirty = false;
+    return 0;
+}
+
+// Reports jobs whose last chunk has completed.  Entries whose chunk is at or before
+// `block_upto` are waited for (their ring event is about to be reused); the rest are
+// only queried.
+int collect(psdm_rtm_engine *e, long long block_upto) {
+    psdm_rtm_ctx *c = e->c;
+    std::vector<DoneRec> keep;
+    for (auto &d : e->done_wait) {
+        hipEvent_t ev = e->ev_chunk[d.chunk % psdm_rtm_engine::kRing];
+        hipError_t q = d.chunk <= block_upto ? hipEventSynchronize(ev) : hipEventQuery(ev);
+        if (q == hipSuccess) {
+            float ms = 0.f;
+            HIP_TRY(c, hipEventElapsedTime(&ms, e->ev_origin, ev));
+            e->done_ready.emplace_back(d.id, e->t_origin + 1e-3 * ms);
+        } else if (q == hipErrorNotReady) {
+            keep.push_back(d);
+        } else {
+            return fail(c, PSDM_RTM_EDEVICE, std::string("chunk event: ") + hipGetErrorString(q));
+        }
+    }
+    e->done_wait.swap(keep);
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int psdm_rtm_engine_create(psdm_rtm_ctx *c, const psdm_rtm_engine_cfg *cfg, psdm_rtm_engine **out) {
+    if (!c || !cfg || !out) return PSDM_RTM_EINVAL;
+    *out = nullptr;
+    if (int rc = set_device(c)) return rc;
+    if (cfg->nslots < 1 || cfg->nslots > 256) return fail(c, PSDM_RTM_EINVAL, "nslots must be in 1..256");
+    if (cfg->ny < 1 || cfg->max_nx < 1 || cfg->max_nt < 2 || cfg->nx_prop < 1)
+        return fail(c, PSDM_RTM_EINVAL, "bad engine capacity");
+    if (cfg->dPML < 2 || cfg->dPML + 6 > kMaxTables) return fail(c, PSDM_RTM_EINVAL, "dPML out of range");
+    if (cfg->store_snap < 1) return fail(c, PSDM_RTM_EINVAL, "store_snap must be >= 1");
+    if (cfg->snap_format < 0 || cfg->snap_format > 2) return fail(c, PSDM_RTM_EINVAL, "snap_format must be 0, 1 or 2");
+    if (cfg->tile_x != 8 && cfg->tile_x != 16) return fail(c, PSDM_RTM_EINVAL, "tile_x must be 8 or 16");
+    if (cfg->chunk_steps < 1 || cfg->chunk_steps > 4096) return fail(c, PSDM_RTM_EINVAL, "chunk_steps out of range");
+    if (!cfg->d_vel) return fail(c, PSDM_RTM_EINVAL, "null velocity model");
+    if (!(cfg->dt > 0.f) || !(cfg->dmodel > 0.f)) return fail(c, PSDM_RTM_EINVAL, "dt and dmodel must be > 0");
+    auto *e = new psdm_rtm_engine();
+    e->c = c;
+    e->cfg = *cfg;
+    e->dn = cfg->dPML + 6;
+    e->nyt = cfg->ny + 2 * e->dn;
+    e->ld = (e->nyt + 63) / 64 * 64;
+    e->nxt_max = cfg->max_nx + 2 * e->dn;
+    e->ntile_y = (e->nyt + kTileY - 1) / kTileY;
+    e->cap_tiles = e->ntile_y * ((e->nxt_max + cfg->tile_x - 1) / cfg->tile_x);
+    e->nwork = cfg->nslots * e->cap_tiles;
+    e->per_xcd = (e->nwork + 7) / 8;
+    e->grid = cfg->grid > 0 ? std::min(cfg->grid, e->per_xcd * 8) : e->per_xcd * 8;
+    e->esize = cfg->snap_format == kSnapF32 ? 4 : 2;
+    const size_t dn2 = 2 * e->dn;
+    e->n_p = (size_t)e->ld * e->nxt_max;
+    e->n_x = (size_t)e->ld * dn2;
+    e->n_y = dn2 * e->nxt_max;
+    e->state_stride = (2 * e->n_p + 3 * e->n_x + 3 * e->n_y + 2 * dn2 + 63) / 64 * 64;
+    const size_t S = cfg->nslots, win = (size_t)cfg->ny * cfg->max_nx;
+    size_t budget = cfg->hbm_budget > 0 ? (size_t)cfg->hbm_budget : S * snap_bytes_of(e, cfg->max_nx, cfg->max_nt);
+    auto bail = [&](int rc) {
+        psdm_rtm_engine_destroy(e);
+        return rc;
+    };
+#define ETRY(expr)                                                                                \
+    do {                                                                                          \
+        hipError_t e_ = (expr);                                                                   \
+        if (e_ != hipSuccess)                                                                     \
+            return bail(fail(c, PSDM_RTM_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_))); \
+    } while (0)
+    ETRY(hipMalloc(&e->d_slots, S * sizeof(SlotDesc)));
+    ETRY(hipMalloc(&e->d_cur, 2 * S * sizeof(StepArgs)));
+    ETRY(hipMalloc(&e->d_gbase, sizeof(long long)));
+    ETRY(hipMalloc(&e->d_tabs, 2 * e->dn * sizeof(float)));
+    ETRY(hipMalloc(&e->d_state, S * e->state_stride * sizeof(float)));
+    ETRY(hipMalloc(&e->d_c2, S * win * sizeof(float)));
+    ETRY(hipMalloc(&e->d_img, S * win * sizeof(float)));
+    ETRY(hipMalloc(&e->d_comp, S * win * sizeof(float)));
+    ETRY(hipMalloc(&e->d_arena, std::max<size_t>(budget, 256)));
+    if (cfg->d_psdm_sum && cfg->d_cov_sum) {
+        e->d_sum = cfg->d_psdm_sum;
+        e->d_fold = cfg->d_cov_sum;
+        e->own_image = false;
+    } else {
+        ETRY(hipMalloc(&e->d_sum, (size_t)cfg->ny * cfg->nx_prop * sizeof(double)));
+        ETRY(hipMalloc(&e->d_fold, (size_t)cfg->ny * cfg->nx_prop * sizeof(int)));
+        ETRY(hipMemsetAsync(e->d_sum, 0, (size_t)cfg->ny * cfg->nx_prop * sizeof(double), c->stream));
+        ETRY(hipMemsetAsync(e->d_fold, 0, (size_t)cfg->ny * cfg->nx_prop * sizeof(int), c->stream));
+    }
+    ETRY(hipHostMalloc((void **)&e->h_slots, 2 * S * sizeof(SlotDesc), hipHostMallocDefault));
+    ETRY(hipMemsetAsync(e->d_img, 0, S * win * sizeof(float), c->stream));
+    ETRY(hipMemsetAsync(e->d_comp, 0, S * win * sizeof(float), c->stream));
+    ETRY(hipMemsetAsync(e->d_gbase, 0, sizeof(long long), c->stream));
+    std::vector<float> fA, fB;
+    pml_profile(cfg->dPML, (double)cfg->dt, (double)cfg->dmodel, fA, fB);
+    fA.insert(fA.end(), fB.begin(), fB.end());
+    ETRY(hipMemcpyAsync(e->d_tabs, fA.data(), fA.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    std::memset(e->h_slots, 0, 2 * S * sizeof(SlotDesc));
+    for (size_t s = 0; s < 2 * S; s++) e->h_slots[s].g0 = -1;
+    for (int q = 0; q < 2; q++) ETRY(hipEventCreate(&e->h_ring_ev[q]));
+    for (int q = 0; q < psdm_rtm_engine::kRing; q++) ETRY(hipEventCreate(&e->ev_chunk[q]));
+    ETRY(hipEventCreate(&e->ev_origin));
+    e->slots.resize(S);
+    e->arena.init(budget);
+    if (int rc = upload_slots(e)) return bail(rc);
+    ETRY(hipStreamSynchronize(c->stream));
+    if (cfg->use_graph)
+        if (int rc = capture_graph(e)) return bail(rc);
+    ETRY(hipEventRecord(e->ev_origin, c->stream));
+    ETRY(hipEventSynchronize(e->ev_origin));
+    e->t_origin = now_s();
+#undef ETRY
+    *out = e;
+    return 0;
+}
+
+int psdm_rtm_engine_destroy(psdm_rtm_engine *e) {
+    if (!e) return 0;
+    psdm_rtm_ctx *c = e->c;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    if (e->exec) (void)hipGraphExecDestroy(e->exec);
+    if (e->graph) (void)hipGraphDestroy(e->graph);
+    for (auto &sl : e->slots) {
+        if (sl.d_ints) (void)hipFree(sl.d_ints);
+        if (sl.d_inj) (void)hipFree(sl.d_inj);
+    }
+    void *bufs[] = {e->d_slots, e->d_cur, e->d_gbase, e->d_tabs, e->d_state, e->d_c2, e->d_img, e->d_comp, e->d_arena};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    if (e->own_image) {
+        if (e->d_sum) (void)hipFree(e->d_sum);
+        if (e->d_fold) (void)hipFree(e->d_fold);
+    }
+    if (e->h_slots) (void)hipHostFree(e->h_slots);
+    for (auto ev : e->h_ring_ev)
+        if (ev) (void)hipEventDestroy(ev);
+    for (auto ev : e->ev_chunk)
+        if (ev) (void)hipEventDestroy(ev);
+    if (e->ev_origin) (void)hipEventDestroy(e->ev_origin);
+    delete e;
+    return 0;
+}
+
+int psdm_rtm_engine_submit(psdm_rtm_engine *e, const psdm_rtm_job *job, long long *id) {
+    if (!e || !job) return PSDM_RTM_EINVAL;
+    psdm_rtm_ctx *c = e->c;
+    if (job->nx_w < 1 || job->nx_w > e->cfg.max_nx) return efail(e, PSDM_RTM_EINVAL, "window wider than the slot capacity");
+    if (job->nt < 2 || job->nt > e->cfg.max_nt) return efail(e, PSDM_RTM_EINVAL, "record length outside 2..max_nt");
+    if (job->x_i < 1 || job->x_i + job->nx_w - 1 > e->cfg.nx_prop)
+        return efail(e, PSDM_RTM_EINVAL, "window outside the velocity model");
+    if (e->cfg.variant == 0 && job->nx_w < e->dn)
+        return efail(e, PSDM_RTM_EINVAL, "the _ram padding needs nx_w >= dPML+6");
+    if (job->NumRec < 0 || (job->NumRec > 0 && (!job->nxRec || !job->nyRec || !job->d_data)) || !job->d_wavelet)
+        return efail(e, PSDM_RTM_EINVAL, "missing gather, receivers or wavelet");
+    Job J;
+    J.id = e->next_id++;
+    J.j = *job;
+    J.nxRec.assign(job->nxRec, job->nxRec + job->NumRec);
+    J.nyRec.assign(job->nyRec, job->nyRec + job->NumRec);
+    J.j.nxRec = nullptr;
+    J.j.nyRec = nullptr;
+    J.snap_bytes = snap_bytes_of(e, job->nx_w, job->nt);
+    if (J.snap_bytes > e->arena.size)
+        return efail(e, PSDM_RTM_EINVAL, "the shot's snapshot store exceeds the engine's HBM budget");
+    if (id) *id = J.id;
+    e->pending.push_back(std::move(J));
+    (void)c;
+    return 0;
+}
+
+int psdm_rtm_engine_run(psdm_rtm_engine *e, int max_chunks, int *n_active, int *n_pending) {
+    if (!e) return PSDM_RTM_EINVAL;
+    psdm_rtm_ctx *c = e->c;
+    if (int rc = set_device(c)) return rc;
+    const int C = e->cfg.chunk_steps;
+    const int inflight = std::max(1, e->cfg.max_inflight);
+    for (int n = 0; n < max_chunks; n++) {
+        // retire shots whose last step is in an issued chunk
+        for (int s = 0; s < e->cfg.nslots; s++)
+            if (e->slots[s].busy && e->slots[s].g_end <= e->G)
+                if (int rc = retire(e, s)) return rc;
+        // admit queued shots into free slots, first fit against the HBM budget
+        for (int s = 0; s < e->cfg.nslots && !e->pending.empty(); s++) {
+            if (e->slots[s].busy) continue;
+            for (auto it = e->pending.begin(); it != e->pending.end(); ++it) {
+                size_t off;
+                if (!e->arena.alloc(it->snap_bytes, &off)) continue;
+                Job job = std::move(*it);
+                e->pending.erase(it);
+                if (int rc = admit(e, s, job, off)) return rc;
+                break;
+            }
+        }
+        bool any = false;
+        for (auto &sl : e->slots) any = any || sl.busy;
+        if (!any) break;
+        if (int rc = upload_slots(e)) return rc;
+        // bounded look-ahead: wait for the chunk issued `inflight` chunks ago
+        if (e->chunks >= inflight) {
+            hipEvent_t old = e->ev_chunk[(e->chunks - inflight) % psdm_rtm_engine::kRing];
+            HIP_TRY(c, hipEventSynchronize(old));
+        }
+        // the ring event about to be re-recorded must not still be awaited by a job
+        if (int rc = collect(e, e->chunks - psdm_rtm_engine::kRing)) return rc;
+        launch_resolve(e->d_slots, e->cfg.nslots, e->d_gbase, e->d_cur, c->stream);
+        if (int rc = issue_chunk(e)) return rc;
+        for (auto &sl : e->slots)
+            if (sl.busy) e->active_slot_steps += std::min<long long>(C, sl.g_end - e->G);
+        e->G += C;
+        HIP_TRY(c, hipEventRecord(e->ev_chunk[e->chunks % psdm_rtm_engine::kRing], c->stream));
+        e->chunks++;
+    }
+    if (int rc = collect(e, -1)) return rc;
+    int na = 0;
+    for (auto &sl : e->slots) na += sl.busy;
+    if (n_active) *n_active = na;
+    if (n_pending) *n_pending = (int)e->pending.size();
+    return 0;
+}
+
+int psdm_rtm_engine_poll(psdm_rtm_engine *e, long long *ids, double *t_done, int cap, int *n) {
+    if (!e) return PSDM_RTM_EINVAL;
+    if (int rc = collect(e, -1)) return rc;
+    const int k = std::min<int>(cap, (int)e->done_ready.size());
+    for (int q = 0; q < k; q++) {
+        if (ids) ids[q] = e->done_ready[q].first;
+        if (t_done) t_done[q] = e->done_ready[q].second;
+    }
+    e->done_ready.erase(e->done_ready.begin(), e->done_ready.begin() + k);
+    if (n) *n = k;
+    return 0;
+}
+
+int psdm_rtm_engine_drain(psdm_rtm_engine *e) {
+    if (!e) return PSDM_RTM_EINVAL;
+    int na = 1, np = 1;
+    while (na > 0 || np > 0)
+        if (int rc = psdm_rtm_engine_run(e, 16, &na, &np)) return rc;
+    // retire the last shots and wait for everything
+    if (int rc = psdm_rtm_engine_run(e, 1, &na, &np)) return rc;
+    HIP_TRY(e->c, hipStreamSynchronize(e->c->stream));
+    return collect(e, e->chunks);
+}
+
+int psdm_rtm_engine_image(psdm_rtm_engine *e, double **d_psdm_sum, int **d_cov_sum) {
+    if (!e) return PSDM_RTM_EINVAL;
+    if (d_psdm_sum) *d_psdm_sum = e->d_sum;
+    if (d_cov_sum) *d_cov_sum = e->d_fold;
+    return 0;
+}
+
+int psdm_rtm_engine_stats(psdm_rtm_engine *e, long long *steps_issued, long long *slot_steps, long long *hbm_used,
+                          long long *hbm_budget) {
+    if (!e) return PSDM_RTM_EINVAL;
+    if (steps_issued) *steps_issued = e->G;
+    if (slot_steps) *slot_steps = e->active_slot_steps;
+    if (hbm_used) *hbm_used = (long long)e->arena.in_use();
+    if (hbm_budget) *hbm_budget = (long long)e->arena.size;
+    return 0;
+}
+
+}  // extern "C"
