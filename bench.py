@@ -83,17 +83,117 @@ def accuracy_check(ctx):
             "traces_rel_l2_vs_fp64": rel_l2(Rec, R64)}
 
 
+def model_gathers(ctx, torch, dev, W, geos_model, d_true, d_wat, d_src, nshots):
+    """Observed gathers: modelled on the true and on the water-only model and subtracted
+    (direct wave removed); natural time order, (NumRec, nt) = Data(nt, NumRec)."""
+    from psdm_amd.solvers import forward_device
+
+    shots, nt = W["shots"], W["nt"]
+    nrec = shots[0].nxRec.size
+    d_a = torch.empty((nrec, nt), device=dev)
+    d_b = torch.empty((nrec, nt), device=dev)
+    obs = {}
+    for k in range(nshots):
+        sh = shots[k]
+        forward_device(ctx, geos_model[k], d_true[sh.x_i - 1:sh.x_f], d_src, d_a, key=-1)
+        forward_device(ctx, geos_model[k], d_wat[sh.x_i - 1:sh.x_f], d_src, d_b, key=-1)
+        ctx.sync()
+        obs[k] = (d_a - d_b).contiguous()
+        torch.cuda.synchronize()
+    return obs
+
+
+def run_plain(ctx, torch, dev, W, geos, d_mig, d_src, obs, steps, warmup, timing_every):
+    """One shot per call: forward (snapshots to HBM) + backward + FP64 stack per shot."""
+    from psdm_amd.solvers import backward_device, forward_device
+
+    m, shots, nt = W["model"], W["shots"], W["nt"]
+    ny = m.nz
+    nrec = shots[0].nxRec.size
+    rev = {k: v.flip(1).contiguous() for k, v in obs.items()}  # BackSource = Data(nt:1:-1)
+    d_kernel = torch.zeros((max(s.nx_w for s in shots), ny), device=dev)
+    psdm_sum = torch.zeros((m.nx, ny), dtype=torch.float64, device=dev)
+    fold = torch.zeros((m.nx, ny), dtype=torch.int32, device=dev)
+    d_rec = torch.empty((nrec, nt), device=dev)
+    torch.cuda.synchronize()
+
+    def migrate(k):
+        sh = shots[k % len(shots)]
+        win = d_mig[sh.x_i - 1:sh.x_f]
+        forward_device(ctx, geos[k % len(shots)], win, d_src, d_rec, key=k)
+        backward_device(ctx, geos[k % len(shots)], win, rev[k % len(shots)], d_kernel, key=k)
+        ctx.check(ctx._L.psdm_rtm_stack_image(ctx.h, d_kernel.data_ptr(), ny, sh.nx_w, sh.x_i,
+                                              psdm_sum.data_ptr(), fold.data_ptr(), m.nx), "stack_image")
+
+    log(f"[bench] plain: warmup {warmup} shot(s)")
+    for k in range(warmup):
+        migrate(k)
+    ctx.sync()
+    psdm_sum.zero_()
+    fold.zero_()
+    torch.cuda.synchronize()
+    ctx.set_timing(timing_every)
+    log(f"[bench] plain: timing {steps} shots")
+    t0 = time.perf_counter()
+    for k in range(warmup, warmup + steps):
+        migrate(k)
+    ctx.sync()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    n_ev, mean_ms = ctx.step_stats()
+    ctx.set_timing(0)
+    return dict(elapsed=elapsed, n_ev=n_ev, mean_ms=mean_ms, shots=[(k % len(shots)) for k in range(warmup, warmup + steps)],
+                psdm_sum=psdm_sum, fold=fold)
+
+
+def run_batched(ctx, torch, dev, W, d_mig, d_src, obs, steps, warmup, cfg):
+    """Many shots per launch: the engine keeps `nslots` shots resident."""
+    from psdm_amd.engine import Engine
+
+    m, shots, nt = W["model"], W["shots"], W["nt"]
+    ny = m.nz
+    psdm_sum = torch.zeros((m.nx, ny), dtype=torch.float64, device=dev)
+    fold = torch.zeros((m.nx, ny), dtype=torch.int32, device=dev)
+    torch.cuda.synchronize()
+    eng = Engine(ctx, d_mig, ny, m.nx, nslots=cfg["nslots"], max_nx=max(s.nx_w for s in shots), max_nt=nt,
+                 dPML=W["dPML"], fs=W["fs"], dmodel=m.dx, dt=W["dt"], tile_x=cfg["tile_x"], grid=cfg["grid"],
+                 snap_format=cfg.get("snap_format", "f32"), store_snap=cfg.get("store_snap", 1),
+                 kahan=cfg.get("kahan", False), chunk_steps=cfg.get("chunk_steps", 64), d_psdm_sum=psdm_sum,
+                 d_cov_sum=fold)
+    for k in range(warmup, warmup + steps):  # the shots the plain form times
+        eng.submit(**job_of(W, k % len(shots), d_src, obs))
+    ctx.sync()
+    torch.cuda.synchronize()
+    log(f"[bench] batched: {steps} shots, {cfg}")
+    t0 = time.perf_counter()
+    eng.drain()
+    ctx.sync()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    st = eng.stats()
+    eng.close()
+    return dict(elapsed=elapsed, stats=st, psdm_sum=psdm_sum, fold=fold)
+
+
+def job_of(W, k, d_src, obs):
+    sh = W["shots"][k]
+    return dict(x_i=sh.x_i, nx_w=sh.nx_w, nt=W["nt"], nxSou=sh.nxSou - sh.x_i + 1, nySou=sh.nySou,
+                nxRec=sh.nxRec - sh.x_i + 1, nyRec=sh.nyRec, d_wavelet=d_src[0], d_data=obs[k])
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=64, help="shots timed")
-    ap.add_argument("--warmup", type=int, default=1, help="untimed shots before the timed region")
-    ap.add_argument("--tile-x", type=int, default=8)
+    ap.add_argument("--steps", type=int, default=64, help="shots timed (per form)")
+    ap.add_argument("--warmup", type=int, default=1, help="untimed shots before the plain timed region")
+    ap.add_argument("--tile-x", type=int, default=8, help="plain form tile width")
     ap.add_argument("--timing-every", type=int, default=16, help="sample every n-th step kernel with HIP events")
     ap.add_argument("--cpu-procs", type=int, default=16)
     ap.add_argument("--cpu-steps", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="batched form with the default launch shape")
+    ap.add_argument("--forms", default="plain,batched")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -101,6 +201,7 @@ def main():
               f"(--gpus {args.gpus} is not supported; shots shard across GPUs with no per-step "
               f"communication, see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    forms = set(args.forms.split(","))
 
     from psdm_amd.workloads import line_workload, step_bytes, updates_per_shot
 
@@ -119,115 +220,118 @@ def main():
     import torch
 
     from psdm_amd._native import Context
-    from psdm_amd.solvers import DeviceShot, backward_device, forward_device, ricker_mcs
+    from psdm_amd.solvers import DeviceShot, ricker_mcs
 
     dev = torch.device("cuda:0")
     ctx = Context(0)
     ctx.set_launch(tile_x=args.tile_x)
-    d_true = torch.from_numpy(np.ascontiguousarray(m.true.T)).to(dev)        # (nx, ny) = Fortran (ny, nx)
+    d_true = torch.from_numpy(np.ascontiguousarray(m.true.T)).to(dev)  # (nx, ny) = Fortran (ny, nx)
     d_mig = torch.from_numpy(np.ascontiguousarray(m.migration.T)).to(dev)
     d_wat = torch.from_numpy(np.ascontiguousarray(m.water.T)).to(dev)
     src = ricker_mcs(nt, dt, W["f0"], 1.2 / W["f0"])
     d_src = torch.from_numpy(src.reshape(1, nt)).to(dev)
-    nrec = shots[0].nxRec.size
     torch.cuda.synchronize()
 
-    nshots = max(args.steps + args.warmup, 1)
+    nshots = min(max(args.steps + args.warmup, 1), len(shots))
     geos, geos_model = [], []
     for k in range(nshots):
-        sh = shots[k % len(shots)]
+        sh = shots[k]
         args_geo = (ny, sh.nx_w, nt, dPML, W["fs"], m.dx, dt, [sh.nxSou - sh.x_i + 1], [sh.nySou],
                     sh.nxRec - sh.x_i + 1, sh.nyRec)
-        geos.append(DeviceShot(*args_geo))               # migration: snapshots every step
+        geos.append(DeviceShot(*args_geo))                   # migration: snapshots every step
         geos_model.append(DeviceShot(*args_geo, nt_snap=0))  # data modelling: no snapshots
+    log(f"[bench] modelling {nshots} gathers")
+    obs = model_gathers(ctx, torch, dev, W, geos_model, d_true, d_wat, d_src, nshots)
 
-    # observed gathers: model on the true and the water-only model, subtract (direct wave
-    # removed), reverse in time (BackSource, calculate_PSDM_RTM.f90:448-450)
-    log(f"[bench] modelling {min(nshots, len(shots))} gathers")
-    gathers = {}
-    d_a = torch.empty((nrec, nt), device=dev)
-    d_b = torch.empty((nrec, nt), device=dev)
-    for k in range(min(nshots, len(shots))):
-        sh = shots[k]
-        forward_device(ctx, geos_model[k], d_true[sh.x_i - 1:sh.x_f], d_src, d_a, key=-1)
-        forward_device(ctx, geos_model[k], d_wat[sh.x_i - 1:sh.x_f], d_src, d_b, key=-1)
-        ctx.sync()
-        gathers[k] = (d_a - d_b).flip(1).contiguous()
-        torch.cuda.synchronize()
-
-    d_kernel = torch.zeros((max(s.nx_w for s in shots), ny), device=dev)
-    psdm_sum = torch.zeros((m.nx, ny), dtype=torch.float64, device=dev)
-    fold = torch.zeros((m.nx, ny), dtype=torch.int32, device=dev)
-    d_rec = torch.empty((nrec, nt), device=dev)
-    torch.cuda.synchronize()
-
-    def migrate(k):
-        sh = shots[k % len(shots)]
-        win = d_mig[sh.x_i - 1:sh.x_f]
-        forward_device(ctx, geos[k], win, d_src, d_rec, key=k)
-        backward_device(ctx, geos[k], win, gathers[k % len(shots)], d_kernel, key=k)
-        ctx.check(ctx._L.psdm_rtm_stack_image(ctx.h, d_kernel.data_ptr(), ny, sh.nx_w, sh.x_i,
-                                              psdm_sum.data_ptr(), fold.data_ptr(), m.nx), "stack_image")
-
-    log(f"[bench] warmup {args.warmup} shot(s)")
-    for k in range(args.warmup):
-        migrate(k)
-    ctx.sync()
-    torch.cuda.synchronize()
-    psdm_sum.zero_()
-    fold.zero_()
-    torch.cuda.synchronize()
-
-    ctx.set_timing(args.timing_every)
-    log(f"[bench] timing {args.steps} shots")
-    t0 = time.perf_counter()
-    for k in range(args.warmup, args.warmup + args.steps):
-        migrate(k)
-    ctx.sync()
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    n_ev, mean_ms = ctx.step_stats()
-    elapsed = t1 - t0
-
-    upd = sum(updates_per_shot(ny, shots[k % len(shots)].nx_w, nt, dPML)
-              for k in range(args.warmup, args.warmup + args.steps))
-    shots_per_s = args.steps / elapsed
-    gupd = upd / elapsed / 1e9
     bfw = step_bytes(ny, nxw, dPML, 4.0, 1, backward=False)
     bbw = step_bytes(ny, nxw, dPML, 4.0, 1, backward=True)
-    step_b = 0.5 * (bfw + bbw)
-    achieved = step_b / (mean_ms * 1e-3) / 1e9 if mean_ms > 0 else None
+    step_b = 0.5 * (bfw + bbw)  # algorithmic bytes of one shot-step (FP32 snapshot every step)
+    upd_shot = updates_per_shot(ny, nxw, nt, dPML)
+
+    plain = batched = None
+    if "plain" in forms:
+        P = run_plain(ctx, torch, dev, W, geos, d_mig, d_src, obs, args.steps, args.warmup, args.timing_every)
+        ach = step_b / (P["mean_ms"] * 1e-3) / 1e9 if P["mean_ms"] > 0 else None
+        plain = {"shots_per_s": round(args.steps / P["elapsed"], 4),
+                 "gpt_updates_per_s": round(args.steps * upd_shot / P["elapsed"] / 1e9, 3),
+                 "ms_per_shot": round(1e3 * P["elapsed"] / args.steps, 3),
+                 "roofline": {"bound": "hbm", "achieved": round(ach, 1) if ach else None, "peak": HBM_PEAK_GBS,
+                              "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4) if ach else None,
+                              "kernel": "rtm_step_kernel", "mean_kernel_ms": round(P["mean_ms"], 5),
+                              "sampled_launches": P["n_ev"], "algorithmic_bytes_per_launch": round(step_b)}}
+        log(f"[bench] plain {plain['shots_per_s']} shots/s")
+
+    if "batched" in forms:
+        cfg = dict(nslots=16, tile_x=8, grid=0, chunk_steps=64)
+        tuned = None
+        if not args.no_tune:
+            from psdm_amd.autotune import tune
+
+            jobs = [job_of(W, k, d_src, obs) for k in range(nshots)]
+            best, trials = tune(ctx, d_mig, ny, m.nx, jobs, max(s.nx_w for s in shots), nt, dPML, W["fs"], m.dx, dt,
+                                slots=(8, 16), tiles=(8, 16), grids=(0, 2048), log=log)
+            cfg.update(nslots=best["nslots"], tile_x=best["tile_x"], grid=best["grid"])
+            tuned = {"best": best, "trials": trials}
+        B = run_batched(ctx, torch, dev, W, d_mig, d_src, obs, args.steps, args.warmup, cfg)
+        st = B["stats"]
+        per_launch_ms = 1e3 * B["elapsed"] / max(st["steps_issued"], 1)
+        bytes_launch = step_b * st["slot_steps"] / max(st["steps_issued"], 1)
+        ach = bytes_launch / (per_launch_ms * 1e-3) / 1e9
+        batched = {"shots_per_s": round(args.steps / B["elapsed"], 4),
+                   "gpt_updates_per_s": round(args.steps * upd_shot / B["elapsed"] / 1e9, 3),
+                   "config": cfg, "autotune": tuned,
+                   "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "rtm_batch_kernel",
+                                "mean_kernel_ms": round(per_launch_ms, 5),
+                                "algorithmic_bytes_per_launch": round(bytes_launch),
+                                "note": "per-launch time = timed region / launches (includes graph-node gaps)"}}
+        if plain is not None:
+            a = B["psdm_sum"].cpu().numpy()
+            b = P["psdm_sum"].cpu().numpy()
+            batched["stack_rel_diff_vs_plain"] = float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+            batched["fold_equal"] = bool(torch.equal(B["fold"], P["fold"]))
+        log(f"[bench] batched {batched['shots_per_s']} shots/s")
+
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "step_kernel_mcs2001_traffic.json")
+    tpath = os.path.join(ROOT, "profiles", "traffic_rtm_step_kernel_mcs2001.md")
     if os.path.exists(tpath):
-        traffic = json.load(open(tpath)).get("bytes_per_launch")
+        import re
+
+        mt = re.search(r"^traffic_bytes_per_launch = (\d+)", open(tpath).read(), re.M)
+        traffic = int(mt.group(1)) if mt else None
 
     acc = None
     if not args.no_accuracy:
         log("[bench] accuracy check vs FP64 oracle")
         acc = accuracy_check(ctx)
 
+    head = batched or plain
+    form = "batched" if batched else "plain"
+    roof = dict(head["roofline"])
+    if form == "plain":
+        roof["traffic"] = traffic
     out = {
         "metric": "migrated shots/s + Gpt-updates/s, batched vs plain, at rel-L2<=1e-4; p99 latency",
-        "value": round(shots_per_s, 4),
+        "value": head["shots_per_s"],
         "unit": "shots/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(1e3 * elapsed / args.steps, 3),
+        "ms_per_step": round(1e3 / head["shots_per_s"], 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded 2001x601 model; gathers modelled on device, direct wave removed)",
         "config": {"workload": "mcs2001 line: 2001x601 @10 m, 64 shots @25 m, 480 ch @12.5 m, near offset 150 m, "
-                               "dt 0.8 ms, nt 6000, dPML 40, plain one-shot-per-call form, FP32 snapshots every step",
-                   "form": "plain", "window": f"{ny}x{nxw}", "tile_x": args.tile_x},
-        "gpt_updates_per_s": round(gupd, 3),
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1) if achieved else None, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
-                     "traffic": traffic, "kernel": "rtm_step_kernel", "mean_kernel_ms": round(mean_ms, 5),
-                     "sampled_launches": n_ev, "algorithmic_bytes_per_launch": round(step_b)},
+                               "dt 0.8 ms, nt 6000, dPML 40, FP32 snapshots every step",
+                   "form": form, "window": f"{ny}x{nxw}"},
+        "gpt_updates_per_s": head["gpt_updates_per_s"],
+        "roofline": roof,
+        "plain": plain,
+        "batched": batched,
+        "speedup_batched_vs_plain": (round(batched["shots_per_s"] / plain["shots_per_s"], 3)
+                                     if batched and plain else None),
         "cpu_baseline": cpu,
         "accuracy": acc,
     }

@@ -1,0 +1,53 @@
+"""Autotuner of the batched engine's launch shape.
+
+For a workload (model, shot windows, record length) it times short runs of the
+engine for each candidate (tile width, grid shape, resident slots) and keeps the one
+with the lowest time per slot-step.  A trial admits `nslots` shots, lets one chunk run
+(admission and first-step effects), then times `chunks` more chunks; nothing is
+drained, so a trial costs a few hundred time steps.
+"""
+from __future__ import annotations
+
+import itertools
+import time
+
+from .engine import Engine
+
+
+def _trial(ctx, d_vel, ny, nx_prop, jobs, nslots, max_nx, nt, dPML, fs, dx, dt, cfg, chunk_steps, chunks):
+    import torch
+
+    eng = Engine(ctx, d_vel, ny, nx_prop, nslots=nslots, max_nx=max_nx, max_nt=nt, dPML=dPML, fs=fs, dmodel=dx, dt=dt,
+                 chunk_steps=chunk_steps, **cfg)
+    try:
+        for q in range(nslots):
+            eng.submit(**jobs[q % len(jobs)])
+        eng.run(1)
+        ctx.sync()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        eng.run(chunks)
+        ctx.sync()
+        t1 = time.perf_counter()
+    finally:
+        eng.close()
+    return (t1 - t0) / (chunks * chunk_steps * nslots)
+
+
+def tune(ctx, d_vel, ny, nx_prop, jobs, max_nx, nt, dPML, fs, dx, dt, snap_format="f32", store_snap=1, kahan=False,
+         slots=(8, 16), tiles=(8, 16), grids=(0, 2048), chunk_steps=32, chunks=3, log=None):
+    """jobs: list of Engine.submit keyword dicts.  Returns (best config dict, trials)."""
+    trials = []
+    for nslots, tile_x, grid in itertools.product(slots, tiles, grids):
+        cfg = dict(tile_x=tile_x, grid=grid, snap_format=snap_format, store_snap=store_snap, kahan=kahan)
+        try:
+            t = _trial(ctx, d_vel, ny, nx_prop, jobs, nslots, max_nx, nt, dPML, fs, dx, dt, cfg, chunk_steps, chunks)
+        except Exception as exc:  # e.g. a slot count whose snapshot stores exceed HBM
+            if log:
+                log(f"[autotune] slots={nslots} tile_x={tile_x} grid={grid}: skipped ({exc})")
+            continue
+        trials.append(dict(nslots=nslots, tile_x=tile_x, grid=grid, us_per_slot_step=1e6 * t))
+        if log:
+            log(f"[autotune] slots={nslots} tile_x={tile_x} grid={grid}: {1e6 * t:.3f} us per slot-step")
+    best = min(trials, key=lambda r: r["us_per_slot_step"])
+    return best, trials

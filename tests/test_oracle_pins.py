@@ -252,3 +252,27 @@ def test_oracle_ram_variant_breaks_mirror_symmetry():
     c = _mirror_case(0)
     R, _, _ = _run(c)
     assert not np.array_equal(R, R[:, ::-1])
+
+
+def test_aligned_decimated_imaging_equals_masked_reference_imaging():
+    """The engine's decimated pairing (backward step k images with the snapshot of forward
+    step nt+2-k, every store_snap-th) has no reference counterpart.  It equals, exactly in
+    FP64 for store_snap = 4 (a power of two), 4 x the reference-bookkeeping imaging at
+    store_snap = 1 with every snapshot the decimated forward would not store set to zero."""
+    c = case("ram_fs0", nt=201)
+    geo = (c["nxSou"], c["nySou"], c["nxRec"], c["nyRec"])
+    nt, ss = c["nt"], 4
+    R, F1 = O.forward(*geo, c["S"], c["dx"], c["dt"], c["ctmp"], 1, nt, nt, c["dPML"], 0, precision="f64")
+    Sadj = np.asfortranarray(R[::-1, :].astype(np.float32))
+    # decimated forward stores F(s') = F1(ss*s'), s' = 1..(nt-1)//ss
+    nsnap = (nt - 1) // ss
+    Rd, Fd = O.forward(*geo, c["S"], c["dx"], c["dt"], c["ctmp"], ss, nsnap, nt, c["dPML"], 0, precision="f64")
+    np.testing.assert_array_equal(Fd, F1[:, :, ss - 1:ss * nsnap:ss])
+    k_aligned = O.backward(*geo, Sadj, c["dx"], c["dt"], c["ctmp"], ss, nsnap, nt, c["dPML"], 0, Fields=Fd,
+                           variant=2, precision="f64")
+    Fm = np.zeros_like(F1)
+    Fm[:, :, ss - 1:ss * nsnap:ss] = F1[:, :, ss - 1:ss * nsnap:ss]
+    k_masked = O.backward(*geo, Sadj, c["dx"], c["dt"], c["ctmp"], 1, nt, nt, c["dPML"], 0, Fields=Fm,
+                          precision="f64")
+    assert np.linalg.norm(k_masked) > 0
+    np.testing.assert_allclose(k_aligned, ss * k_masked, rtol=1e-13, atol=1e-13 * np.abs(k_masked).max())
