@@ -19,9 +19,10 @@ namespace {
 
 // Geometry of one call, 0-based padded-grid quantities.
 struct Layout {
-    int ny, nx, nt, dn, nyt, nxt, ld, fs, ram, ss, nt_snap;
+    int ny, nx, nt, dn, nyt, nxt, ld, lds, fs, ram, ss, nt_snap;
     float dx, dt;
     int tile_x, ntile_y, ntiles, per_xcd;
+    psdm::Tiling T;
 };
 
 }  // namespace
@@ -134,7 +135,7 @@ int check_geom(psdm_rtm_ctx *c, const psdm_rtm_geom *g) {
                     "dPML+6: nx must be >= dPML+6");
     if (g->NumRec < 0 || g->NumSou < 0) return fail(c, PSDM_RTM_EINVAL, "negative point count");
     const long long nyt = g->ny + 2LL * (g->dPML + 6), nxt = g->nx + 2LL * (g->dPML + 6);
-    if (nyt * nxt >= (1LL << 31)) return fail(c, PSDM_RTM_EINVAL, "padded grid exceeds 2^31 cells");
+    if ((nyt + 63) / 64 * 64 * nxt >= (1LL << 31)) return fail(c, PSDM_RTM_EINVAL, "padded grid exceeds 2^31 cells");
     return 0;
 }
 
@@ -147,6 +148,7 @@ Layout make_layout(const psdm_rtm_geom *g, int tile_x) {
     L.nyt = g->ny + 2 * L.dn;
     L.nxt = g->nx + 2 * L.dn;
     L.ld = (L.nyt + 63) / 64 * 64;
+    L.lds = psdm::interior_ld(L.ny);
     L.fs = g->fs != 0;
     L.ram = g->variant == 0;
     L.ss = g->store_snap;
@@ -154,9 +156,9 @@ Layout make_layout(const psdm_rtm_geom *g, int tile_x) {
     L.dx = g->dmodel;
     L.dt = g->dt;
     L.tile_x = tile_x;
-    L.ntile_y = (L.nyt + psdm::kTileY - 1) / psdm::kTileY;
-    const int ntile_x = (L.nxt + tile_x - 1) / tile_x;
-    L.ntiles = L.ntile_y * ntile_x;
+    L.T = psdm::make_tiling(L.ny, L.nx, L.dn, tile_x);
+    L.ntile_y = L.T.nrt;
+    L.ntiles = L.T.ntiles;
     L.per_xcd = (L.ntiles + 7) / 8;
     return L;
 }
@@ -192,10 +194,10 @@ int prepare(psdm_rtm_ctx *c, const Layout &L, const float *d_ctmp, const float *
     P.hs0 = a.chy + n_y;
     P.hs1 = P.hs0 + dn2;
 
-    if (int rc = get_buf(c, "c2", (size_t)L.ny * L.nx * sizeof(float), &v)) return rc;
+    if (int rc = get_buf(c, "c2", (size_t)L.lds * L.nx * sizeof(float), &v)) return rc;
     a.c2 = (float *)v;
-    a.ldc = L.ny;
-    psdm::launch_c2(d_ctmp, (float *)a.c2, L.ny, L.nx, L.ny, L.dt, L.dx, c->stream);
+    a.ldc = L.lds;
+    psdm::launch_c2(d_ctmp, (float *)a.c2, L.ny, L.nx, L.lds, L.dt, L.dx, c->stream);
 
     std::vector<float> fA, fB;
     pml_profile(L.dn - 6, (double)L.dt, (double)L.dx, fA, fB);
@@ -216,7 +218,7 @@ int prepare(psdm_rtm_ctx *c, const Layout &L, const float *d_ctmp, const float *
     std::vector<std::vector<std::pair<int, int>>> per_tile(L.ntiles);
     for (auto &kv : last) {
         const int j = kv.first % L.nyt, i = kv.first / L.nyt;
-        const int t = (i / L.tile_x) * L.ntile_y + j / psdm::kTileY;
+        const int t = psdm::tile_of_cell(L.T, j, i, L.ny, L.nx, L.dn, L.tile_x);
         per_tile[t].push_back(kv);
     }
     std::vector<int> ev_ptr(L.ntiles + 1, 0), ev_cell, ev_inj, cols;
@@ -269,11 +271,16 @@ int prepare(psdm_rtm_ctx *c, const Layout &L, const float *d_ctmp, const float *
     a.ntile_y = L.ntile_y;
     a.ntiles = L.ntiles;
     a.per_xcd = L.per_xcd;
+    a.rt_pml = L.T.rt_pml;
+    a.rt_int = L.T.rt_int;
+    a.ct_pml = L.T.ct_pml;
+    a.ct_int = L.T.ct_int;
     a.fs = L.fs;
     a.ram = L.ram;
     a.dt = L.dt;
     a.ss_f = (float)L.ss;
-    a.snap_stride = (size_t)L.ny * L.nx;
+    a.lds = L.lds;
+    a.snap_stride = (size_t)L.lds * L.nx;
     a.rec_row = -1;
     a.snap_slot = -1;
     return 0;
@@ -374,7 +381,7 @@ int fwd_device(psdm_rtm_ctx *c, const psdm_rtm_geom *g, const float *d_ctmp, con
     st.ny = L.ny;
     st.nx = L.nx;
     st.nslots = nslots;
-    st.bytes = (size_t)nslots * L.ny * L.nx * sizeof(float);
+    st.bytes = (size_t)nslots * L.lds * L.nx * sizeof(float);
     if (int rc = alloc_store(c, st.bytes, &st.d)) return rc;
     c->snaps[key] = st;
     P.a.snaps = st.d;
@@ -393,8 +400,21 @@ int bwd_device(psdm_rtm_ctx *c, const psdm_rtm_geom *g, const float *d_ctmp, con
     if (int rc = prepare(c, L, d_ctmp, d_Sadj, g->NumRec, g->nxRec, g->nyRec, g->NumRec, nullptr, nullptr, 0, P))
         return rc;
     P.a.snaps = it->second.d;
-    P.a.image = d_kernel;
+    // the kernel accumulates into an image with leading dimension lds, seeded from the
+    // caller's kernel(ny,nx) and copied back afterwards
+    float *img = d_kernel;
+    if (L.lds != L.ny) {
+        void *v;
+        if (int rc = get_buf(c, "img", (size_t)L.lds * L.nx * sizeof(float), &v)) return rc;
+        img = (float *)v;
+        HIP_TRY(c, hipMemcpy2DAsync(img, (size_t)L.lds * sizeof(float), d_kernel, (size_t)L.ny * sizeof(float),
+                                    (size_t)L.ny * sizeof(float), L.nx, hipMemcpyDeviceToDevice, c->stream));
+    }
+    P.a.image = img;
     if (int rc = run_loop(c, L, P, 1, it->second.nslots, nullptr)) return rc;
+    if (img != d_kernel)
+        HIP_TRY(c, hipMemcpy2DAsync(d_kernel, (size_t)L.ny * sizeof(float), img, (size_t)L.lds * sizeof(float),
+                                    (size_t)L.ny * sizeof(float), L.nx, hipMemcpyDeviceToDevice, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     return free_store(c, key);
 }
@@ -445,8 +465,10 @@ int host_forward(psdm_rtm_ctx *c, const psdm_rtm_geom &g, const float *S, const 
         SnapStore &st = c->snaps[key];
         if (st.nslots > 0) {
             if (!Fields) return fail(c, PSDM_RTM_EINVAL, "null Fields with host_fields");
-            HIP_TRY(c, hipMemcpyAsync(Fields, st.d, (size_t)st.nslots * nv * sizeof(float), hipMemcpyDeviceToHost,
-                                      c->stream));
+            // Fields(ny, nx, nt_snap) <- snapshots with leading dimension lds
+            HIP_TRY(c, hipMemcpy2DAsync(Fields, (size_t)g.ny * sizeof(float), st.d,
+                                        (size_t)psdm::interior_ld(g.ny) * sizeof(float), (size_t)g.ny * sizeof(float),
+                                        (size_t)st.nslots * g.nx, hipMemcpyDeviceToHost, c->stream));
         }
         HIP_TRY(c, hipStreamSynchronize(c->stream));
         return free_store(c, key);
@@ -479,11 +501,13 @@ int host_backward(psdm_rtm_ctx *c, const psdm_rtm_geom &g, const float *Sadj, co
         st.nslots = g.nt_snap;
         if (st.nslots > 0) {
             if (!Fields) return fail(c, PSDM_RTM_EINVAL, "null Fields with host_fields");
-            st.bytes = (size_t)st.nslots * nv * sizeof(float);
+            const size_t lds = psdm::interior_ld(g.ny);
+            st.bytes = (size_t)st.nslots * lds * g.nx * sizeof(float);
             if (int rc = alloc_store(c, st.bytes, &st.d)) return rc;
             c->snaps[key] = st;
-            HIP_TRY(c, hipMemcpyAsync(st.d, Fields, (size_t)st.nslots * nv * sizeof(float), hipMemcpyHostToDevice,
-                                      c->stream));
+            HIP_TRY(c, hipMemcpy2DAsync(st.d, lds * sizeof(float), Fields, (size_t)g.ny * sizeof(float),
+                                        (size_t)g.ny * sizeof(float), (size_t)st.nslots * g.nx, hipMemcpyHostToDevice,
+                                        c->stream));
         } else {
             c->snaps[key] = st;
         }
@@ -602,7 +626,7 @@ int psdm_rtm_stack_image(psdm_rtm_ctx *c, float *d_kernel, int ny, int nx_w, int
     if (ny < 1 || nx_w < 1 || x_i < 1 || x_i + nx_w - 1 > nx_prop)
         return fail(c, PSDM_RTM_EINVAL, "window [x_i, x_i+nx_w-1] outside 1..nx_prop");
     if (int rc = set_device(c)) return rc;
-    psdm::launch_stack(d_kernel, nullptr, ny, nx_w, x_i - 1, d_psdm_sum, ny, d_cov_sum, c->stream);
+    psdm::launch_stack(d_kernel, nullptr, ny, ny, nx_w, x_i - 1, d_psdm_sum, ny, d_cov_sum, c->stream);
     HIP_TRY(c, hipGetLastError());
     return 0;
 }

@@ -31,9 +31,75 @@ enum SnapFmt { kSnapF32 = 0, kSnapF16 = 1, kSnapBF16 = 2 };
 //   c2          the unpadded window c2w[(i-dn)*ldc + (j-dn)], read with clamped
 //               indices: the reference replicates edge velocities into the pads
 //   snapshots   F[s][i][j] over the interior (ny x nx), s = snapshot slot
+// Region-aligned tiling: tile rows cover the top CPML [0,dn), the interior [dn,dn+ny)
+// and the bottom CPML separately (64 rows each, the last one of a region partial);
+// tile columns likewise with TX columns.  Every tile is then wholly interior, x strip,
+// y strip or corner.  Tiles are numbered column-major: t = C * nrt + R.
+struct Tiling {
+    int rt_pml, rt_int, ct_pml, ct_int, nrt, nct, ntiles;
+};
+
+__host__ __device__ inline int cdiv(int a, int b) { return (a + b - 1) / b; }
+// Leading dimension of the interior arrays (snapshots, image, c2): whole 64-float
+// column segments, so a wavefront's 64 rows are one aligned 256-byte access.
+__host__ __device__ inline int interior_ld(int ny) { return cdiv(ny, 64) * 64; }
+
+__host__ __device__ inline Tiling make_tiling(int ny, int nx, int dn, int tx) {
+    Tiling T;
+    T.rt_pml = cdiv(dn, kTileY);
+    T.rt_int = cdiv(ny, kTileY);
+    T.ct_pml = cdiv(dn, tx);
+    T.ct_int = cdiv(nx, tx);
+    T.nrt = 2 * T.rt_pml + T.rt_int;
+    T.nct = 2 * T.ct_pml + T.ct_int;
+    T.ntiles = T.nrt * T.nct;
+    return T;
+}
+
+// Region (0 interior, 1 low-side CPML, 2 high-side CPML) and [lo, hi) of tile index r
+// along one axis with `npml` CPML tiles on each side sharing its `n_pml` cells evenly
+// (so every CPML tile spans at least 4 cells when n_pml >= 7 and w >= 8: the clamped
+// edge stencils then stay inside the tile's staged halo), and `nint` interior tiles of
+// `w` cells over `n_int` cells.
+__host__ __device__ inline int axis_tile(int r, int npml, int nint, int n_pml, int n_int, int w, int *lo, int *hi) {
+    if (r < npml) {
+        *lo = r * n_pml / npml;
+        *hi = (r + 1) * n_pml / npml;
+        return 1;
+    }
+    r -= npml;
+    if (r < nint) {
+        *lo = n_pml + r * w;
+        *hi = *lo + w < n_pml + n_int ? *lo + w : n_pml + n_int;
+        return 0;
+    }
+    r -= nint;
+    *lo = n_pml + n_int + r * n_pml / npml;
+    *hi = n_pml + n_int + (r + 1) * n_pml / npml;
+    return 2;
+}
+
+// Tile index along one axis of cell c (inverse of axis_tile).
+__host__ __device__ inline int axis_tile_of(int c, int npml, int n_pml, int n_int, int w) {
+    if (c >= n_pml && c < n_pml + n_int) return npml + (c - n_pml) / w;
+    const int side = c < n_pml ? 0 : 1;
+    const int cl = side ? c - n_pml - n_int : c;
+    int r = 0;
+    while ((r + 1) * n_pml / npml <= cl) ++r;
+    return side ? npml + cdiv(n_int, w) + r : r;
+}
+
+// Tile index of padded cell (j, i).
+__host__ __device__ inline int tile_of_cell(const Tiling &T, int j, int i, int ny, int nx, int dn, int tx) {
+    const int R = axis_tile_of(j, T.rt_pml, dn, ny, kTileY);
+    const int C = axis_tile_of(i, T.ct_pml, dn, nx, tx);
+    return C * T.nrt + R;
+}
+
 struct StepArgs {
     int ny, nx, dn, nyt, nxt, ld;
-    int ntile_y, ntiles, per_xcd;
+    int ntile_y, ntiles, per_xcd;  // ntile_y = Tiling::nrt
+    int rt_pml, rt_int, ct_pml, ct_int;
     int fs, ram;      // free surface flag; ram = 1 -> _ram semantics, 0 -> _disk
     int k;            // time step index of the reference loop, 2..nt
     int mode;         // 0 forward (snapshots), 1 backward (imaging)
@@ -60,9 +126,10 @@ struct StepArgs {
     int ninj;
     void *snaps;           // forward: snapshot base (format snap_fmt); backward: read
     float *snap_scale;     // FP16/BF16: scale[slot * ntiles + tile]
-    float *image;          // backward: kernel(ny,nx), ld = ny
+    float *image;          // backward: kernel(ny,nx), leading dimension lds
     float *image_comp;     // Kahan compensation (kahan != 0)
-    size_t snap_stride;    // elements per snapshot (ny*nx)
+    int lds;               // leading dimension of snapshots, image and c2: ny rounded up to 64
+    size_t snap_stride;    // elements per snapshot (lds*nx)
     const int *rec_cell;   // receivers: cell j + i*ld, or -1 outside the grid
     float *rec;            // Rec_point(nt, NumRec) column-major, leading dimension rec_ld = nt
     int rec_ld;
@@ -103,7 +170,7 @@ void launch_resolve(const SlotDesc *slots, int nslots, const long long *gbase, S
 void launch_advance(long long *gbase, long long by, hipStream_t s);
 void launch_record(const float *p, const int *rec_cell, float *rec, int rec_ld, int row, int nrec, hipStream_t s);
 void launch_c2(const float *vel, float *c2, int ny, int nx, int ldc, float dt, float dx, hipStream_t s);
-void launch_stack(float *img, float *comp, int ny, int nx_w, int x0, double *sum, int ld_sum, int *fold,
+void launch_stack(float *img, float *comp, int ld_img, int ny, int nx_w, int x0, double *sum, int ld_sum, int *fold,
                   hipStream_t s);
 void launch_normalize(const double *sum, const int *fold, float *out, size_t n, hipStream_t s);
 // out[r*ncols + e] = traces[cols[e]*nt + (reverse ? nt-1-r : r)], r in [0, nt-1)

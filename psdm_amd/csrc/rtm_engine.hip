@@ -104,7 +104,7 @@ double now_s() {
 struct psdm_rtm_engine {
     psdm_rtm_ctx *c = nullptr;
     psdm_rtm_engine_cfg cfg{};
-    int dn = 0, ld = 0, nyt = 0, nxt_max = 0, ntile_y = 0, cap_tiles = 0, nwork = 0, per_xcd = 0, grid = 0;
+    int dn = 0, ld = 0, lds = 0, nyt = 0, nxt_max = 0, ntile_y = 0, cap_tiles = 0, nwork = 0, per_xcd = 0, grid = 0;
     size_t n_p = 0, n_x = 0, n_y = 0, state_stride = 0, esize = 4;
     // device
     SlotDesc *d_slots = nullptr;
@@ -173,8 +173,8 @@ int capture_graph(psdm_rtm_engine *e) {
 
 size_t snap_bytes_of(const psdm_rtm_engine *e, int nx_w, int nt) {
     const int nsnap = (nt - 1) / e->cfg.store_snap;
-    const int ntiles = e->ntile_y * ((nx_w + 2 * e->dn + e->cfg.tile_x - 1) / e->cfg.tile_x);
-    size_t b = (size_t)nsnap * e->cfg.ny * nx_w * e->esize;
+    const int ntiles = make_tiling(e->cfg.ny, nx_w, e->dn, e->cfg.tile_x).ntiles;
+    size_t b = (size_t)nsnap * e->lds * nx_w * e->esize;
     b = (b + 255) & ~(size_t)255;
     if (e->cfg.snap_format != kSnapF32) b += (size_t)nsnap * ntiles * sizeof(float);
     return b;
@@ -186,7 +186,8 @@ int admit(psdm_rtm_engine *e, int s, Job &job, size_t arena_off) {
     const psdm_rtm_job &J = job.j;
     const int ny = e->cfg.ny, nx = J.nx_w, nt = J.nt, dn = e->dn, TX = e->cfg.tile_x;
     const int nxt = nx + 2 * dn;
-    const int ntiles = e->ntile_y * ((nxt + TX - 1) / TX);
+    const Tiling T = make_tiling(ny, nx, dn, TX);
+    const int ntiles = T.ntiles;
     Slot &sl = e->slots[s];
 
     // injection events: forward source (1 point), backward receivers (last writer wins)
@@ -201,7 +202,7 @@ int admit(psdm_rtm_engine *e, int s, Job &job, size_t arena_off) {
         std::vector<std::vector<std::pair<int, int>>> per(ntiles);
         for (auto &kv : last) {
             const int j = kv.first % e->nyt, i = kv.first / e->nyt;
-            per[(i / TX) * e->ntile_y + j / kTileY].push_back(kv);
+            per[tile_of_cell(T, j, i, ny, nx, dn, TX)].push_back(kv);
         }
         ptr.assign(ntiles + 1, 0);
         for (int t = 0; t < ntiles; t++) {
@@ -243,8 +244,8 @@ int admit(psdm_rtm_engine *e, int s, Job &job, size_t arena_off) {
     float *inj_f = sl.d_inj, *inj_b = sl.d_inj + (size_t)(nt - 1) * ninj_f;
     launch_gather(J.d_wavelet, nt, sl.d_ints + ofcol, ninj_f, inj_f, 0, c->stream);
     launch_gather(J.d_data, nt, sl.d_ints + obcol, ninj_b, inj_b, 1, c->stream);
-    float *c2 = e->d_c2 + (size_t)s * ny * e->cfg.max_nx;
-    launch_c2(e->cfg.d_vel + (size_t)(J.x_i - 1) * ny, c2, ny, nx, ny, e->cfg.dt, e->cfg.dmodel, c->stream);
+    float *c2 = e->d_c2 + (size_t)s * e->lds * e->cfg.max_nx;
+    launch_c2(e->cfg.d_vel + (size_t)(J.x_i - 1) * ny, c2, ny, nx, e->lds, e->cfg.dt, e->cfg.dmodel, c->stream);
     // the host vectors above are read by the queued copy: wait before they go away
     HIP_TRY(c, hipStreamSynchronize(c->stream));
 
@@ -257,9 +258,13 @@ int admit(psdm_rtm_engine *e, int s, Job &job, size_t arena_off) {
     a.nyt = e->nyt;
     a.nxt = nxt;
     a.ld = e->ld;
-    a.ntile_y = e->ntile_y;
+    a.ntile_y = T.nrt;
     a.ntiles = ntiles;
     a.per_xcd = 0;
+    a.rt_pml = T.rt_pml;
+    a.rt_int = T.rt_int;
+    a.ct_pml = T.ct_pml;
+    a.ct_int = T.ct_int;
     a.fs = e->cfg.fs != 0;
     a.ram = e->cfg.variant == 0;
     a.snap_fmt = e->cfg.snap_format;
@@ -267,16 +272,17 @@ int admit(psdm_rtm_engine *e, int s, Job &job, size_t arena_off) {
     a.dt = e->cfg.dt;
     a.ss_f = (float)e->cfg.store_snap;
     a.c2 = c2;
-    a.ldc = ny;
+    a.ldc = e->lds;
+    a.lds = e->lds;
     a.fA = e->d_tabs;
     a.fB = e->d_tabs + dn;
     a.snaps = e->d_arena + arena_off;
     const int nsnap = (nt - 1) / e->cfg.store_snap;
-    const size_t data_b = ((size_t)nsnap * ny * nx * e->esize + 255) & ~(size_t)255;
+    const size_t data_b = ((size_t)nsnap * e->lds * nx * e->esize + 255) & ~(size_t)255;
     a.snap_scale = e->cfg.snap_format != kSnapF32 ? (float *)(e->d_arena + arena_off + data_b) : nullptr;
-    a.image = e->d_img + (size_t)s * ny * e->cfg.max_nx;
-    a.image_comp = e->d_comp + (size_t)s * ny * e->cfg.max_nx;
-    a.snap_stride = (size_t)ny * nx;
+    a.image = e->d_img + (size_t)s * e->lds * e->cfg.max_nx;
+    a.image_comp = e->d_comp + (size_t)s * e->lds * e->cfg.max_nx;
+    a.snap_stride = (size_t)e->lds * nx;
     a.rec_row = -1;
     float *st = e->d_state + (size_t)s * e->state_stride;
     d.pA = st;
@@ -317,12 +323,13 @@ int retire(psdm_rtm_engine *e, int s) {
     Slot &sl = e->slots[s];
     const psdm_rtm_job &J = sl.job.j;
     const int ny = e->cfg.ny;
-    float *img = e->d_img + (size_t)s * ny * e->cfg.max_nx;
-    float *comp = e->d_comp + (size_t)s * ny * e->cfg.max_nx;
+    float *img = e->d_img + (size_t)s * e->lds * e->cfg.max_nx;
+    float *comp = e->d_comp + (size_t)s * e->lds * e->cfg.max_nx;
     if (J.d_image_out)
-        HIP_TRY(c, hipMemcpyAsync(J.d_image_out, img, (size_t)ny * J.nx_w * sizeof(float), hipMemcpyDeviceToDevice,
-                                  c->stream));
-    launch_stack(img, e->cfg.kahan ? comp : nullptr, ny, J.nx_w, J.x_i - 1, e->d_sum, ny, e->d_fold, c->stream);
+        HIP_TRY(c, hipMemcpy2DAsync(J.d_image_out, (size_t)ny * sizeof(float), img, (size_t)e->lds * sizeof(float),
+                                    (size_t)ny * sizeof(float), J.nx_w, hipMemcpyDeviceToDevice, c->stream));
+    launch_stack(img, e->cfg.kahan ? comp : nullptr, e->lds, ny, J.nx_w, J.x_i - 1, e->d_sum, ny, e->d_fold,
+                 c->stream);
     e->arena.release(sl.arena_off);
     const int C = e->cfg.chunk_steps;
     e->done_wait.push_back({sl.job.id, (sl.g_end - 1) / C});
@@ -396,9 +403,13 @@ int psdm_rtm_engine_create(psdm_rtm_ctx *c, const psdm_rtm_engine_cfg *cfg, psdm
     e->dn = cfg->dPML + 6;
     e->nyt = cfg->ny + 2 * e->dn;
     e->ld = (e->nyt + 63) / 64 * 64;
+    e->lds = psdm::interior_ld(cfg->ny);
     e->nxt_max = cfg->max_nx + 2 * e->dn;
-    e->ntile_y = (e->nyt + kTileY - 1) / kTileY;
-    e->cap_tiles = e->ntile_y * ((e->nxt_max + cfg->tile_x - 1) / cfg->tile_x);
+    {
+        const Tiling T = make_tiling(cfg->ny, cfg->max_nx, e->dn, cfg->tile_x);
+        e->ntile_y = T.nrt;
+        e->cap_tiles = T.ntiles;
+    }
     e->nwork = cfg->nslots * e->cap_tiles;
     e->per_xcd = (e->nwork + 7) / 8;
     e->grid = cfg->grid > 0 ? std::min(cfg->grid, e->per_xcd * 8) : e->per_xcd * 8;
@@ -408,7 +419,7 @@ int psdm_rtm_engine_create(psdm_rtm_ctx *c, const psdm_rtm_engine_cfg *cfg, psdm
     e->n_x = (size_t)e->ld * dn2;
     e->n_y = dn2 * e->nxt_max;
     e->state_stride = (2 * e->n_p + 3 * e->n_x + 3 * e->n_y + 2 * dn2 + 63) / 64 * 64;
-    const size_t S = cfg->nslots, win = (size_t)cfg->ny * cfg->max_nx;
+    const size_t S = cfg->nslots, win = (size_t)psdm::interior_ld(cfg->ny) * cfg->max_nx;
     size_t budget = cfg->hbm_budget > 0 ? (size_t)cfg->hbm_budget : S * snap_bytes_of(e, cfg->max_nx, cfg->max_nt);
     auto bail = [&](int rc) {
         psdm_rtm_engine_destroy(e);

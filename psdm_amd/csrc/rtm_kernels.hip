@@ -27,19 +27,7 @@ __device__ __forceinline__ float c2_at(const StepArgs &a, int j, int i) {
     const bool pad_row = j < a.dn || j >= a.dn + a.ny;
     if (a.ram && pad_row && iw >= a.dn && iw < a.nx) return 0.f;
     const int jw = clampi(j - a.dn, 0, a.ny - 1);
-    return a.c2[(size_t)clampi(iw, 0, a.nx - 1) * a.ldc + jw];
-}
-
-// Source term at an interior cell (injection :348-358; duplicates resolved on the host
-// to the last writer).
-__device__ __forceinline__ bool source_at(const StepArgs &a, int ev_s, int ev_e, int cell, float &src) {
-    bool has = false;
-    for (int e = ev_s; e < ev_e; ++e)
-        if (a.ev_cell[e] == cell) {
-            src = a.inj_val[(size_t)(a.k - 2) * a.ninj + a.ev_inj[e]];
-            has = true;
-        }
-    return has;
+    return ((const float __attribute__((address_space(1))) *)a.c2)[clampi(iw, 0, a.nx - 1) * a.ldc + jw];
 }
 
 __device__ __forceinline__ unsigned short f32_to_bf16(float f) {
@@ -58,212 +46,130 @@ __device__ __forceinline__ float wave_max(float v) {
 
 }  // namespace
 
-// One time step of one 64 x TX tile (tile index t) of one shot.  All threads of the
-// workgroup take part in the barriers.
+// Global-memory views of the step arguments: pointers read from a StepArgs in memory
+// carry no address space of their own, and would otherwise be accessed with flat
+// instructions (which wait on both the LDS and the vector-memory counters).
+template <class T>
+using gptr = T __attribute__((address_space(1))) *;
+template <class T>
+__device__ __forceinline__ gptr<T> gp(T *p) {
+    return (gptr<T>)p;
+}
+
+// Stages p(k-1) rows j0-3..j0+66 and columns i0-3..i0+TX+2 of the padded grid into
+// LDS tile[(i-i0+3)*(TY+6) + (j-j0+3)] (zero outside the grid or when `zero`).
 template <int TX>
-__device__ __forceinline__ void step_tile(const StepArgs &a, int t, float *tile, float *red) {
-    constexpr int TY = kTileY, HY = TY + 6, HX = TX + 6, CPW = TX / 4;
+__device__ __forceinline__ void stage_p(gptr<const float> pc, int ld, int nyt, int nxt, int j0, int i0, bool zero,
+                                        float *tile) {
+    constexpr int HY = kTileY + 6, HX = TX + 6;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int tj = t % a.ntile_y, ti = t / a.ntile_y;
-    const int j0 = tj * TY, i0 = ti * TX;
-    const int dn = a.dn;
-    const int j = j0 + lane;
-    const bool rowok = j < a.nyt;
-    const bool interior_tile = j0 >= dn && j0 + TY <= dn + a.ny && i0 >= dn && i0 + TX <= dn + a.nx;
+#pragma unroll
+    for (int li = w; li < HX; li += 4) {
+        const int gi = i0 - 3 + li;
+        const bool colok = gi >= 0 && gi < nxt && !zero;
+        gptr<const float> col = pc + (colok ? gi : 0) * ld;
+        int gj = j0 - 3 + lane;
+        tile[li * HY + lane] = (colok && gj >= 0 && gj < nyt) ? col[gj] : 0.f;
+        if (lane < 6) {
+            gj = j0 + 61 + lane;
+            tile[li * HY + 64 + lane] = (colok && gj < nyt) ? col[gj] : 0.f;
+        }
+    }
+}
+
+// Interior tile: stencil update, source injection, snapshot store (forward) or
+// zero-lag imaging (backward).
+template <int TX>
+__device__ __forceinline__ void interior_tile(const StepArgs &a, int t, int j0, int j1, int i0, int i1, float *tile,
+                                              float *srcv, float *red) {
+    constexpr int TY = kTileY, HY = TY + 6, CPW = TX / 4;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int dn = a.dn, ld = a.ld, lds = a.lds;
+    const int j = j0 + lane, jw = j - dn;
+    const bool rowok = j < j1;
+    gptr<float> pn = gp(a.pn);
+    gptr<const float> c2 = gp(a.c2);
 
     // loads that do not depend on the staged tile are issued before the barrier
-    float p1v[CPW], c2v[CPW];
+    float p1v[CPW], c2v[CPW], Fv[CPW], imv[CPW], cmv[CPW];
+    const bool imaging = a.mode == 1 && a.snap_slot >= 0;
+    const size_t sbase = (size_t)(a.snap_slot < 0 ? 0 : a.snap_slot) * a.snap_stride;
 #pragma unroll
     for (int q = 0; q < CPW; ++q) {
         const int i = i0 + w * CPW + q;
-        const bool ok = rowok && i < a.nxt;
-        p1v[q] = (ok && !a.zp) ? a.pn[(size_t)i * a.ld + j] : 0.f;
-        c2v[q] = ok ? (interior_tile ? a.c2[(size_t)(i - dn) * a.ldc + (j - dn)] : c2_at(a, j, i)) : 0.f;
-    }
-
-    // stage p(k-1) rows j0-3..j0+66, columns i0-3..i0+TX+2 (zero outside the grid)
-    for (int li = w; li < HX; li += 4) {
-        const int gi = i0 - 3 + li;
-        const bool colok = gi >= 0 && gi < a.nxt && !a.zc;
-        const float *col = a.pc + (size_t)(colok ? gi : 0) * a.ld;
-        int gj = j0 - 3 + lane;
-        tile[li * HY + lane] = (colok && gj >= 0 && gj < a.nyt) ? col[gj] : 0.f;
-        if (lane < 6) {
-            gj = j0 + 61 + lane;
-            tile[li * HY + 64 + lane] = (colok && gj < a.nyt) ? col[gj] : 0.f;
+        const bool ok = rowok && i < i1;
+        const int o = (i - dn) * lds + jw;
+        p1v[q] = (ok && !a.zp) ? pn[i * ld + j] : 0.f;
+        c2v[q] = ok ? c2[(i - dn) * a.ldc + jw] : 0.f;
+        Fv[q] = imv[q] = cmv[q] = 0.f;
+        if (imaging && ok) {
+            if (a.snap_fmt == kSnapF32)
+                Fv[q] = gp((const float *)a.snaps)[sbase + o];
+            else if (a.snap_fmt == kSnapF16)
+                Fv[q] = (float)gp((const _Float16 *)a.snaps)[sbase + o];
+            else
+                Fv[q] = bf16_to_f32(gp((const unsigned short *)a.snaps)[sbase + o]);
+            imv[q] = gp(a.image)[o];
+            if (a.kahan) cmv[q] = gp(a.image_comp)[o];
         }
     }
+    // injection: the tile's events (unique cells, the last writer resolved on the host)
+    // are scattered into a dense LDS array; cells without an event read 0
+    const int ev_s = gp(a.ev_ptr)[t], ev_e = gp(a.ev_ptr)[t + 1];
+    const bool has_src = ev_s < ev_e;
+    if (has_src) {
+#pragma unroll
+        for (int q = 0; q < CPW; ++q) srcv[(w * CPW + q) * TY + lane] = 0.f;
+    }
+    stage_p<TX>(gp(a.pc), ld, a.nyt, a.nxt, j0, i0, a.zc, tile);
     __syncthreads();
+    if (has_src) {
+        for (int e = ev_s + (int)threadIdx.x; e < ev_e; e += kBlock) {
+            const int cell = gp(a.ev_cell)[e];
+            const int cj = cell % a.nyt - j0, ci = cell / a.nyt - i0;
+            srcv[ci * TY + cj] = gp(a.inj_val)[(size_t)(a.k - 2) * a.ninj + gp(a.ev_inj)[e]];
+        }
+        __syncthreads();
+    }
 
-    // d = p(k) - p(k-2) of this thread's interior cells, for the snapshot / imaging pass
+    // d = p(k) - p(k-2) of this thread's cells, for the snapshot / imaging pass
     float dv[CPW];
     bool dok[CPW];
 #pragma unroll
     for (int q = 0; q < CPW; ++q) {
-        dv[q] = 0.f;
-        dok[q] = false;
-    }
-    const int ev_s = a.ev_ptr[t], ev_e = a.ev_ptr[t + 1];
-
-#pragma unroll
-    for (int q = 0; q < CPW; ++q) {
         const int i = i0 + w * CPW + q;
-        if (!rowok || i >= a.nxt) continue;
-        const size_t idx = (size_t)i * a.ld + j;
+        dv[q] = 0.f;
+        dok[q] = rowok && i < i1;
+        if (!dok[q]) continue;
         const float *T = tile + (i - i0 + 3) * HY + (lane + 3);  // T[di*HY + dj]
-        const float p1 = p1v[q], c2 = c2v[q];
-        if (interior_tile) {
-            const float p = T[0];
-            float d2x = (p + p) * kC2_0;
-            d2x = d2x + (T[HY] + T[-HY]) * kC2_1;
-            d2x = d2x + (T[2 * HY] + T[-2 * HY]) * kC2_2;
-            d2x = d2x + (T[3 * HY] + T[-3 * HY]) * kC2_3;
-            float d2y = (p + p) * kC2_0;
-            d2y = d2y + (T[1] + T[-1]) * kC2_1;
-            d2y = d2y + (T[2] + T[-2]) * kC2_2;
-            d2y = d2y + (T[3] + T[-3]) * kC2_3;
-            float lap = d2x + d2y;
-            float src;
-            if (ev_s < ev_e && source_at(a, ev_s, ev_e, j + i * a.nyt, src)) lap = lap + src;
-            float v = (-p1 + 2.f * p) + lap * c2;  // :443-444
-            if (a.fs && j == dn) v = 0.f;          // :463-466
-            a.pn[idx] = v;
-            dv[q] = v - p1;
-            dok[q] = true;
-            continue;
-        }
-
-        // ---- general cell: CPML strips, corners, clamped derivative edges ----
-        const int xs = i < dn ? 1 : (i >= dn + a.nx ? 2 : 0);
-        const int ys = j < dn ? 1 : (j >= dn + a.ny ? 2 : 0);
-        if (a.fs && ys == 1) {  // with a free surface the top CPML is never updated (:391)
-            a.pn[idx] = 0.f;
-            continue;
-        }
-        auto P = [&](int gj, int gi) -> float {
-            const int lj = gj - j0 + 3, li = gi - i0 + 3;
-            if (lj >= 0 && lj < HY && li >= 0 && li < HX) return tile[li * HY + lj];
-            return a.zc ? 0.f : a.pc[(size_t)gi * a.ld + gj];
-        };
-        // derivatives at the clamped column/row (solver_space_derivs_clean.f90:24-34)
-        const int ic = clampi(i, 3, a.nxt - 4), jc = clampi(j, 3, a.nyt - 4);
-        const float px = P(j, ic), py = P(jc, i);
-        const float xm1 = P(j, ic - 1), xp1 = P(j, ic + 1), xm2 = P(j, ic - 2), xp2 = P(j, ic + 2);
-        const float xm3 = P(j, ic - 3), xp3 = P(j, ic + 3);
-        const float ym1 = P(jc - 1, i), yp1 = P(jc + 1, i), ym2 = P(jc - 2, i), yp2 = P(jc + 2, i);
-        const float ym3 = P(jc - 3, i), yp3 = P(jc + 3, i);
-        float d2x = (px + px) * kC2_0;
-        d2x = d2x + (xp1 + xm1) * kC2_1;
-        d2x = d2x + (xp2 + xm2) * kC2_2;
-        d2x = d2x + (xp3 + xm3) * kC2_3;
-        float d2y = (py + py) * kC2_0;
-        d2y = d2y + (yp1 + ym1) * kC2_1;
-        d2y = d2y + (yp2 + ym2) * kC2_2;
-        d2y = d2y + (yp3 + ym3) * kC2_3;
-        const float p = T[0];
-        if (xs == 0 && ys == 0) {
-            float lap = d2x + d2y;
-            float src;
-            if (ev_s < ev_e && source_at(a, ev_s, ev_e, j + i * a.nyt, src)) lap = lap + src;
-            float v = (-p1 + 2.f * p) + lap * c2;
-            if (a.fs && j == dn) v = 0.f;
-            a.pn[idx] = v;
-            dv[q] = v - p1;
-            dok[q] = true;
-            continue;
-        }
-        const size_t ld = (size_t)a.ld;
-        const int dn2 = 2 * dn;
-        const int distx = xs == 0 ? 0 : (xs == 1 ? dn - i : i - (dn + a.nx) + 1);  // 1 next to the interior
-        const int disty = ys == 0 ? 0 : (ys == 1 ? dn - j : j - (dn + a.ny) + 1);
-        // corners use F(max(dist_x, dist_y)) for both directions (solver_PML_coeffs.f90:288-322)
-        const int dist = distx > disty ? distx : disty;
-        const float A = a.fA[dist - 1], B = a.fB[dist - 1];
-        // x-direction memory (strips xl/xr and the corners), pml_matrices_x/_xycorner(1)
-        float propx = 0.f;
-        if (xs != 0) {
-            const int il = xs == 1 ? i : i - (dn + a.nx);  // column inside the strip
-            const int xc = xs == 1 ? i : i - a.nx;         // column of the x memory arrays
-            const int xcc = xc - il + clampi(il, 3, dn - 4);
-            float dphi = 0.f, pho = 0.f, cho = 0.f;
-            const size_t xo = (size_t)xc * ld + j;
-            if (!a.zm) {
-                const float *ph = a.phx_o + j;
-                dphi = (ph[(xcc + 1) * ld] - ph[(xcc - 1) * ld]) * kC1_1;
-                dphi = dphi + (ph[(xcc + 2) * ld] - ph[(xcc - 2) * ld]) * kC1_2;
-                dphi = dphi + (ph[(xcc + 3) * ld] - ph[(xcc - 3) * ld]) * kC1_3;
-                pho = a.phx_o[xo];
-                cho = a.chx[xo];
-            }
-            float d1x = (xp1 - xm1) * kC1_1;
-            d1x = d1x + (xp2 - xm2) * kC1_2;
-            d1x = d1x + (xp3 - xm3) * kC1_3;
-            const float phn = A * pho + B * d1x;
-            const float chn = A * cho + B * (d2x + dphi);
-            a.phx_n[xo] = phn;
-            a.chx[xo] = chn;
-            propx = dphi + chn;
-        }
-        float propy = 0.f;
-        if (ys != 0) {
-            const int jl = ys == 1 ? j : j - (dn + a.ny);
-            const int yr = ys == 1 ? j : j - a.ny;  // row of the y memory arrays
-            const int yrc = yr - jl + clampi(jl, 3, dn - 4);
-            float dphi = 0.f, pho = 0.f, cho = 0.f;
-            const size_t yo = (size_t)i * dn2 + yr;
-            if (!a.zm) {
-                const float *ph = a.phy_o + (size_t)i * dn2;
-                dphi = (ph[yrc + 1] - ph[yrc - 1]) * kC1_1;
-                dphi = dphi + (ph[yrc + 2] - ph[yrc - 2]) * kC1_2;
-                dphi = dphi + (ph[yrc + 3] - ph[yrc - 3]) * kC1_3;
-                pho = a.phy_o[yo];
-                cho = a.chy[yo];
-            }
-            float d1y = (yp1 - ym1) * kC1_1;
-            d1y = d1y + (yp2 - ym2) * kC1_2;
-            d1y = d1y + (yp3 - ym3) * kC1_3;
-            const float phn = A * pho + B * d1y;
-            const float chn = A * cho + B * (d2y + dphi);
-            a.phy_n[yo] = phn;
-            a.chy[yo] = chn;
-            propy = dphi + chn;
-        }
-        float v;
-        if (ys == 0) {
-            // x strip, interior rows (:376-382).  With a free surface the _ram form
-            // zeroes row dn+1 of the grid but the strip keeps its own history there.
-            const bool own = a.ram && a.fs && j == dn;
-            const int hsi = xs == 1 ? i : i - a.nx;
-            const float ps = own ? (a.zc ? 0.f : a.hs_c[hsi]) : p;
-            const float p1s = own ? (a.zp ? 0.f : a.hs_n[hsi]) : p1;
-            v = (-p1s + 2.f * ps) + c2 * ((d2x + d2y) + propx);
-            if (own) {
-                a.hs_n[hsi] = v;
-                v = 0.f;
-            }
-        } else if (xs == 0) {
-            v = (-p1 + 2.f * p) + c2 * ((d2x + d2y) + propy);  // y strip (:388-396)
-        } else {
-            v = (-p1 + 2.f * p) + c2 * ((d2x + d2y) + propx);  // corner x call (:402-403)
-            v = v + c2 * propy;                                 // corner y call (:407)
-        }
-        a.pn[idx] = v;
+        const float p = T[0], p1 = p1v[q];
+        float d2x = (p + p) * kC2_0;
+        d2x = d2x + (T[HY] + T[-HY]) * kC2_1;
+        d2x = d2x + (T[2 * HY] + T[-2 * HY]) * kC2_2;
+        d2x = d2x + (T[3 * HY] + T[-3 * HY]) * kC2_3;
+        float d2y = (p + p) * kC2_0;
+        d2y = d2y + (T[1] + T[-1]) * kC2_1;
+        d2y = d2y + (T[2] + T[-2]) * kC2_2;
+        d2y = d2y + (T[3] + T[-3]) * kC2_3;
+        float lap = d2x + d2y;
+        if (has_src) lap = lap + srcv[(w * CPW + q) * TY + lane];  // (d2x + d2y + source), :443
+        float v = (-p1 + 2.f * p) + lap * c2v[q];  // :443-444
+        if (a.fs && j == dn) v = 0.f;              // :463-466
+        pn[i * ld + j] = v;
+        dv[q] = v - p1;
     }
 
     if (a.snap_slot < 0) return;  // uniform over the workgroup
-    const size_t sbase = (size_t)a.snap_slot * a.snap_stride;
     if (a.mode == 0) {
         // snapshot F = ((p(k) - p(k-2)) * 0.5) / dt (:485)
         float fv[CPW];
 #pragma unroll
         for (int q = 0; q < CPW; ++q) fv[q] = dok[q] ? (dv[q] * 0.5f) / a.dt : 0.f;
         if (a.snap_fmt == kSnapF32) {
+            gptr<float> S = gp((float *)a.snaps) + sbase;
 #pragma unroll
             for (int q = 0; q < CPW; ++q)
-                if (dok[q]) {
-                    const int i = i0 + w * CPW + q;
-                    ((float *)a.snaps)[sbase + (size_t)(i - dn) * a.ny + (j - dn)] = fv[q];
-                }
+                if (dok[q]) S[(i0 + w * CPW + q - dn) * lds + jw] = fv[q];
             return;
         }
         // 16-bit snapshot: power-of-two scale from the tile's max-abs
@@ -277,51 +183,214 @@ __device__ __forceinline__ void step_tile(const StepArgs &a, int t, float *tile,
         __syncthreads();  // red is reused by the next tile
         const int target = a.snap_fmt == kSnapF16 ? 14 : 0;
         const int e = m > 0.f ? ilogbf(m) - target : 0;
-        if (threadIdx.x == 0) a.snap_scale[(size_t)a.snap_slot * a.ntiles + t] = ldexpf(1.f, e);
+        if (threadIdx.x == 0) gp(a.snap_scale)[(size_t)a.snap_slot * a.ntiles + t] = ldexpf(1.f, e);
 #pragma unroll
         for (int q = 0; q < CPW; ++q)
             if (dok[q]) {
-                const int i = i0 + w * CPW + q;
-                const size_t so = sbase + (size_t)(i - dn) * a.ny + (j - dn);
+                const size_t so = sbase + (size_t)((i0 + w * CPW + q - dn) * lds + jw);
                 const float s = ldexpf(fv[q], -e);
                 if (a.snap_fmt == kSnapF16)
-                    ((_Float16 *)a.snaps)[so] = (_Float16)s;
+                    gp((_Float16 *)a.snaps)[so] = (_Float16)s;
                 else
-                    ((unsigned short *)a.snaps)[so] = f32_to_bf16(s);
+                    gp((unsigned short *)a.snaps)[so] = f32_to_bf16(s);
             }
         return;
     }
     // imaging: kernel += 0.5*(q(k)-q(k-2)) * F(snap) * store_snap (solver_ac_backward_ram.f90:465)
-    const float scale = a.snap_fmt == kSnapF32 ? 1.f : a.snap_scale[(size_t)a.snap_slot * a.ntiles + t];
+    const float scale = a.snap_fmt == kSnapF32 ? 1.f : gp(a.snap_scale)[(size_t)a.snap_slot * a.ntiles + t];
+    gptr<float> img = gp(a.image);
 #pragma unroll
     for (int q = 0; q < CPW; ++q)
         if (dok[q]) {
-            const int i = i0 + w * CPW + q;
-            const size_t o = (size_t)(i - dn) * a.ny + (j - dn);
-            float F;
-            if (a.snap_fmt == kSnapF32)
-                F = ((const float *)a.snaps)[sbase + o];
-            else if (a.snap_fmt == kSnapF16)
-                F = (float)((const _Float16 *)a.snaps)[sbase + o] * scale;
-            else
-                F = bf16_to_f32(((const unsigned short *)a.snaps)[sbase + o]) * scale;
+            const int o = (i0 + w * CPW + q - dn) * lds + jw;
+            const float F = a.snap_fmt == kSnapF32 ? Fv[q] : Fv[q] * scale;
             const float term = ((0.5f * dv[q]) * F) * a.ss_f;
             if (a.kahan) {
-                const float img = a.image[o];
-                const float y = term - a.image_comp[o];
-                const float tt = img + y;
-                a.image_comp[o] = (tt - img) - y;
-                a.image[o] = tt;
+                const float y = term - cmv[q];
+                const float tt = imv[q] + y;
+                gp(a.image_comp)[o] = (tt - imv[q]) - y;
+                img[o] = tt;
             } else {
-                a.image[o] = a.image[o] + term;
+                img[o] = imv[q] + term;
             }
         }
+}
+
+// CPML tile: x strip (xs != 0, ys == 0), y strip (xs == 0, ys != 0) or corner.  xs / ys:
+// 1 low side, 2 high side.  Every tile spans >= 4 cells along each CPML axis, so all
+// clamped-edge stencil points (solver_space_derivs_clean.f90:24-34) and x-memory
+// neighbours lie inside the LDS-staged ranges.
+template <int TX>
+__device__ __forceinline__ void cpml_tile(const StepArgs &a, int xs, int ys, int j0, int j1, int i0, int i1,
+                                          float *tile, float *phs) {
+    constexpr int TY = kTileY, HY = TY + 6, HX = TX + 6, CPW = TX / 4;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int dn = a.dn, ld = a.ld, dn2 = 2 * dn;
+    const int j = j0 + lane;
+    const bool rowok = j < j1;
+    gptr<float> pn = gp(a.pn);
+
+    // the top CPML is never updated with a free surface (:391); the _ram top and bottom
+    // pads past column dn have c2 = 0 (velocity padding :288-302), so p stays exactly 0
+    // there and their CPML memory never reaches p
+    const bool zero_tile = (a.fs && ys == 1) || (a.ram && ys != 0 && xs == 0 && i0 - dn >= dn);
+    if (zero_tile) {
+        if (rowok)
+#pragma unroll
+            for (int q = 0; q < CPW; ++q) {
+                const int i = i0 + w * CPW + q;
+                if (i < i1) pn[i * ld + j] = 0.f;
+            }
+        return;
+    }
+
+    // x memory of this strip: columns xm0-3 .. xm0+TX+2 of the memory arrays (strip
+    // columns [xlo, xlo+dn)), rows j0..j0+63, staged next to the wavefield tile
+    const int xoff = xs == 2 ? a.nx : 0;  // grid column i -> memory column i - xoff
+    const int xlo = xs == 2 ? dn : 0;
+    const int xm0 = i0 - xoff;
+    if (xs != 0) {
+        gptr<const float> ph = gp(a.phx_o);
+#pragma unroll
+        for (int li = w; li < HX; li += 4) {
+            const int mc = xm0 - 3 + li;
+            const bool ok = rowok && !a.zm && mc >= xlo && mc < xlo + dn;
+            phs[li * TY + lane] = ok ? ph[mc * ld + j] : 0.f;
+        }
+    }
+    float p1v[CPW], c2v[CPW];
+#pragma unroll
+    for (int q = 0; q < CPW; ++q) {
+        const int i = i0 + w * CPW + q;
+        const bool ok = rowok && i < i1;
+        p1v[q] = (ok && !a.zp) ? pn[i * ld + j] : 0.f;
+        c2v[q] = ok ? c2_at(a, j, i) : 0.f;
+    }
+    stage_p<TX>(gp(a.pc), ld, a.nyt, a.nxt, j0, i0, a.zc, tile);
+    __syncthreads();
+    if (!rowok) return;
+
+    gptr<const float> fA = gp(a.fA), fB = gp(a.fB);
+    const int jc = clampi(j, 3, a.nyt - 4);
+    const int disty = ys == 0 ? 0 : (ys == 1 ? dn - j : j - (dn + a.ny) + 1);  // 1 next to the interior
+#pragma unroll
+    for (int q = 0; q < CPW; ++q) {
+        const int i = i0 + w * CPW + q;
+        if (i >= i1) continue;
+        const int idx = i * ld + j;
+        if (a.ram && ys != 0 && xs == 0 && i - dn >= dn) {
+            pn[idx] = 0.f;
+            continue;
+        }
+        const float p1 = p1v[q], c2 = c2v[q];
+        const int ic = clampi(i, 3, a.nxt - 4);
+        // P(gj, gi) = tile[(gi - i0 + 3) * HY + (gj - j0 + 3)]
+        const float *Tx = tile + (ic - i0 + 3) * HY + (lane + 3);       // row j, column ic
+        const float *Ty = tile + (i - i0 + 3) * HY + (jc - j0 + 3);     // row jc, column i
+        const float px = Tx[0], py = Ty[0];
+        const float xm1 = Tx[-HY], xp1 = Tx[HY], xm2 = Tx[-2 * HY], xp2 = Tx[2 * HY];
+        const float xm3 = Tx[-3 * HY], xp3 = Tx[3 * HY];
+        const float ym1 = Ty[-1], yp1 = Ty[1], ym2 = Ty[-2], yp2 = Ty[2];
+        const float ym3 = Ty[-3], yp3 = Ty[3];
+        float d2x = (px + px) * kC2_0;
+        d2x = d2x + (xp1 + xm1) * kC2_1;
+        d2x = d2x + (xp2 + xm2) * kC2_2;
+        d2x = d2x + (xp3 + xm3) * kC2_3;
+        float d2y = (py + py) * kC2_0;
+        d2y = d2y + (yp1 + ym1) * kC2_1;
+        d2y = d2y + (yp2 + ym2) * kC2_2;
+        d2y = d2y + (yp3 + ym3) * kC2_3;
+        const float p = tile[(i - i0 + 3) * HY + (lane + 3)];
+        const int distx = xs == 0 ? 0 : (xs == 1 ? dn - i : i - (dn + a.nx) + 1);
+        // corners use F(max(dist_x, dist_y)) for both directions (solver_PML_coeffs.f90:288-322)
+        const int dist = distx > disty ? distx : disty;
+        const float A = fA[dist - 1], B = fB[dist - 1];
+        // x-direction memory (strips xl/xr and the corners), pml_matrices_x/_xycorner(1)
+        float propx = 0.f;
+        if (xs != 0) {
+            const int xc = i - xoff;                          // memory column of this cell
+            const int xcc = xlo + clampi(xc - xlo, 3, dn - 4);  // clamped derivative column
+            const float *ph = phs + (xcc - xm0 + 3) * TY + lane;
+            float dphi = (ph[TY] - ph[-TY]) * kC1_1;
+            dphi = dphi + (ph[2 * TY] - ph[-2 * TY]) * kC1_2;
+            dphi = dphi + (ph[3 * TY] - ph[-3 * TY]) * kC1_3;
+            const float pho = phs[(xc - xm0 + 3) * TY + lane];
+            const int xo = xc * ld + j;
+            const float cho = a.zm ? 0.f : gp((const float *)a.chx)[xo];
+            float d1x = (xp1 - xm1) * kC1_1;
+            d1x = d1x + (xp2 - xm2) * kC1_2;
+            d1x = d1x + (xp3 - xm3) * kC1_3;
+            const float phn = A * pho + B * d1x;
+            const float chn = A * cho + B * (d2x + dphi);
+            gp(a.phx_n)[xo] = phn;
+            gp(a.chx)[xo] = chn;
+            propx = dphi + chn;
+        }
+        float propy = 0.f;
+        if (ys != 0) {
+            const int jl = ys == 1 ? j : j - (dn + a.ny);
+            const int yr = ys == 1 ? j : j - a.ny;  // row of the y memory arrays
+            const int yrc = yr - jl + clampi(jl, 3, dn - 4);
+            float dphi = 0.f, pho = 0.f, cho = 0.f;
+            const int yo = i * dn2 + yr;
+            if (!a.zm) {
+                gptr<const float> ph = gp(a.phy_o) + i * dn2;
+                dphi = (ph[yrc + 1] - ph[yrc - 1]) * kC1_1;
+                dphi = dphi + (ph[yrc + 2] - ph[yrc - 2]) * kC1_2;
+                dphi = dphi + (ph[yrc + 3] - ph[yrc - 3]) * kC1_3;
+                pho = ph[yr];
+                cho = gp((const float *)a.chy)[yo];
+            }
+            float d1y = (yp1 - ym1) * kC1_1;
+            d1y = d1y + (yp2 - ym2) * kC1_2;
+            d1y = d1y + (yp3 - ym3) * kC1_3;
+            const float phn = A * pho + B * d1y;
+            const float chn = A * cho + B * (d2y + dphi);
+            gp(a.phy_n)[yo] = phn;
+            gp(a.chy)[yo] = chn;
+            propy = dphi + chn;
+        }
+        float v;
+        if (ys == 0) {
+            // x strip, interior rows (:376-382).  With a free surface the _ram form
+            // zeroes row dn+1 of the grid but the strip keeps its own history there.
+            const bool own = a.ram && a.fs && j == dn;
+            const int hsi = i - xoff;
+            const float ps = own ? (a.zc ? 0.f : gp(a.hs_c)[hsi]) : p;
+            const float p1s = own ? (a.zp ? 0.f : gp((const float *)a.hs_n)[hsi]) : p1;
+            v = (-p1s + 2.f * ps) + c2 * ((d2x + d2y) + propx);
+            if (own) {
+                gp(a.hs_n)[hsi] = v;
+                v = 0.f;
+            }
+        } else if (xs == 0) {
+            v = (-p1 + 2.f * p) + c2 * ((d2x + d2y) + propy);  // y strip (:388-396)
+        } else {
+            v = (-p1 + 2.f * p) + c2 * ((d2x + d2y) + propx);  // corner x call (:402-403)
+            v = v + c2 * propy;                                 // corner y call (:407)
+        }
+        pn[idx] = v;
+    }
+}
+
+// One time step of one tile (tile index t, region-aligned tiling) of one shot.  The
+// branch is uniform over the workgroup; all its threads take part in the barriers.
+template <int TX>
+__device__ __forceinline__ void step_tile(const StepArgs &a, int t, float *tile, float *aux, float *red) {
+    int j0, j1, i0, i1;
+    const int ys = axis_tile(t % a.ntile_y, a.rt_pml, a.rt_int, a.dn, a.ny, kTileY, &j0, &j1);
+    const int xs = axis_tile(t / a.ntile_y, a.ct_pml, a.ct_int, a.dn, a.nx, TX, &i0, &i1);
+    if ((xs | ys) == 0)
+        interior_tile<TX>(a, t, j0, j1, i0, i1, tile, aux, red);
+    else
+        cpml_tile<TX>(a, xs, ys, j0, j1, i0, i1, tile, aux);
 }
 
 template <int TX>
 __global__ __launch_bounds__(kBlock) void rtm_step_kernel(StepArgs a) {
     __shared__ float tile[(TX + 6) * (kTileY + 6)];
     __shared__ float red[4];
+    __shared__ float aux[(TX + 6) * kTileY];
     const int b = blockIdx.x;
     // traces of the previous step (:567-571), read from p(k-1) before it is overwritten
     if (a.rec_row >= 0)
@@ -333,7 +402,7 @@ __global__ __launch_bounds__(kBlock) void rtm_step_kernel(StepArgs a) {
     // contiguous tile range [x*per_xcd, (x+1)*per_xcd) and shares halos in its own L2.
     const int t = (b & 7) * a.per_xcd + (b >> 3);
     if (t >= a.ntiles) return;
-    step_tile<TX>(a, t, tile, red);
+    step_tile<TX>(a, t, tile, aux, red);
 }
 
 // Per-step view of a slot at global step G (forward steps k = 2..nt, then backward
@@ -393,16 +462,18 @@ __device__ void resolve_slot(const SlotDesc &sd, long long G, StepArgs *out) {
 
 // One time step of every active slot: a persistent grid walks the slot x tile work
 // list; each XCD takes a contiguous range of items (neighbouring tiles of one shot).
-// Block 0 also resolves the next step's per-slot view into cur[(G+1)&1].
+// Block 0 also resolves the next step's per-slot views into `next`.  `now` and `next`
+// are the two halves of a double buffer (chosen by the parity of the step within its
+// chunk); `now` is only read here, so its words are fetched with scalar loads.
 template <int TX>
-__global__ __launch_bounds__(kBlock) void rtm_batch_kernel(BatchArgs b, StepArgs *cur) {
+__global__ __launch_bounds__(kBlock) void rtm_batch_kernel(BatchArgs b, const StepArgs *__restrict__ now,
+                                                           StepArgs *__restrict__ next) {
     __shared__ float tile[(TX + 6) * (kTileY + 6)];
     __shared__ float red[4];
+    __shared__ float aux[(TX + 6) * kTileY];
     const long long G = *b.gbase + b.goff;
     if (blockIdx.x == 0)
-        for (int s = threadIdx.x; s < b.nslots; s += blockDim.x)
-            resolve_slot(b.slots[s], G + 1, &cur[((G + 1) & 1) * b.nslots + s]);
-    const StepArgs *now = cur + (G & 1) * b.nslots;
+        for (int s = threadIdx.x; s < b.nslots; s += blockDim.x) resolve_slot(b.slots[s], G + 1, &next[s]);
     const int x = blockIdx.x & 7, nbx = gridDim.x >> 3;
     for (int m = blockIdx.x >> 3; m < b.per_xcd; m += nbx) {
         const int wk = x * b.per_xcd + m;
@@ -410,15 +481,16 @@ __global__ __launch_bounds__(kBlock) void rtm_batch_kernel(BatchArgs b, StepArgs
         const int s = wk / b.cap_tiles, t = wk - s * b.cap_tiles;
         const StepArgs &a = now[s];
         if (a.k == 0 || t >= a.ntiles) continue;
-        step_tile<TX>(a, t, tile, red);
-        __syncthreads();  // the tile buffer is reused by the next item
+        step_tile<TX>(a, t, tile, aux, red);
+        __syncthreads();  // the LDS buffers are reused by the next item
     }
 }
 
-// Resolves the per-slot views of step G (used before the first step of a chunk).
+// Resolves the per-slot views of step G = *gbase, the first step of a chunk, into the
+// even half of the double buffer.
 __global__ void rtm_resolve_kernel(const SlotDesc *slots, int nslots, const long long *gbase, StepArgs *cur) {
     const long long G = *gbase;
-    for (int s = threadIdx.x; s < nslots; s += blockDim.x) resolve_slot(slots[s], G, &cur[(G & 1) * nslots + s]);
+    for (int s = threadIdx.x; s < nslots; s += blockDim.x) resolve_slot(slots[s], G, &cur[s]);
 }
 
 __global__ void rtm_advance_kernel(long long *gbase, long long by) {
@@ -455,12 +527,12 @@ __global__ void rtm_gather_kernel(const float *traces, int nt, const int *cols, 
 // is embedded at columns x0..x0+nx_w-1 of PSDM_sum (FP64) and the fold Cov_sum gains 1
 // there (Cov_w = 1 on the window).  The shot image (and its compensation term) is
 // cleared for the next shot (the driver's PSDM_w = 0, :384).
-__global__ void rtm_stack_kernel(float *img, float *comp, int ny, int nx_w, int x0, double *sum, int ld_sum,
-                                 int *fold) {
+__global__ void rtm_stack_kernel(float *img, float *comp, int ld_img, int ny, int nx_w, int x0, double *sum,
+                                 int ld_sum, int *fold) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     const int i = blockIdx.y;
     if (j < ny && i < nx_w) {
-        const size_t o = (size_t)i * ny + j;
+        const size_t o = (size_t)i * ld_img + j;
         const size_t g = (size_t)(x0 + i) * ld_sum + j;
         double v = (double)img[o];
         if (comp) {
@@ -493,17 +565,19 @@ void launch_step(const StepArgs &a, int tile_x, hipStream_t s, hipEvent_t e0, hi
 void launch_batch(const BatchArgs &b, StepArgs *cur, int tile_x, int grid, hipStream_t s, hipEvent_t e0,
                   hipEvent_t e1) {
     grid = (grid + 7) / 8 * 8;
+    const StepArgs *now = cur + (b.goff & 1) * b.nslots;
+    StepArgs *next = cur + ((b.goff + 1) & 1) * b.nslots;
     if (!e0 && !e1) {  // plain launch: also valid under stream capture
         if (tile_x == 16)
-            hipLaunchKernelGGL(rtm_batch_kernel<16>, dim3(grid), dim3(kBlock), 0, s, b, cur);
+            hipLaunchKernelGGL(rtm_batch_kernel<16>, dim3(grid), dim3(kBlock), 0, s, b, now, next);
         else
-            hipLaunchKernelGGL(rtm_batch_kernel<8>, dim3(grid), dim3(kBlock), 0, s, b, cur);
+            hipLaunchKernelGGL(rtm_batch_kernel<8>, dim3(grid), dim3(kBlock), 0, s, b, now, next);
         return;
     }
     if (tile_x == 16)
-        hipExtLaunchKernelGGL(rtm_batch_kernel<16>, dim3(grid), dim3(kBlock), 0, s, e0, e1, 0, b, cur);
+        hipExtLaunchKernelGGL(rtm_batch_kernel<16>, dim3(grid), dim3(kBlock), 0, s, e0, e1, 0, b, now, next);
     else
-        hipExtLaunchKernelGGL(rtm_batch_kernel<8>, dim3(grid), dim3(kBlock), 0, s, e0, e1, 0, b, cur);
+        hipExtLaunchKernelGGL(rtm_batch_kernel<8>, dim3(grid), dim3(kBlock), 0, s, e0, e1, 0, b, now, next);
 }
 
 void launch_resolve(const SlotDesc *slots, int nslots, const long long *gbase, StepArgs *cur, hipStream_t s) {
@@ -530,10 +604,10 @@ void launch_gather(const float *traces, int nt, const int *cols, int ncols, floa
                        out, reverse);
 }
 
-void launch_stack(float *img, float *comp, int ny, int nx_w, int x0, double *sum, int ld_sum, int *fold,
+void launch_stack(float *img, float *comp, int ld_img, int ny, int nx_w, int x0, double *sum, int ld_sum, int *fold,
                   hipStream_t s) {
-    hipLaunchKernelGGL(rtm_stack_kernel, dim3((ny + 255) / 256, nx_w), dim3(256), 0, s, img, comp, ny, nx_w, x0, sum,
-                       ld_sum, fold);
+    hipLaunchKernelGGL(rtm_stack_kernel, dim3((ny + 255) / 256, nx_w), dim3(256), 0, s, img, comp, ld_img, ny, nx_w, x0,
+                       sum, ld_sum, fold);
 }
 
 void launch_normalize(const double *sum, const int *fold, float *out, size_t n, hipStream_t s) {
