@@ -186,7 +186,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=64, help="shots timed (per form)")
     ap.add_argument("--warmup", type=int, default=1, help="untimed shots before the plain timed region")
-    ap.add_argument("--tile-x", type=int, default=8, help="plain form tile width")
+    ap.add_argument("--tile-x", type=int, default=16, help="plain form tile width (8, 16 or 32)")
     ap.add_argument("--timing-every", type=int, default=16, help="sample every n-th step kernel with HIP events")
     ap.add_argument("--cpu-procs", type=int, default=16)
     ap.add_argument("--cpu-steps", type=int, default=200)
@@ -199,7 +199,7 @@ def main():
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build runs the RTM path on a single MI355X "
               f"(--gpus {args.gpus} is not supported; shots shard across GPUs with no per-step "
-              f"communication, see DESIGN.md)", file=sys.stderr)
+              f"communication)", file=sys.stderr)
         sys.exit(2)
     forms = set(args.forms.split(","))
 
@@ -262,14 +262,14 @@ def main():
         log(f"[bench] plain {plain['shots_per_s']} shots/s")
 
     if "batched" in forms:
-        cfg = dict(nslots=16, tile_x=8, grid=0, chunk_steps=64)
+        cfg = dict(nslots=16, tile_x=32, grid=0, chunk_steps=64)
         tuned = None
         if not args.no_tune:
             from psdm_amd.autotune import tune
 
             jobs = [job_of(W, k, d_src, obs) for k in range(nshots)]
             best, trials = tune(ctx, d_mig, ny, m.nx, jobs, max(s.nx_w for s in shots), nt, dPML, W["fs"], m.dx, dt,
-                                slots=(8, 16), tiles=(8, 16), grids=(0, 2048), log=log)
+                                slots=(8, 16, 24), tiles=(16, 32), grids=(0, 2048), log=log)
             cfg.update(nslots=best["nslots"], tile_x=best["tile_x"], grid=best["grid"])
             tuned = {"best": best, "trials": trials}
         B = run_batched(ctx, torch, dev, W, d_mig, d_src, obs, args.steps, args.warmup, cfg)

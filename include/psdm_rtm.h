@@ -116,7 +116,7 @@ int psdm_rtm_normalize(psdm_rtm_ctx *ctx, const double *d_psdm_sum, const int *d
 long long psdm_rtm_device_bytes(const psdm_rtm_ctx *ctx);
 
 /* Launch-shape and timing controls for the device form.
- * tile_x: columns per workgroup tile (8 or 16).  use_graph: capture the time loop in
+ * tile_x: columns per workgroup tile (8, 16 or 32).  use_graph: capture the time loop in
  * hipGraphs of `graph_steps` steps.  timing != 0 records HIP events around every step
  * kernel; psdm_rtm_step_stats returns their count and mean duration (ms). */
 int psdm_rtm_set_launch(psdm_rtm_ctx *ctx, int tile_x, int use_graph, int graph_steps);
@@ -147,7 +147,7 @@ typedef struct {
     int snap_format;         /* 0 FP32, 1 FP16, 2 BF16 (power-of-two scale per snapshot tile) */
     int kahan;               /* compensated FP32 image accumulation */
     int chunk_steps;         /* time steps per captured graph */
-    int tile_x;              /* 8 or 16 columns per tile */
+    int tile_x;              /* 8, 16 or 32 columns per tile */
     int grid;                /* persistent grid size in workgroups (0 = one per work item) */
     int use_graph;           /* replay captured graphs (1) or launch each step (0) */
     int max_inflight;        /* chunks queued ahead of the host (>= 1) */

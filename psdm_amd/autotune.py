@@ -35,7 +35,7 @@ def _trial(ctx, d_vel, ny, nx_prop, jobs, nslots, max_nx, nt, dPML, fs, dx, dt, 
 
 
 def tune(ctx, d_vel, ny, nx_prop, jobs, max_nx, nt, dPML, fs, dx, dt, snap_format="f32", store_snap=1, kahan=False,
-         slots=(8, 16), tiles=(8, 16), grids=(0, 2048), chunk_steps=32, chunks=3, log=None):
+         slots=(8, 16, 24), tiles=(16, 32), grids=(0, 2048), chunk_steps=32, chunks=3, log=None):
     """jobs: list of Engine.submit keyword dicts.  Returns (best config dict, trials)."""
     trials = []
     for nslots, tile_x, grid in itertools.product(slots, tiles, grids):

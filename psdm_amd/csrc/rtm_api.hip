@@ -658,7 +658,8 @@ long long psdm_rtm_device_bytes(const psdm_rtm_ctx *c) {
 
 int psdm_rtm_set_launch(psdm_rtm_ctx *c, int tile_x, int use_graph, int graph_steps) {
     if (!c) return PSDM_RTM_EINVAL;
-    if (tile_x != 8 && tile_x != 16) return fail(c, PSDM_RTM_EINVAL, "tile_x must be 8 or 16");
+    if (tile_x != 8 && tile_x != 16 && tile_x != 32)
+        return fail(c, PSDM_RTM_EINVAL, "tile_x must be 8, 16 or 32");
     if (graph_steps < 1) return fail(c, PSDM_RTM_EINVAL, "graph_steps must be >= 1");
     c->tile_x = tile_x;
     c->use_graph = use_graph;

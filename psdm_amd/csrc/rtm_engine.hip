@@ -108,7 +108,6 @@ struct psdm_rtm_engine {
     size_t n_p = 0, n_x = 0, n_y = 0, state_stride = 0, esize = 4;
     // device
     SlotDesc *d_slots = nullptr;
-    StepArgs *d_cur = nullptr;
     long long *d_gbase = nullptr;
     float *d_tabs = nullptr, *d_state = nullptr, *d_c2 = nullptr, *d_img = nullptr, *d_comp = nullptr;
     char *d_arena = nullptr;
@@ -150,7 +149,7 @@ int issue_chunk(psdm_rtm_engine *e) {
     } else {
         for (int g = 0; g < C; g++) {
             BatchArgs b{e->d_slots, e->cfg.nslots, e->cap_tiles, e->nwork, e->per_xcd, e->d_gbase, g};
-            launch_batch(b, e->d_cur, e->cfg.tile_x, e->grid, c->stream);
+            launch_batch(b, e->cfg.tile_x, e->grid, c->stream);
         }
         launch_advance(e->d_gbase, C, c->stream);
     }
@@ -163,7 +162,7 @@ int capture_graph(psdm_rtm_engine *e) {
     HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
     for (int g = 0; g < e->cfg.chunk_steps; g++) {
         BatchArgs b{e->d_slots, e->cfg.nslots, e->cap_tiles, e->nwork, e->per_xcd, e->d_gbase, g};
-        launch_batch(b, e->d_cur, e->cfg.tile_x, e->grid, c->stream);
+        launch_batch(b, e->cfg.tile_x, e->grid, c->stream);
     }
     launch_advance(e->d_gbase, e->cfg.chunk_steps, c->stream);
     HIP_TRY(c, hipStreamEndCapture(c->stream, &e->graph));
@@ -393,7 +392,8 @@ int psdm_rtm_engine_create(psdm_rtm_ctx *c, const psdm_rtm_engine_cfg *cfg, psdm
     if (cfg->dPML < 2 || cfg->dPML + 6 > kMaxTables) return fail(c, PSDM_RTM_EINVAL, "dPML out of range");
     if (cfg->store_snap < 1) return fail(c, PSDM_RTM_EINVAL, "store_snap must be >= 1");
     if (cfg->snap_format < 0 || cfg->snap_format > 2) return fail(c, PSDM_RTM_EINVAL, "snap_format must be 0, 1 or 2");
-    if (cfg->tile_x != 8 && cfg->tile_x != 16) return fail(c, PSDM_RTM_EINVAL, "tile_x must be 8 or 16");
+    if (cfg->tile_x != 8 && cfg->tile_x != 16 && cfg->tile_x != 32)
+        return fail(c, PSDM_RTM_EINVAL, "tile_x must be 8, 16 or 32");
     if (cfg->chunk_steps < 1 || cfg->chunk_steps > 4096) return fail(c, PSDM_RTM_EINVAL, "chunk_steps out of range");
     if (!cfg->d_vel) return fail(c, PSDM_RTM_EINVAL, "null velocity model");
     if (!(cfg->dt > 0.f) || !(cfg->dmodel > 0.f)) return fail(c, PSDM_RTM_EINVAL, "dt and dmodel must be > 0");
@@ -432,7 +432,6 @@ int psdm_rtm_engine_create(psdm_rtm_ctx *c, const psdm_rtm_engine_cfg *cfg, psdm
             return bail(fail(c, PSDM_RTM_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_))); \
     } while (0)
     ETRY(hipMalloc(&e->d_slots, S * sizeof(SlotDesc)));
-    ETRY(hipMalloc(&e->d_cur, 2 * S * sizeof(StepArgs)));
     ETRY(hipMalloc(&e->d_gbase, sizeof(long long)));
     ETRY(hipMalloc(&e->d_tabs, 2 * e->dn * sizeof(float)));
     ETRY(hipMalloc(&e->d_state, S * e->state_stride * sizeof(float)));
@@ -488,7 +487,7 @@ int psdm_rtm_engine_destroy(psdm_rtm_engine *e) {
         if (sl.d_ints) (void)hipFree(sl.d_ints);
         if (sl.d_inj) (void)hipFree(sl.d_inj);
     }
-    void *bufs[] = {e->d_slots, e->d_cur, e->d_gbase, e->d_tabs, e->d_state, e->d_c2, e->d_img, e->d_comp, e->d_arena};
+    void *bufs[] = {e->d_slots, e->d_gbase, e->d_tabs, e->d_state, e->d_c2, e->d_img, e->d_comp, e->d_arena};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     if (e->own_image) {
@@ -566,7 +565,6 @@ int psdm_rtm_engine_run(psdm_rtm_engine *e, int max_chunks, int *n_active, int *
         }
         // the ring event about to be re-recorded must not still be awaited by a job
         if (int rc = collect(e, e->chunks - psdm_rtm_engine::kRing)) return rc;
-        launch_resolve(e->d_slots, e->cfg.nslots, e->d_gbase, e->d_cur, c->stream);
         if (int rc = issue_chunk(e)) return rc;
         for (auto &sl : e->slots)
             if (sl.busy) e->active_slot_steps += std::min<long long>(C, sl.g_end - e->G);

@@ -62,7 +62,7 @@ template <int TX>
 __device__ __forceinline__ void stage_p(gptr<const float> pc, int ld, int nyt, int nxt, int j0, int i0, bool zero,
                                         float *tile) {
     constexpr int HY = kTileY + 6, HX = TX + 6;
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // w: wave-uniform
 #pragma unroll
     for (int li = w; li < HX; li += 4) {
         const int gi = i0 - 3 + li;
@@ -83,7 +83,7 @@ template <int TX>
 __device__ __forceinline__ void interior_tile(const StepArgs &a, int t, int j0, int j1, int i0, int i1, float *tile,
                                               float *srcv, float *red) {
     constexpr int TY = kTileY, HY = TY + 6, CPW = TX / 4;
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // w: wave-uniform
     const int dn = a.dn, ld = a.ld, lds = a.lds;
     const int j = j0 + lane, jw = j - dn;
     const bool rowok = j < j1;
@@ -224,7 +224,7 @@ template <int TX>
 __device__ __forceinline__ void cpml_tile(const StepArgs &a, int xs, int ys, int j0, int j1, int i0, int i1,
                                           float *tile, float *phs) {
     constexpr int TY = kTileY, HY = TY + 6, HX = TX + 6, CPW = TX / 4;
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // w: wave-uniform
     const int dn = a.dn, ld = a.ld, dn2 = 2 * dn;
     const int j = j0 + lane;
     const bool rowok = j < j1;
@@ -405,92 +405,79 @@ __global__ __launch_bounds__(kBlock) void rtm_step_kernel(StepArgs a) {
     step_tile<TX>(a, t, tile, aux, red);
 }
 
-// Per-step view of a slot at global step G (forward steps k = 2..nt, then backward
-// steps k = 2..nt; ping-pong buffers alternate with k; each phase starts from zero state).
-__device__ void resolve_slot(const SlotDesc &sd, long long G, StepArgs *out) {
-    const unsigned *src = (const unsigned *)&sd.a;
-    unsigned *dst = (unsigned *)out;
-    for (int q = 0; q < (int)(sizeof(StepArgs) / 4); ++q) dst[q] = src[q];
+// Per-step fields of a slot at global step G (forward steps k = 2..nt, then backward
+// steps k = 2..nt; ping-pong buffers alternate with k; each phase starts from zero
+// state).  `a` holds the slot's static fields; returns false when the slot is idle.
+__device__ __forceinline__ bool resolve_step(const SlotDesc &sd, long long G, StepArgs &a) {
     const long long tl = G - sd.g0;
     const int nt = sd.nt;
-    if (sd.g0 < 0 || tl < 0 || tl >= 2LL * (nt - 1)) {
-        out->k = 0;  // idle
-        return;
-    }
+    if (sd.g0 < 0 || tl < 0 || tl >= 2LL * (nt - 1)) return false;
     const bool fwd = tl < nt - 1;
     const int k = (int)(fwd ? tl + 2 : tl - (nt - 1) + 2);
     const int m = (k - 2) & 1;
-    out->k = k;
-    out->mode = fwd ? 0 : 1;
-    out->pc = m ? sd.pB : sd.pA;
-    out->pn = m ? sd.pA : sd.pB;
-    out->phx_o = m ? sd.phxB : sd.phxA;
-    out->phx_n = m ? sd.phxA : sd.phxB;
-    out->phy_o = m ? sd.phyB : sd.phyA;
-    out->phy_n = m ? sd.phyA : sd.phyB;
-    out->hs_c = m ? sd.hsB : sd.hsA;
-    out->hs_n = m ? sd.hsA : sd.hsB;
-    out->zc = k == 2;
-    out->zp = k <= 3;
-    out->zm = k == 2;
-    out->rec_row = -1;
+    a.k = k;
+    a.mode = fwd ? 0 : 1;
+    a.pc = m ? sd.pB : sd.pA;
+    a.pn = m ? sd.pA : sd.pB;
+    a.phx_o = m ? sd.phxB : sd.phxA;
+    a.phx_n = m ? sd.phxA : sd.phxB;
+    a.phy_o = m ? sd.phyB : sd.phyA;
+    a.phy_n = m ? sd.phyA : sd.phyB;
+    a.hs_c = m ? sd.hsB : sd.hsA;
+    a.hs_n = m ? sd.hsA : sd.hsB;
+    a.zc = k == 2;
+    a.zp = k <= 3;
+    a.zm = k == 2;
+    a.rec_row = -1;
     int slot = -1;
     if (fwd) {
         if ((k - 1) % sd.ss == 0) {
             const int s = (k - 1) / sd.ss;
             if (s <= sd.nsnap) slot = s - 1;
         }
-        out->ev_ptr = sd.ev_ptr_f;
-        out->ev_cell = sd.ev_cell_f;
-        out->ev_inj = sd.ev_inj_f;
-        out->inj_val = sd.inj_f;
-        out->ninj = sd.ninj_f;
+        a.ev_ptr = sd.ev_ptr_f;
+        a.ev_cell = sd.ev_cell_f;
+        a.ev_inj = sd.ev_inj_f;
+        a.inj_val = sd.inj_f;
+        a.ninj = sd.ninj_f;
     } else {
         // backward step k pairs with the forward snapshot taken at k_f = nt + 2 - k
         if ((nt + 1 - k) % sd.ss == 0) {
             const int s = (nt + 1 - k) / sd.ss;
             if (s >= 1 && s <= sd.nsnap) slot = s - 1;
         }
-        out->ev_ptr = sd.ev_ptr_b;
-        out->ev_cell = sd.ev_cell_b;
-        out->ev_inj = sd.ev_inj_b;
-        out->inj_val = sd.inj_b;
-        out->ninj = sd.ninj_b;
+        a.ev_ptr = sd.ev_ptr_b;
+        a.ev_cell = sd.ev_cell_b;
+        a.ev_inj = sd.ev_inj_b;
+        a.inj_val = sd.inj_b;
+        a.ninj = sd.ninj_b;
     }
-    out->snap_slot = slot;
+    a.snap_slot = slot;
+    return true;
 }
 
-// One time step of every active slot: a persistent grid walks the slot x tile work
+// One time step of every active slot: a grid of workgroups walks the slot x tile work
 // list; each XCD takes a contiguous range of items (neighbouring tiles of one shot).
-// Block 0 also resolves the next step's per-slot views into `next`.  `now` and `next`
-// are the two halves of a double buffer (chosen by the parity of the step within its
-// chunk); `now` is only read here, so its words are fetched with scalar loads.
+// The slot descriptors are read-only during a launch (the host rewrites them between
+// chunks), so their words are fetched with scalar loads.
 template <int TX>
-__global__ __launch_bounds__(kBlock) void rtm_batch_kernel(BatchArgs b, const StepArgs *__restrict__ now,
-                                                           StepArgs *__restrict__ next) {
+__global__ __launch_bounds__(kBlock) void rtm_batch_kernel(BatchArgs b, const SlotDesc *__restrict__ slots,
+                                                           const long long *__restrict__ gbase) {
     __shared__ float tile[(TX + 6) * (kTileY + 6)];
     __shared__ float red[4];
     __shared__ float aux[(TX + 6) * kTileY];
-    const long long G = *b.gbase + b.goff;
-    if (blockIdx.x == 0)
-        for (int s = threadIdx.x; s < b.nslots; s += blockDim.x) resolve_slot(b.slots[s], G + 1, &next[s]);
+    const long long G = *gbase + b.goff;
     const int x = blockIdx.x & 7, nbx = gridDim.x >> 3;
     for (int m = blockIdx.x >> 3; m < b.per_xcd; m += nbx) {
         const int wk = x * b.per_xcd + m;
         if (wk >= b.nwork) break;
         const int s = wk / b.cap_tiles, t = wk - s * b.cap_tiles;
-        const StepArgs &a = now[s];
-        if (a.k == 0 || t >= a.ntiles) continue;
+        const SlotDesc &sd = slots[s];
+        StepArgs a = sd.a;
+        if (!resolve_step(sd, G, a) || t >= a.ntiles) continue;
         step_tile<TX>(a, t, tile, aux, red);
         __syncthreads();  // the LDS buffers are reused by the next item
     }
-}
-
-// Resolves the per-slot views of step G = *gbase, the first step of a chunk, into the
-// even half of the double buffer.
-__global__ void rtm_resolve_kernel(const SlotDesc *slots, int nslots, const long long *gbase, StepArgs *cur) {
-    const long long G = *gbase;
-    for (int s = threadIdx.x; s < nslots; s += blockDim.x) resolve_slot(slots[s], G, &cur[s]);
 }
 
 __global__ void rtm_advance_kernel(long long *gbase, long long by) {
@@ -554,34 +541,36 @@ __global__ void rtm_normalize_kernel(const double *sum, const int *fold, float *
 
 // hipExtLaunchKernelGGL stamps start/stop on the dispatch packet itself, so a sampled
 // step reports the kernel's own duration (as a profiler's kernel trace does).
+template <int TX>
+void launch_step_tx(const StepArgs &a, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+    hipExtLaunchKernelGGL(rtm_step_kernel<TX>, dim3(8 * a.per_xcd), dim3(kBlock), 0, s, e0, e1, 0, a);
+}
+
 void launch_step(const StepArgs &a, int tile_x, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
-    const int grid = 8 * a.per_xcd;
-    if (tile_x == 16)
-        hipExtLaunchKernelGGL(rtm_step_kernel<16>, dim3(grid), dim3(kBlock), 0, s, e0, e1, 0, a);
+    if (tile_x == 32)
+        launch_step_tx<32>(a, s, e0, e1);
+    else if (tile_x == 16)
+        launch_step_tx<16>(a, s, e0, e1);
     else
-        hipExtLaunchKernelGGL(rtm_step_kernel<8>, dim3(grid), dim3(kBlock), 0, s, e0, e1, 0, a);
+        launch_step_tx<8>(a, s, e0, e1);
 }
 
-void launch_batch(const BatchArgs &b, StepArgs *cur, int tile_x, int grid, hipStream_t s, hipEvent_t e0,
-                  hipEvent_t e1) {
+template <int TX>
+void launch_batch_tx(const BatchArgs &b, int grid, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+    if (!e0 && !e1)  // plain launch: also valid under stream capture
+        hipLaunchKernelGGL(rtm_batch_kernel<TX>, dim3(grid), dim3(kBlock), 0, s, b, b.slots, b.gbase);
+    else
+        hipExtLaunchKernelGGL(rtm_batch_kernel<TX>, dim3(grid), dim3(kBlock), 0, s, e0, e1, 0, b, b.slots, b.gbase);
+}
+
+void launch_batch(const BatchArgs &b, int tile_x, int grid, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
     grid = (grid + 7) / 8 * 8;
-    const StepArgs *now = cur + (b.goff & 1) * b.nslots;
-    StepArgs *next = cur + ((b.goff + 1) & 1) * b.nslots;
-    if (!e0 && !e1) {  // plain launch: also valid under stream capture
-        if (tile_x == 16)
-            hipLaunchKernelGGL(rtm_batch_kernel<16>, dim3(grid), dim3(kBlock), 0, s, b, now, next);
-        else
-            hipLaunchKernelGGL(rtm_batch_kernel<8>, dim3(grid), dim3(kBlock), 0, s, b, now, next);
-        return;
-    }
-    if (tile_x == 16)
-        hipExtLaunchKernelGGL(rtm_batch_kernel<16>, dim3(grid), dim3(kBlock), 0, s, e0, e1, 0, b, now, next);
+    if (tile_x == 32)
+        launch_batch_tx<32>(b, grid, s, e0, e1);
+    else if (tile_x == 16)
+        launch_batch_tx<16>(b, grid, s, e0, e1);
     else
-        hipExtLaunchKernelGGL(rtm_batch_kernel<8>, dim3(grid), dim3(kBlock), 0, s, e0, e1, 0, b, now, next);
-}
-
-void launch_resolve(const SlotDesc *slots, int nslots, const long long *gbase, StepArgs *cur, hipStream_t s) {
-    hipLaunchKernelGGL(rtm_resolve_kernel, dim3(1), dim3(256), 0, s, slots, nslots, gbase, cur);
+        launch_batch_tx<8>(b, grid, s, e0, e1);
 }
 
 void launch_advance(long long *gbase, long long by, hipStream_t s) {

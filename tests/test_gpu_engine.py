@@ -136,13 +136,15 @@ def test_engine_shot_alone_equals_shot_in_mix(ctx, fmt):
 
 def test_engine_graph_equals_eager_and_grid_shapes(ctx):
     vel = model(2)
-    say("[gpu] engine graph vs eager vs persistent grid")
+    say("[gpu] engine graph vs eager vs persistent grid vs tile_x 32")
     a, _, _ = run_engine(ctx, vel, SHOTS, [0, 1, 3], nslots=2, use_graph=True, chunk_steps=24)
     b, _, _ = run_engine(ctx, vel, SHOTS, [0, 1, 3], nslots=2, use_graph=False, chunk_steps=7)
     c, _, _ = run_engine(ctx, vel, SHOTS, [0, 1, 3], nslots=3, grid=64, tile_x=16, chunk_steps=40)
+    d, _, _ = run_engine(ctx, vel, SHOTS, [0, 1, 3], nslots=2, tile_x=32, chunk_steps=64)
     for q in (0, 1, 3):
         np.testing.assert_array_equal(a[q], b[q])
         np.testing.assert_array_equal(a[q], c[q])
+        np.testing.assert_array_equal(a[q], d[q])
 
 
 def test_engine_hbm_budget_admission(ctx):

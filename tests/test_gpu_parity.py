@@ -59,7 +59,7 @@ def oracle_rtm(c, table_mode=0):
     return Rec, Fields, kernel
 
 
-@pytest.mark.parametrize("tile_x", [8, 16])
+@pytest.mark.parametrize("tile_x", [8, 16, 32])
 @pytest.mark.parametrize("name", list(CASES))
 def test_gpu_matches_oracle_bit_exact_and_reference(ctx, name, tile_x):
     c = case(name)
