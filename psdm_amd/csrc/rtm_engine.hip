@@ -92,7 +92,8 @@ struct Slot {
 };
 
 struct DoneRec {
-    long long id, chunk;
+    long long id;
+    hipEvent_t ev;  // recorded after the shot's image is stacked
 };
 
 double now_s() {
@@ -129,7 +130,8 @@ struct psdm_rtm_engine {
     hipEvent_t ev_chunk[kRing] = {};
     hipEvent_t ev_origin = nullptr;
     double t_origin = 0.0;
-    std::vector<DoneRec> done_wait;  // retired, waiting for their chunk to complete
+    std::vector<DoneRec> done_wait;  // retired, waiting for their stack to complete
+    std::vector<hipEvent_t> ev_pool;  // completion events ready for reuse
     std::vector<std::pair<long long, double>> done_ready;
     double chunk_ms_sum = 0.0;
     long long chunk_ms_n = 0;
@@ -330,8 +332,15 @@ int retire(psdm_rtm_engine *e, int s) {
     launch_stack(img, e->cfg.kahan ? comp : nullptr, e->lds, ny, J.nx_w, J.x_i - 1, e->d_sum, ny, e->d_fold,
                  c->stream);
     e->arena.release(sl.arena_off);
-    const int C = e->cfg.chunk_steps;
-    e->done_wait.push_back({sl.job.id, (sl.g_end - 1) / C});
+    hipEvent_t ev;
+    if (!e->ev_pool.empty()) {
+        ev = e->ev_pool.back();
+        e->ev_pool.pop_back();
+    } else {
+        HIP_TRY(c, hipEventCreate(&ev));
+    }
+    HIP_TRY(c, hipEventRecord(ev, c->stream));
+    e->done_wait.push_back({sl.job.id, ev});
     sl.busy = false;
     SlotDesc &d = e->h_slots[e->h_ring * e->cfg.nslots + s];
     d.g0 = -1;
@@ -358,16 +367,18 @@ int upload_slots(psdm_rtm_engine *e) {
 // Reports jobs whose last chunk has completed.  Entries whose chunk is at or before
 // `block_upto` are waited for (their ring event is about to be reused); the rest are
 // only queried.
-int collect(psdm_rtm_engine *e, long long block_upto) {
+// Moves retired shots whose image stack has completed on the device to done_ready
+// (waiting for them when `block`), with the completion time on the host steady clock.
+int collect(psdm_rtm_engine *e, bool block) {
     psdm_rtm_ctx *c = e->c;
     std::vector<DoneRec> keep;
     for (auto &d : e->done_wait) {
-        hipEvent_t ev = e->ev_chunk[d.chunk % psdm_rtm_engine::kRing];
-        hipError_t q = d.chunk <= block_upto ? hipEventSynchronize(ev) : hipEventQuery(ev);
+        hipError_t q = block ? hipEventSynchronize(d.ev) : hipEventQuery(d.ev);
         if (q == hipSuccess) {
             float ms = 0.f;
-            HIP_TRY(c, hipEventElapsedTime(&ms, e->ev_origin, ev));
+            HIP_TRY(c, hipEventElapsedTime(&ms, e->ev_origin, d.ev));
             e->done_ready.emplace_back(d.id, e->t_origin + 1e-3 * ms);
+            e->ev_pool.push_back(d.ev);
         } else if (q == hipErrorNotReady) {
             keep.push_back(d);
         } else {
@@ -500,6 +511,8 @@ int psdm_rtm_engine_destroy(psdm_rtm_engine *e) {
     for (auto ev : e->ev_chunk)
         if (ev) (void)hipEventDestroy(ev);
     if (e->ev_origin) (void)hipEventDestroy(e->ev_origin);
+    for (auto ev : e->ev_pool) (void)hipEventDestroy(ev);
+    for (auto &d : e->done_wait) (void)hipEventDestroy(d.ev);
     delete e;
     return 0;
 }
@@ -563,8 +576,6 @@ int psdm_rtm_engine_run(psdm_rtm_engine *e, int max_chunks, int *n_active, int *
             hipEvent_t old = e->ev_chunk[(e->chunks - inflight) % psdm_rtm_engine::kRing];
             HIP_TRY(c, hipEventSynchronize(old));
         }
-        // the ring event about to be re-recorded must not still be awaited by a job
-        if (int rc = collect(e, e->chunks - psdm_rtm_engine::kRing)) return rc;
         if (int rc = issue_chunk(e)) return rc;
         for (auto &sl : e->slots)
             if (sl.busy) e->active_slot_steps += std::min<long long>(C, sl.g_end - e->G);
@@ -572,7 +583,7 @@ int psdm_rtm_engine_run(psdm_rtm_engine *e, int max_chunks, int *n_active, int *
         HIP_TRY(c, hipEventRecord(e->ev_chunk[e->chunks % psdm_rtm_engine::kRing], c->stream));
         e->chunks++;
     }
-    if (int rc = collect(e, -1)) return rc;
+    if (int rc = collect(e, false)) return rc;
     int na = 0;
     for (auto &sl : e->slots) na += sl.busy;
     if (n_active) *n_active = na;
@@ -582,7 +593,7 @@ int psdm_rtm_engine_run(psdm_rtm_engine *e, int max_chunks, int *n_active, int *
 
 int psdm_rtm_engine_poll(psdm_rtm_engine *e, long long *ids, double *t_done, int cap, int *n) {
     if (!e) return PSDM_RTM_EINVAL;
-    if (int rc = collect(e, -1)) return rc;
+    if (int rc = collect(e, false)) return rc;
     const int k = std::min<int>(cap, (int)e->done_ready.size());
     for (int q = 0; q < k; q++) {
         if (ids) ids[q] = e->done_ready[q].first;
@@ -601,7 +612,7 @@ int psdm_rtm_engine_drain(psdm_rtm_engine *e) {
     // retire the last shots and wait for everything
     if (int rc = psdm_rtm_engine_run(e, 1, &na, &np)) return rc;
     HIP_TRY(e->c, hipStreamSynchronize(e->c->stream));
-    return collect(e, e->chunks);
+    return collect(e, true);
 }
 
 int psdm_rtm_engine_image(psdm_rtm_engine *e, double **d_psdm_sum, int **d_cov_sum) {

@@ -122,3 +122,55 @@ def step_bytes(ny: int, nx_w: int, dPML: int, snap_bytes: float = 4.0, snap_ever
     n_int = ny * nx_w
     per = (snap_bytes + (8.0 if backward else 0.0)) / snap_every
     return 16.0 * n_pad + 16.0 * n_strip + 32.0 * n_corner + per * n_int
+
+
+def _mcs_shot(index, xs, dx, model_nx, nrec, drec, near_offset, shot_depth, streamer_depth, added_space,
+              dir_boat=-1):
+    from .solvers import dynamic_window
+
+    pos_rec = xs - dir_boat * (near_offset + np.arange(nrec) * drec)
+    nxR = (1 + np.floor(pos_rec / dx)).astype(np.int32)
+    nyR = np.full(nrec, 1 + int(np.floor(streamer_depth / dx)), np.int32)
+    sh = Shot(index, xs, 1 + int(np.floor(xs / dx)), 1 + int(np.floor(shot_depth / dx)), nxR, nyR)
+    sh.nx_w, sh.x_i, sh.x_f = dynamic_window([sh.nxSou], nxR, dx, model_nx, added_space)
+    return sh
+
+
+def uneven_line(nshots: int = 256, seed: int = 0, line_km: float = 40.0, nz: int = 601, dx: float = 10.0,
+                nx_range=(600, 2000), nts=(4000, 6000, 8000)):
+    """Shots along a `line_km` line whose aperture windows are uneven: each shot's window
+    width is drawn uniformly from `nx_range` columns (the streamer is sized to it: 12.5 m
+    channels from 150 m near offset, 500 m added space each side) and its record length
+    from `nts` (dt 0.8 ms).  Returns dict(model, shots, nt per shot, dt, f0, dPML, fs)."""
+    rng = np.random.default_rng(seed)
+    model_nx = int(round(line_km * 1000.0 / dx)) + 1
+    m = velocity_model(nz, model_nx, dx, 1500.0, seed=seed)
+    drec, near, added = 12.5, 150.0, 500.0
+    ddx = 1 + int(np.floor(added / dx))
+    shots, nt = [], []
+    for s in range(nshots):
+        w = int(rng.integers(nx_range[0], nx_range[1] + 1))
+        span_cols = max(w - 2 * ddx - int(near / dx), 2)
+        nrec = max(2, int(np.floor(span_cols * dx / drec)) + 1)
+        reach = near + (nrec - 1) * drec + added
+        xs = float(rng.uniform(added + dx, (model_nx - 1) * dx - reach - dx))
+        shots.append(_mcs_shot(s, xs, dx, model_nx, nrec, drec, near, 6.0, 8.0, added))
+        nt.append(int(rng.choice(nts)))
+    return dict(model=m, shots=shots, nt=nt, dt=0.0008, f0=15.0, dPML=40, fs=0)
+
+
+def was_gathers(n: int, seed: int = 0, nz: int = 801, nx: int = 6001, dx: float = 10.0, offset_km: float = 50.0,
+                shot_spacing: float = 50.0):
+    """Long-offset wide-angle gathers treated by reciprocity: a seafloor receiver (OBS)
+    acts as the source and the shots of one source line (6 m depth, every
+    `shot_spacing` m out to `offset_km`) act as receivers; grid nz x nx at dx,
+    dt 1 ms, nt 12000, 8 Hz Ricker.  OBS positions are drawn so the line fits the model."""
+    rng = np.random.default_rng(seed)
+    m = velocity_model(nz, nx, dx, 1500.0, seed=seed)
+    nrec = int(offset_km * 1000.0 / shot_spacing)
+    out = []
+    for s in range(n):
+        x_obs = float(rng.uniform(600.0, (nx - 1) * dx - offset_km * 1000.0 - 600.0))
+        sh = _mcs_shot(s, x_obs, dx, nx, nrec, shot_spacing, shot_spacing, 1500.0, 6.0, 500.0)
+        out.append(sh)
+    return dict(model=m, shots=out, nt=12000, dt=0.001, f0=8.0, dPML=40, fs=0)
