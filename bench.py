@@ -48,18 +48,20 @@ def cpu_leg(ny, nx_w, dPML, dx, dt, nprocs, nt_sample, nt_full):
 
 
 def accuracy_check(ctx):
-    """Single-shot 401 x 201 @ 12.5 m workload cut to 600 steps: GPU image and traces vs
-    the FP64 oracle (and the FP32 oracle, which the GPU reproduces bit for bit)."""
+    """Single-shot 401 x 201 @ 12.5 m workload cut to 600 steps: the gather is modelled on
+    the true model on the GPU, migrated on the GPU (plain form), and the image and the
+    forward traces are compared with the FP64 oracle's, kept as a fixture in
+    tests/golden (accuracy_single401_fp64.npz)."""
     import numpy as np
 
-    from oracle import pyoracle as O
     from psdm_amd import solvers as S
     from psdm_amd.workloads import line_workload
     from tests.cases import rel_l2
 
+    fx = np.load(os.path.join(ROOT, "tests", "golden", "accuracy_single401_fp64.npz"))
     W = line_workload("single401")
     m, sh = W["model"], W["shots"][0]
-    nt, dt, dPML = 600, W["dt"], W["dPML"]
+    nt, dt, dPML = int(fx["nt"]), W["dt"], W["dPML"]
     ctmp = np.asfortranarray(m.migration[:, sh.x_i - 1:sh.x_f])
     ctrue = np.asfortranarray(m.true[:, sh.x_i - 1:sh.x_f])
     src = S.ricker_mcs(nt, dt, W["f0"], 1.2 / W["f0"]).reshape(nt, 1)
@@ -68,19 +70,23 @@ def accuracy_check(ctx):
     ny, nx = ctmp.shape
     par = S.ParFile(dPML=dPML, fs=0)
     NumRec = nxR.size
-    d, _ = O.forward(nxS, nyS, nxR, nyR, src, m.dx, dt, ctrue, 1, nt, nt, dPML, 0)
+    d = np.zeros((nt, NumRec), np.float32, order="F")
+    F = np.zeros((ny, nx, nt), np.float32, order="F")
+    S.solver_ac_forward_ram(0, 1, NumRec, nxS, nyS, nxR, nyR, src, m.dx, dt, ny, nx, ctrue, 1, nt, nt, F, d,
+                            par=par, host_fields=False, ctx=ctx)
+    same_gather = (abs(float(np.sum(d.astype(np.float64) ** 2)) - float(fx["data_sumsq"])) <=
+                   1e-12 * float(fx["data_sumsq"]) and
+                   np.array_equal(d.reshape(-1, order="F")[fx["data_pick"]], fx["data_sample"]))
     Sadj = np.asfortranarray(d[::-1, :])
     Rec = np.zeros((nt, NumRec), np.float32, order="F")
-    F = np.zeros((ny, nx, nt), np.float32, order="F")
     k = np.zeros((ny, nx), np.float32, order="F")
     S.solver_ac_forward_ram(0, 1, NumRec, nxS, nyS, nxR, nyR, src, m.dx, dt, ny, nx, ctmp, 1, nt, nt, F, Rec,
                             par=par, host_fields=False, ctx=ctx)
     S.solver_ac_backward_ram(0, 1, NumRec, nxS, nyS, nxR, nyR, Sadj, m.dx, dt, ny, nx, ctmp, 1, nt, nt, F, k,
                              par=par, host_fields=False, ctx=ctx)
-    R64, F64 = O.forward(nxS, nyS, nxR, nyR, src, m.dx, dt, ctmp, 1, nt, nt, dPML, 0, precision="f64")
-    k64 = O.backward(nxS, nyS, nxR, nyR, Sadj, m.dx, dt, ctmp, 1, nt, nt, dPML, 0, Fields=F64, precision="f64")
-    return {"workload": f"single401 shot, {ny}x{nx} window, {nt} steps", "image_rel_l2_vs_fp64": rel_l2(k, k64),
-            "traces_rel_l2_vs_fp64": rel_l2(Rec, R64)}
+    return {"workload": f"single401 shot, {ny}x{nx} window, {nt} steps", "reference": "FP64 oracle fixture",
+            "gather_matches_fixture": bool(same_gather), "image_rel_l2_vs_fp64": rel_l2(k, fx["image_fp64"]),
+            "traces_rel_l2_vs_fp64": rel_l2(Rec, fx["traces_fp64"])}
 
 
 def model_gathers(ctx, torch, dev, W, geos_model, d_true, d_wat, d_src, nshots):
@@ -181,6 +187,258 @@ def job_of(W, k, d_src, obs):
                 nxRec=sh.nxRec - sh.x_i + 1, nyRec=sh.nyRec, d_wavelet=d_src[0], d_data=obs[k])
 
 
+# ---------------------------------------------------------------------------------
+# Other workloads (not the default line): uneven 256-shot line (batched vs plain),
+# mixed-precision snapshots on it, and the served stream.  Each prints its own JSON line.
+
+
+def interior_ld(ny):
+    return -(-ny // 64) * 64
+
+
+def snap_store_bytes(ny, nx_w, nt, store_snap, esize, ntiles=0):
+    """Device bytes of one shot's snapshot store (rows padded to a multiple of 64, plus
+    one scale per snapshot tile for 16-bit formats)."""
+    nsnap = (nt - 1) // store_snap
+    b = (nsnap * interior_ld(ny) * nx_w * esize + 255) // 256 * 256
+    return b + (nsnap * ntiles * 4 if esize == 2 else 0)
+
+
+def random_gathers(torch, dev, shots, nts, seed):
+    """Seeded noise gathers (NumRec, nt) on the device: throughput does not depend on the
+    recorded values."""
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    return {k: 1e-3 * torch.randn((sh.nxRec.size, nts[k]), generator=g, device=dev) for k, sh in enumerate(shots)}
+
+
+def uneven_jobs(W, d_src, obs, idx, nt_override=None):
+    out = []
+    for k in idx:
+        sh = W["shots"][k]
+        nt = nt_override or W["nt"][k]
+        out.append(dict(x_i=sh.x_i, nx_w=sh.nx_w, nt=nt, nxSou=sh.nxSou - sh.x_i + 1, nySou=sh.nySou,
+                        nxRec=sh.nxRec - sh.x_i + 1, nyRec=sh.nyRec, d_wavelet=d_src[0], d_data=obs[k]))
+    return out
+
+
+def run_engine_jobs(ctx, torch, dev, d_mig, ny, nx_prop, jobs, W, cfg, budget, images=None):
+    """Migrates `jobs` through one engine; returns elapsed seconds and stats.  `images`:
+    optional list of device tensors (nx_w, ny) receiving each job's image."""
+    from psdm_amd.engine import Engine
+
+    eng = Engine(ctx, d_mig, ny, nx_prop, nslots=cfg["nslots"], max_nx=max(j["nx_w"] for j in jobs),
+                 max_nt=max(j["nt"] for j in jobs), dPML=W["dPML"], fs=W["fs"], dmodel=W["model"].dx, dt=W["dt"],
+                 tile_x=cfg["tile_x"], grid=cfg.get("grid", 0), snap_format=cfg.get("snap_format", "f32"),
+                 store_snap=cfg.get("store_snap", 1), kahan=cfg.get("kahan", False),
+                 chunk_steps=cfg.get("chunk_steps", 64), hbm_budget=budget)
+    for q, j in enumerate(jobs):
+        eng.submit(**j, d_image_out=images[q] if images is not None else None)
+    ctx.sync()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    eng.drain()
+    ctx.sync()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    st = eng.stats()
+    eng.close()
+    return el, st
+
+
+def bench_uneven(args, mixed=False):
+    """Uneven 256-shot line (windows 600-2000 columns, nt 4000/6000/8000): the batched
+    engine vs the plain one-shot form on the same shots; with `mixed`, FP32 / FP16 / BF16
+    snapshots every 4th step (Kahan image) against FP32 every step: throughput, snapshot
+    HBM and speedup (their accuracy against the FP64 oracle is measured by
+    tests/accuracy_report.py)."""
+    import numpy as np
+    import torch
+
+    from psdm_amd._native import Context
+    from psdm_amd.solvers import DeviceShot, backward_device, forward_device, ricker_mcs
+    from psdm_amd.workloads import step_bytes, uneven_line, updates_per_shot
+
+    W = uneven_line(256, seed=0)
+    m, shots = W["model"], W["shots"][:args.shots]
+    nts = W["nt"][:args.shots]
+    ny, dPML = m.nz, W["dPML"]
+    dev = torch.device("cuda:0")
+    ctx = Context(0)
+    d_mig = torch.from_numpy(np.ascontiguousarray(m.migration.T)).to(dev)
+    ntmax = max(nts)
+    d_src = torch.from_numpy(ricker_mcs(ntmax, W["dt"], W["f0"], 1.2 / W["f0"]).reshape(1, ntmax)).to(dev)
+    obs = random_gathers(torch, dev, shots, nts, seed=0)
+    torch.cuda.synchronize()
+    jobs = uneven_jobs(W, d_src, obs, range(len(shots)))
+    upd = sum(updates_per_shot(ny, sh.nx_w, nts[k], dPML) for k, sh in enumerate(shots))
+    budget = int(args.hbm_gb * 2 ** 30)
+    log(f"[bench] uneven line: {len(shots)} shots, windows {min(s.nx_w for s in shots)}-"
+        f"{max(s.nx_w for s in shots)}, {upd / 1e12:.2f} T updates, snapshot budget {args.hbm_gb} GB")
+
+    cfg = dict(nslots=args.slots, tile_x=32, grid=0, chunk_steps=64)
+    forms = [("f32", 1, False)] + ([("f32", 4, True), ("f16", 4, True), ("bf16", 4, True)] if mixed else [])
+    results = {}
+    for fmt, ss, kahan in forms:
+        c = dict(cfg, snap_format=fmt, store_snap=ss, kahan=kahan)
+        log(f"[bench] batched {fmt} every {ss}: {c}")
+        el, st = run_engine_jobs(ctx, torch, dev, d_mig, ny, m.nx, jobs, W, c, budget)
+        esize = 4 if fmt == "f32" else 2
+        sb = sum(snap_store_bytes(ny, sh.nx_w, nts[k], ss, esize) for k, sh in enumerate(shots))
+        byts = sum(0.5 * (step_bytes(ny, sh.nx_w, dPML, esize, ss) + step_bytes(ny, sh.nx_w, dPML, esize, ss, True))
+                   * 2 * (nts[k] - 1) for k, sh in enumerate(shots))
+        r = {"snap_format": fmt, "store_snap": ss, "kahan": kahan, "shots_per_s": round(len(shots) / el, 4),
+             "gpt_updates_per_s": round(upd / el / 1e9, 3), "elapsed_s": round(el, 3),
+             "snapshot_bytes_total": int(sb), "hbm_roofline_frac": round(byts / el / (HBM_PEAK_GBS * 1e9), 4),
+             "steps_issued": st["steps_issued"], "slot_steps": st["slot_steps"]}
+        results[f"{fmt}_ss{ss}"] = r
+        log(f"[bench]   {r['shots_per_s']} shots/s, {r['gpt_updates_per_s']} Gupd/s, snapshots "
+            f"{sb / 2 ** 30:.1f} GiB")
+
+    plain = None
+    if not args.no_plain:
+        log(f"[bench] plain: {len(shots)} shots one by one")
+        ctx.set_launch(tile_x=args.tile_x)
+        d_kernel = torch.zeros((max(s.nx_w for s in shots), ny), device=dev)
+        d_rec = torch.empty((max(s.nxRec.size for s in shots), ntmax), device=dev)
+        geos = [DeviceShot(ny, sh.nx_w, nts[k], dPML, W["fs"], m.dx, W["dt"], [sh.nxSou - sh.x_i + 1], [sh.nySou],
+                           sh.nxRec - sh.x_i + 1, sh.nyRec) for k, sh in enumerate(shots)]
+        psdm_sum = torch.zeros((m.nx, ny), dtype=torch.float64, device=dev)
+        fold = torch.zeros((m.nx, ny), dtype=torch.int32, device=dev)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k, sh in enumerate(shots):
+            win = d_mig[sh.x_i - 1:sh.x_f]
+            rev_k = obs[k].flip(1).contiguous()  # BackSource = Data(nt:1:-1)
+            forward_device(ctx, geos[k], win, d_src, d_rec, key=k)
+            backward_device(ctx, geos[k], win, rev_k, d_kernel, key=k)
+            ctx.check(ctx._L.psdm_rtm_stack_image(ctx.h, d_kernel.data_ptr(), ny, sh.nx_w, sh.x_i,
+                                                  psdm_sum.data_ptr(), fold.data_ptr(), m.nx), "stack_image")
+        ctx.sync()
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        plain = {"shots_per_s": round(len(shots) / el, 4), "gpt_updates_per_s": round(upd / el / 1e9, 3),
+                 "elapsed_s": round(el, 3), "tile_x": args.tile_x}
+        log(f"[bench] plain {plain['shots_per_s']} shots/s")
+
+    base = results["f32_ss1"]
+    if mixed:
+        for key, r in results.items():
+            r["speedup_vs_f32_every_step"] = round(r["shots_per_s"] / base["shots_per_s"], 3)
+            r["snapshot_hbm_saving_vs_f32_every_step"] = round(base["snapshot_bytes_total"] /
+                                                               r["snapshot_bytes_total"], 3)
+    head = results["f16_ss4"] if mixed else base
+    out = {
+        "metric": "migrated shots/s + Gpt-updates/s, batched vs plain, at rel-L2<=1e-4; p99 latency",
+        "value": head["shots_per_s"], "unit": "shots/s", "n_gpus": 1, "steps": len(shots), "warmup": 0,
+        "ms_per_step": round(1e3 / head["shots_per_s"], 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded 4001x601 model; seeded noise gathers)",
+        "config": {"workload": f"uneven line: {len(shots)} shots on a 40 km 601-deep model @10 m, windows "
+                               f"600-2000 columns, nt in {{4000,6000,8000}}, dt 0.8 ms, dPML 40",
+                   "form": "batched" + (" FP16 snapshots every 4th step" if mixed else ""),
+                   "slots": args.slots, "tile_x": 32, "snapshot_budget_gb": args.hbm_gb},
+        "gpt_updates_per_s": head["gpt_updates_per_s"],
+        "forms": results, "plain": plain,
+        "speedup_batched_vs_plain": round(base["shots_per_s"] / plain["shots_per_s"], 3) if plain else None,
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+def bench_served(args):
+    """Served stream: Poisson arrivals at 4 rates; 70% MCS gathers (480 ch @12.5 m, 8 km
+    aperture, 601-deep model @10 m, nt 6000, FP32 snapshots every step) and 30% WAS
+    gathers (801 x 6001 model @10 m, 50 km offset, dt 1 ms, nt 12000, 8 Hz, FP16
+    snapshots every 4th step); one engine per model, admission by each engine's HBM
+    budget; per-rate p50/p99 latency (arrival -> image stacked), shots/s, occupancy."""
+    import numpy as np
+    import torch
+
+    from psdm_amd._native import Context
+    from psdm_amd.engine import Engine
+    from psdm_amd.serve import Request, Server, poisson_arrivals
+    from psdm_amd.solvers import ricker_mcs
+    from psdm_amd.workloads import _mcs_shot, velocity_model, was_gathers
+
+    dev = torch.device("cuda:0")
+    ctx = Context(0)
+    dx = 10.0
+    mcs_model = velocity_model(601, 4001, dx, 1500.0, seed=1)
+    added = (8000.0 - 150.0 - 479 * 12.5) / 2  # 8 km aperture
+    rng = np.random.default_rng(5)
+    n = args.requests
+    nmcs = int(round(0.7 * n))
+    mcs_shots = [_mcs_shot(q, float(rng.uniform(1000.0, 40000.0 - 8000.0)), dx, 4001, 480, 12.5, 150.0, 6.0, 8.0,
+                           added) for q in range(nmcs)]
+    WAS = was_gathers(n - nmcs, seed=2)
+    was_model, was_shots = WAS["model"], WAS["shots"]
+    nt_m, dt_m, nt_w, dt_w = 6000, 0.0008, WAS["nt"], WAS["dt"]
+    d_vm = torch.from_numpy(np.ascontiguousarray(mcs_model.migration.T)).to(dev)
+    d_vw = torch.from_numpy(np.ascontiguousarray(was_model.migration.T)).to(dev)
+    s_m = torch.from_numpy(ricker_mcs(nt_m, dt_m, 15.0, 1.2 / 15.0)).to(dev)
+    s_w = torch.from_numpy(ricker_mcs(nt_w, dt_w, 8.0, 1.2 / 8.0)).to(dev)
+    obs_m = random_gathers(torch, dev, mcs_shots, [nt_m] * nmcs, seed=3)
+    obs_w = random_gathers(torch, dev, was_shots, [nt_w] * len(was_shots), seed=4)
+    torch.cuda.synchronize()
+
+    def mk_req(kind, q, sh, nt, d_s, d_o):
+        job = dict(x_i=sh.x_i, nx_w=sh.nx_w, nt=nt, nxSou=sh.nxSou - sh.x_i + 1, nySou=sh.nySou,
+                   nxRec=sh.nxRec - sh.x_i + 1, nyRec=sh.nyRec, d_wavelet=d_s, d_data=d_o)
+        return kind, job
+
+    pool_reqs = [mk_req("mcs", q, sh, nt_m, s_m, obs_m[q]) for q, sh in enumerate(mcs_shots)]
+    pool_reqs += [mk_req("was", q, sh, nt_w, s_w, obs_w[q]) for q, sh in enumerate(was_shots)]
+    order = np.random.default_rng(6).permutation(len(pool_reqs))
+    gb = 2 ** 30
+    mcs_cfg = dict(nslots=args.slots, max_nx=max(s.nx_w for s in mcs_shots), max_nt=nt_m, snap_format="f32",
+                   store_snap=1, kahan=False, budget=int(args.hbm_gb * 0.62 * gb))
+    was_cfg = dict(nslots=4, max_nx=max(s.nx_w for s in was_shots), max_nt=nt_w, snap_format="f16", store_snap=4,
+                   kahan=True, budget=int(args.hbm_gb * 0.38 * gb))
+    log(f"[bench] served: {n} requests ({nmcs} MCS windows ~{mcs_shots[0].nx_w}, {n - nmcs} WAS windows "
+        f"{was_shots[0].nx_w}), rates {args.rates}")
+
+    def engines():
+        e = {"mcs": Engine(ctx, d_vm, 601, 4001, nslots=mcs_cfg["nslots"], max_nx=mcs_cfg["max_nx"],
+                           max_nt=nt_m, dPML=40, fs=0, dmodel=dx, dt=dt_m, tile_x=32, snap_format="f32",
+                           store_snap=1, hbm_budget=mcs_cfg["budget"]),
+             "was": Engine(ctx, d_vw, 801, 6001, nslots=was_cfg["nslots"], max_nx=was_cfg["max_nx"], max_nt=nt_w,
+                           dPML=40, fs=0, dmodel=dx, dt=dt_w, tile_x=32, snap_format="f16", store_snap=4,
+                           kahan=True, hbm_budget=was_cfg["budget"])}
+        return e
+
+    rates = [float(r) for r in args.rates.split(",")]
+    per_rate = []
+    for ri, rate in enumerate(rates):
+        arr = poisson_arrivals(rate, n, seed=10 + ri)
+        reqs = [Request(q, float(arr[q]), pool_reqs[order[q]][0], pool_reqs[order[q]][1], kind=pool_reqs[order[q]][0])
+                for q in range(n)]
+        eng = engines()
+        log(f"[bench] served: rate {rate}/s")
+        res = Server(eng).run(reqs, log=log)
+        for e in eng.values():
+            e.close()
+        res.pop("requests")
+        res["rate_per_s"] = rate
+        per_rate.append(res)
+    best = max(per_rate, key=lambda r: r["shots_per_s"])
+    out = {
+        "metric": "migrated shots/s + Gpt-updates/s, batched vs plain, at rel-L2<=1e-4; p99 latency",
+        "value": round(best["shots_per_s"], 4), "unit": "shots/s", "n_gpus": 1, "steps": n, "warmup": 0,
+        "ms_per_step": round(1e3 / best["shots_per_s"], 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded models; seeded noise gathers; seeded Poisson arrivals)",
+        "config": {"workload": "served stream: 70% MCS (480 ch @12.5 m, 8 km aperture, 601 deep @10 m, nt 6000, "
+                               "dt 0.8 ms, FP32 snapshots) + 30% WAS (801x6001 @10 m, 50 km offset, nt 12000, "
+                               "dt 1 ms, 8 Hz, FP16 snapshots every 4th step)", "form": "served",
+                   "mcs_engine": {k: v for k, v in mcs_cfg.items()}, "was_engine": {k: v for k, v in was_cfg.items()}},
+        "per_rate": per_rate,
+        "p99_latency_s_at_best_rate": best["latency"]["p99_s"],
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -194,6 +452,15 @@ def main():
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="batched form with the default launch shape")
     ap.add_argument("--forms", default="plain,batched")
+    ap.add_argument("--workload", default="mcs2001", choices=["mcs2001", "uneven", "mixed", "served"],
+                    help="mcs2001: the 64-shot line (default line); uneven: 256 uneven shots, batched vs plain; "
+                         "mixed: uneven + FP16/BF16 snapshots every 4th step; served: Poisson-arrival stream")
+    ap.add_argument("--shots", type=int, default=256, help="uneven/mixed: shots of the 256-shot line used")
+    ap.add_argument("--slots", type=int, default=16, help="uneven/mixed/served: engine slots")
+    ap.add_argument("--hbm-gb", type=float, default=200.0, help="uneven/mixed/served: snapshot budget (GiB)")
+    ap.add_argument("--no-plain", action="store_true", help="uneven/mixed: skip the plain form")
+    ap.add_argument("--requests", type=int, default=40, help="served: requests per rate")
+    ap.add_argument("--rates", default="0.5,1,2,4", help="served: Poisson arrival rates (requests/s)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -201,6 +468,10 @@ def main():
               f"(--gpus {args.gpus} is not supported; shots shard across GPUs with no per-step "
               f"communication)", file=sys.stderr)
         sys.exit(2)
+    if args.workload in ("uneven", "mixed"):
+        return bench_uneven(args, mixed=args.workload == "mixed")
+    if args.workload == "served":
+        return bench_served(args)
     forms = set(args.forms.split(","))
 
     from psdm_amd.workloads import line_workload, step_bytes, updates_per_shot
@@ -302,7 +573,7 @@ def main():
 
     acc = None
     if not args.no_accuracy:
-        log("[bench] accuracy check vs FP64 oracle")
+        log("[bench] accuracy check vs the FP64 oracle fixture")
         acc = accuracy_check(ctx)
 
     head = batched or plain

@@ -276,3 +276,27 @@ def test_aligned_decimated_imaging_equals_masked_reference_imaging():
                           precision="f64")
     assert np.linalg.norm(k_masked) > 0
     np.testing.assert_allclose(k_aligned, ss * k_masked, rtol=1e-13, atol=1e-13 * np.abs(k_masked).max())
+
+
+def test_fp64_accuracy_fixture_is_the_oracle():
+    """The FP64 image / traces fixture the accuracy figures are stated against is what the
+    FP64 oracle computes for its workload (regenerated here and compared)."""
+    from psdm_amd.solvers import ricker_mcs
+    from psdm_amd.workloads import line_workload
+
+    fx = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "accuracy_single401_fp64.npz"))
+    W = line_workload("single401")
+    m, sh = W["model"], W["shots"][0]
+    nt, dt, dPML = int(fx["nt"]), W["dt"], W["dPML"]
+    ctmp = np.asfortranarray(m.migration[:, sh.x_i - 1:sh.x_f])
+    ctrue = np.asfortranarray(m.true[:, sh.x_i - 1:sh.x_f])
+    src = ricker_mcs(nt, dt, W["f0"], 1.2 / W["f0"]).reshape(nt, 1)
+    geo = ([sh.nxSou - sh.x_i + 1], [sh.nySou], sh.nxRec - sh.x_i + 1, sh.nyRec)
+    d, _ = O.forward(*geo, src, m.dx, dt, ctrue, 1, nt, nt, dPML, 0)
+    assert float(np.sum(d.astype(np.float64) ** 2)) == float(fx["data_sumsq"])
+    R64, F64 = O.forward(*geo, src, m.dx, dt, ctmp, 1, nt, nt, dPML, 0, precision="f64")
+    k64 = O.backward(*geo, np.asfortranarray(d[::-1, :]), m.dx, dt, ctmp, 1, nt, nt, dPML, 0, Fields=F64,
+                     precision="f64")
+    np.testing.assert_array_equal(k64.astype(np.float32), fx["image_fp64"])
+    np.testing.assert_array_equal(R64.astype(np.float32), fx["traces_fp64"])
+    assert np.abs(fx["image_fp64"]).max() > 0
