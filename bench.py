@@ -163,9 +163,9 @@ def run_batched(ctx, torch, dev, W, d_mig, d_src, obs, steps, warmup, cfg):
     torch.cuda.synchronize()
     eng = Engine(ctx, d_mig, ny, m.nx, nslots=cfg["nslots"], max_nx=max(s.nx_w for s in shots), max_nt=nt,
                  dPML=W["dPML"], fs=W["fs"], dmodel=m.dx, dt=W["dt"], tile_x=cfg["tile_x"], grid=cfg["grid"],
-                 snap_format=cfg.get("snap_format", "f32"), store_snap=cfg.get("store_snap", 1),
-                 kahan=cfg.get("kahan", False), chunk_steps=cfg.get("chunk_steps", 64), d_psdm_sum=psdm_sum,
-                 d_cov_sum=fold)
+                 stream_w=cfg.get("stream_w", 0), snap_format=cfg.get("snap_format", "f32"),
+                 store_snap=cfg.get("store_snap", 1), kahan=cfg.get("kahan", False),
+                 chunk_steps=cfg.get("chunk_steps", 64), d_psdm_sum=psdm_sum, d_cov_sum=fold)
     for k in range(warmup, warmup + steps):  # the shots the plain form times
         eng.submit(**job_of(W, k % len(shots), d_src, obs))
     ctx.sync()
@@ -229,7 +229,8 @@ def run_engine_jobs(ctx, torch, dev, d_mig, ny, nx_prop, jobs, W, cfg, budget, i
 
     eng = Engine(ctx, d_mig, ny, nx_prop, nslots=cfg["nslots"], max_nx=max(j["nx_w"] for j in jobs),
                  max_nt=max(j["nt"] for j in jobs), dPML=W["dPML"], fs=W["fs"], dmodel=W["model"].dx, dt=W["dt"],
-                 tile_x=cfg["tile_x"], grid=cfg.get("grid", 0), snap_format=cfg.get("snap_format", "f32"),
+                 tile_x=cfg["tile_x"], stream_w=cfg.get("stream_w", 0), grid=cfg.get("grid", 0),
+                 snap_format=cfg.get("snap_format", "f32"),
                  store_snap=cfg.get("store_snap", 1), kahan=cfg.get("kahan", False),
                  chunk_steps=cfg.get("chunk_steps", 64), hbm_budget=budget)
     for q, j in enumerate(jobs):
@@ -533,15 +534,15 @@ def main():
         log(f"[bench] plain {plain['shots_per_s']} shots/s")
 
     if "batched" in forms:
-        cfg = dict(nslots=16, tile_x=32, grid=0, chunk_steps=64)
+        cfg = dict(nslots=16, tile_x=32, stream_w=4, grid=0, chunk_steps=64)
         tuned = None
         if not args.no_tune:
             from psdm_amd.autotune import tune
 
             jobs = [job_of(W, k, d_src, obs) for k in range(nshots)]
             best, trials = tune(ctx, d_mig, ny, m.nx, jobs, max(s.nx_w for s in shots), nt, dPML, W["fs"], m.dx, dt,
-                                slots=(8, 16, 24), tiles=(16, 32), grids=(0, 2048), log=log)
-            cfg.update(nslots=best["nslots"], tile_x=best["tile_x"], grid=best["grid"])
+                                log=log)
+            cfg.update(nslots=best["nslots"], tile_x=best["tile_x"], stream_w=best["stream_w"], grid=best["grid"])
             tuned = {"best": best, "trials": trials}
         B = run_batched(ctx, torch, dev, W, d_mig, d_src, obs, args.steps, args.warmup, cfg)
         st = B["stats"]

@@ -147,7 +147,8 @@ typedef struct {
     int snap_format;         /* 0 FP32, 1 FP16, 2 BF16 (power-of-two scale per snapshot tile) */
     int kahan;               /* compensated FP32 image accumulation */
     int chunk_steps;         /* time steps per captured graph */
-    int tile_x;              /* 8, 16 or 32 columns per tile */
+    int tile_x;              /* 8, 16 or 32 columns per CPML tile */
+    int stream_w;            /* interior columns per wavefront work item (0 = 4) */
     int grid;                /* persistent grid size in workgroups (0 = one per work item) */
     int use_graph;           /* replay captured graphs (1) or launch each step (0) */
     int max_inflight;        /* chunks queued ahead of the host (>= 1) */

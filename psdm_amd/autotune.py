@@ -1,7 +1,8 @@
 """Autotuner of the batched engine's launch shape.
 
 For a workload (model, shot windows, record length) it times short runs of the
-engine for each candidate (tile width, grid shape, resident slots) and keeps the one
+engine for each candidate (CPML tile width, interior stream-item width, grid shape,
+resident slots) and keeps the one
 with the lowest time per slot-step.  A trial admits `nslots` shots, lets one chunk run
 (admission and first-step effects), then times `chunks` more chunks; nothing is
 drained, so a trial costs a few hundred time steps.
@@ -35,19 +36,19 @@ def _trial(ctx, d_vel, ny, nx_prop, jobs, nslots, max_nx, nt, dPML, fs, dx, dt, 
 
 
 def tune(ctx, d_vel, ny, nx_prop, jobs, max_nx, nt, dPML, fs, dx, dt, snap_format="f32", store_snap=1, kahan=False,
-         slots=(8, 16, 24), tiles=(16, 32), grids=(0, 2048), chunk_steps=32, chunks=3, log=None):
+         slots=(16, 24), tiles=(16, 32), stream_ws=(4, 8), grids=(0,), chunk_steps=32, chunks=3, log=None):
     """jobs: list of Engine.submit keyword dicts.  Returns (best config dict, trials)."""
     trials = []
-    for nslots, tile_x, grid in itertools.product(slots, tiles, grids):
-        cfg = dict(tile_x=tile_x, grid=grid, snap_format=snap_format, store_snap=store_snap, kahan=kahan)
+    for nslots, tile_x, sw, grid in itertools.product(slots, tiles, stream_ws, grids):
+        cfg = dict(tile_x=tile_x, stream_w=sw, grid=grid, snap_format=snap_format, store_snap=store_snap, kahan=kahan)
         try:
             t = _trial(ctx, d_vel, ny, nx_prop, jobs, nslots, max_nx, nt, dPML, fs, dx, dt, cfg, chunk_steps, chunks)
         except Exception as exc:  # e.g. a slot count whose snapshot stores exceed HBM
             if log:
-                log(f"[autotune] slots={nslots} tile_x={tile_x} grid={grid}: skipped ({exc})")
+                log(f"[autotune] slots={nslots} tile_x={tile_x} stream_w={sw} grid={grid}: skipped ({exc})")
             continue
-        trials.append(dict(nslots=nslots, tile_x=tile_x, grid=grid, us_per_slot_step=1e6 * t))
+        trials.append(dict(nslots=nslots, tile_x=tile_x, stream_w=sw, grid=grid, us_per_slot_step=1e6 * t))
         if log:
-            log(f"[autotune] slots={nslots} tile_x={tile_x} grid={grid}: {1e6 * t:.3f} us per slot-step")
+            log(f"[autotune] slots={nslots} tile_x={tile_x} stream_w={sw} grid={grid}: {1e6 * t:.3f} us per slot-step")
     best = min(trials, key=lambda r: r["us_per_slot_step"])
     return best, trials

@@ -17,11 +17,15 @@ using psdm::StepArgs;
 
 namespace {
 
+// Interior columns per stream item of the one-shot solver: short items, so that one
+// shot's interior spreads over enough wavefronts to fill the chip.
+constexpr int kPlainStreamW = 4;
+
 // Geometry of one call, 0-based padded-grid quantities.
 struct Layout {
-    int ny, nx, nt, dn, nyt, nxt, ld, lds, fs, ram, ss, nt_snap;
+    int ny, nx, nt, dn, nyt, nxt, ld, lds, off, fs, ram, ss, nt_snap;
     float dx, dt;
-    int tile_x, ntile_y, ntiles, per_xcd;
+    int tile_x, per_xcd;
     psdm::Tiling T;
 };
 
@@ -134,8 +138,10 @@ int check_geom(psdm_rtm_ctx *c, const psdm_rtm_geom *g) {
                     "the _ram padding loops (solver_ac_forward_ram.f90:288-302) read ctmp(1,j) for j up to "
                     "dPML+6: nx must be >= dPML+6");
     if (g->NumRec < 0 || g->NumSou < 0) return fail(c, PSDM_RTM_EINVAL, "negative point count");
-    const long long nyt = g->ny + 2LL * (g->dPML + 6), nxt = g->nx + 2LL * (g->dPML + 6);
-    if ((nyt + 63) / 64 * 64 * nxt >= (1LL << 31)) return fail(c, PSDM_RTM_EINVAL, "padded grid exceeds 2^31 cells");
+    if (g->ny >= 65536 || g->nx >= 32768) return fail(c, PSDM_RTM_EINVAL, "window exceeds 65535 x 32767 cells");
+    const long long nxt = g->nx + 2LL * (g->dPML + 6);
+    if ((long long)psdm::state_ld(g->ny, g->dPML + 6) * nxt >= (1LL << 31))
+        return fail(c, PSDM_RTM_EINVAL, "padded grid exceeds 2^31 cells");
     return 0;
 }
 
@@ -147,8 +153,9 @@ Layout make_layout(const psdm_rtm_geom *g, int tile_x) {
     L.dn = g->dPML + 6;
     L.nyt = g->ny + 2 * L.dn;
     L.nxt = g->nx + 2 * L.dn;
-    L.ld = (L.nyt + 63) / 64 * 64;
+    L.ld = psdm::state_ld(L.ny, L.dn);
     L.lds = psdm::interior_ld(L.ny);
+    L.off = psdm::state_off(L.dn);
     L.fs = g->fs != 0;
     L.ram = g->variant == 0;
     L.ss = g->store_snap;
@@ -156,10 +163,8 @@ Layout make_layout(const psdm_rtm_geom *g, int tile_x) {
     L.dx = g->dmodel;
     L.dt = g->dt;
     L.tile_x = tile_x;
-    L.T = psdm::make_tiling(L.ny, L.nx, L.dn, tile_x);
-    L.ntile_y = L.T.nrt;
-    L.ntiles = L.T.ntiles;
-    L.per_xcd = (L.ntiles + 7) / 8;
+    L.T = psdm::make_tiling(L.ny, L.nx, L.dn, tile_x, kPlainStreamW);
+    L.per_xcd = (L.T.nitems + 7) / 8;
     return L;
 }
 
@@ -176,15 +181,15 @@ int prepare(psdm_rtm_ctx *c, const Layout &L, const float *d_ctmp, const float *
     StepArgs &a = P.a;
     std::memset(&a, 0, sizeof(a));
     const size_t ld = L.ld, nxt = L.nxt, dn2 = 2 * L.dn;
-    const size_t n_p = ld * nxt, n_x = ld * dn2, n_y = dn2 * nxt;
+    const size_t n_p = (ld * nxt + 32 + 63) / 64 * 64, n_x = ld * dn2, n_y = dn2 * nxt;  // n_p: + row-offset room
     void *v;
     // one allocation for the state that must start at zero
     const size_t n_state = 2 * n_p + 3 * n_x + 3 * n_y + 2 * dn2;
     if (int rc = get_buf(c, "state", n_state * sizeof(float), &v)) return rc;
     float *s = (float *)v;
     HIP_TRY(c, hipMemsetAsync(s, 0, n_state * sizeof(float), c->stream));
-    P.p0 = s;
-    P.p1 = s + n_p;
+    P.p0 = s + L.off;
+    P.p1 = s + n_p + L.off;
     P.phx0 = s + 2 * n_p;
     P.phx1 = P.phx0 + n_x;
     a.chx = P.phx1 + n_x;
@@ -194,8 +199,8 @@ int prepare(psdm_rtm_ctx *c, const Layout &L, const float *d_ctmp, const float *
     P.hs0 = a.chy + n_y;
     P.hs1 = P.hs0 + dn2;
 
-    if (int rc = get_buf(c, "c2", (size_t)L.lds * L.nx * sizeof(float), &v)) return rc;
-    a.c2 = (float *)v;
+    if (int rc = get_buf(c, "c2", ((size_t)L.lds * L.nx + 64) * sizeof(float), &v)) return rc;
+    a.c2 = (float *)v + psdm::kIntPad;
     a.ldc = L.lds;
     psdm::launch_c2(d_ctmp, (float *)a.c2, L.ny, L.nx, L.lds, L.dt, L.dx, c->stream);
 
@@ -208,27 +213,23 @@ int prepare(psdm_rtm_ctx *c, const Layout &L, const float *d_ctmp, const float *
     a.fA = (float *)v;
     a.fB = (float *)v + L.dn;
 
-    // injection events
-    std::map<int, int> last;  // cell (j + i*nyt) -> trace column, last writer wins
+    // injection events, per stream item, ordered by column then row
+    std::map<int, int> last;  // (iw << 16 | jw) -> trace column, last writer wins
     for (int s2 = 0; s2 < npts; s2++) {
         const int jw = py[s2] - 1, iw = px[s2] - 1;
         if (jw < 0 || jw >= L.ny || iw < 0 || iw >= L.nx) continue;  // only interior cells get source
-        last[(L.dn + jw) + (L.dn + iw) * L.nyt] = s2;
+        last[(iw << 16) | jw] = s2;
     }
-    std::vector<std::vector<std::pair<int, int>>> per_tile(L.ntiles);
-    for (auto &kv : last) {
-        const int j = kv.first % L.nyt, i = kv.first / L.nyt;
-        const int t = psdm::tile_of_cell(L.T, j, i, L.ny, L.nx, L.dn, L.tile_x);
-        per_tile[t].push_back(kv);
-    }
-    std::vector<int> ev_ptr(L.ntiles + 1, 0), ev_cell, ev_inj, cols;
-    for (int t = 0; t < L.ntiles; t++) {
-        for (auto &kv : per_tile[t]) {
+    std::vector<std::vector<std::pair<int, int>>> per_item(L.T.nstream);
+    for (auto &kv : last) per_item[psdm::stream_item_of_cell(L.T, kv.first & 0xffff, kv.first >> 16)].push_back(kv);
+    std::vector<int> ev_ptr(L.T.nstream + 1, 0), ev_cell, ev_inj, cols;
+    for (int u = 0; u < L.T.nstream; u++) {
+        for (auto &kv : per_item[u]) {
             ev_cell.push_back(kv.first);
             ev_inj.push_back((int)cols.size());
             cols.push_back(kv.second);
         }
-        ev_ptr[t + 1] = (int)ev_cell.size();
+        ev_ptr[u + 1] = (int)ev_cell.size();
     }
     const int ninj = (int)cols.size();
     if (ninj > 0 && ntr <= 0) return fail(c, PSDM_RTM_EINVAL, "injection points without traces");
@@ -268,19 +269,15 @@ int prepare(psdm_rtm_ctx *c, const Layout &L, const float *d_ctmp, const float *
     a.nyt = L.nyt;
     a.nxt = L.nxt;
     a.ld = L.ld;
-    a.ntile_y = L.ntile_y;
-    a.ntiles = L.ntiles;
+    psdm::set_tiling(a, L.T);
     a.per_xcd = L.per_xcd;
-    a.rt_pml = L.T.rt_pml;
-    a.rt_int = L.T.rt_int;
-    a.ct_pml = L.T.ct_pml;
-    a.ct_int = L.T.ct_int;
     a.fs = L.fs;
     a.ram = L.ram;
     a.dt = L.dt;
     a.ss_f = (float)L.ss;
     a.lds = L.lds;
     a.snap_stride = (size_t)L.lds * L.nx;
+    a.scale_stride = L.T.ri * L.nx;
     a.rec_row = -1;
     a.snap_slot = -1;
     return 0;
@@ -381,10 +378,10 @@ int fwd_device(psdm_rtm_ctx *c, const psdm_rtm_geom *g, const float *d_ctmp, con
     st.ny = L.ny;
     st.nx = L.nx;
     st.nslots = nslots;
-    st.bytes = (size_t)nslots * L.lds * L.nx * sizeof(float);
+    st.bytes = nslots > 0 ? ((size_t)nslots * L.lds * L.nx + 64) * sizeof(float) : 0;
     if (int rc = alloc_store(c, st.bytes, &st.d)) return rc;
     c->snaps[key] = st;
-    P.a.snaps = st.d;
+    P.a.snaps = st.d ? st.d + psdm::kIntPad : nullptr;
     return run_loop(c, L, P, 0, nslots, d_rec);
 }
 
@@ -399,22 +396,18 @@ int bwd_device(psdm_rtm_ctx *c, const psdm_rtm_geom *g, const float *d_ctmp, con
     Prepared P;
     if (int rc = prepare(c, L, d_ctmp, d_Sadj, g->NumRec, g->nxRec, g->nyRec, g->NumRec, nullptr, nullptr, 0, P))
         return rc;
-    P.a.snaps = it->second.d;
-    // the kernel accumulates into an image with leading dimension lds, seeded from the
-    // caller's kernel(ny,nx) and copied back afterwards
-    float *img = d_kernel;
-    if (L.lds != L.ny) {
-        void *v;
-        if (int rc = get_buf(c, "img", (size_t)L.lds * L.nx * sizeof(float), &v)) return rc;
-        img = (float *)v;
-        HIP_TRY(c, hipMemcpy2DAsync(img, (size_t)L.lds * sizeof(float), d_kernel, (size_t)L.ny * sizeof(float),
-                                    (size_t)L.ny * sizeof(float), L.nx, hipMemcpyDeviceToDevice, c->stream));
-    }
+    P.a.snaps = it->second.d ? it->second.d + psdm::kIntPad : nullptr;
+    // the kernel accumulates into an image with leading dimension lds (kIntPad front
+    // rows), seeded from the caller's kernel(ny,nx) and copied back afterwards
+    void *v;
+    if (int rc = get_buf(c, "img", ((size_t)L.lds * L.nx + 64) * sizeof(float), &v)) return rc;
+    float *img = (float *)v + psdm::kIntPad;
+    HIP_TRY(c, hipMemcpy2DAsync(img, (size_t)L.lds * sizeof(float), d_kernel, (size_t)L.ny * sizeof(float),
+                                (size_t)L.ny * sizeof(float), L.nx, hipMemcpyDeviceToDevice, c->stream));
     P.a.image = img;
     if (int rc = run_loop(c, L, P, 1, it->second.nslots, nullptr)) return rc;
-    if (img != d_kernel)
-        HIP_TRY(c, hipMemcpy2DAsync(d_kernel, (size_t)L.ny * sizeof(float), img, (size_t)L.lds * sizeof(float),
-                                    (size_t)L.ny * sizeof(float), L.nx, hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(c, hipMemcpy2DAsync(d_kernel, (size_t)L.ny * sizeof(float), img, (size_t)L.lds * sizeof(float),
+                                (size_t)L.ny * sizeof(float), L.nx, hipMemcpyDeviceToDevice, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     return free_store(c, key);
 }
@@ -466,7 +459,7 @@ int host_forward(psdm_rtm_ctx *c, const psdm_rtm_geom &g, const float *S, const 
         if (st.nslots > 0) {
             if (!Fields) return fail(c, PSDM_RTM_EINVAL, "null Fields with host_fields");
             // Fields(ny, nx, nt_snap) <- snapshots with leading dimension lds
-            HIP_TRY(c, hipMemcpy2DAsync(Fields, (size_t)g.ny * sizeof(float), st.d,
+            HIP_TRY(c, hipMemcpy2DAsync(Fields, (size_t)g.ny * sizeof(float), st.d + psdm::kIntPad,
                                         (size_t)psdm::interior_ld(g.ny) * sizeof(float), (size_t)g.ny * sizeof(float),
                                         (size_t)st.nslots * g.nx, hipMemcpyDeviceToHost, c->stream));
         }
@@ -502,10 +495,10 @@ int host_backward(psdm_rtm_ctx *c, const psdm_rtm_geom &g, const float *Sadj, co
         if (st.nslots > 0) {
             if (!Fields) return fail(c, PSDM_RTM_EINVAL, "null Fields with host_fields");
             const size_t lds = psdm::interior_ld(g.ny);
-            st.bytes = (size_t)st.nslots * lds * g.nx * sizeof(float);
+            st.bytes = ((size_t)st.nslots * lds * g.nx + 64) * sizeof(float);
             if (int rc = alloc_store(c, st.bytes, &st.d)) return rc;
             c->snaps[key] = st;
-            HIP_TRY(c, hipMemcpy2DAsync(st.d, lds * sizeof(float), Fields, (size_t)g.ny * sizeof(float),
+            HIP_TRY(c, hipMemcpy2DAsync(st.d + psdm::kIntPad, lds * sizeof(float), Fields, (size_t)g.ny * sizeof(float),
                                         (size_t)g.ny * sizeof(float), (size_t)st.nslots * g.nx, hipMemcpyHostToDevice,
                                         c->stream));
         } else {

@@ -5,6 +5,8 @@
 #include <cstddef>
 #include <cstdint>
 
+#include "fastdiv.hpp"
+
 namespace psdm {
 
 // Central-difference weights of accuracy 3 (radius 3), finite_difference_coef
@@ -13,38 +15,72 @@ namespace psdm {
 constexpr float kC1_1 = 3.f / 4.f, kC1_2 = -3.f / 20.f, kC1_3 = 1.f / 60.f;
 constexpr float kC2_0 = (-49.f / 18.f) / 2.f, kC2_1 = 3.f / 2.f, kC2_2 = -3.f / 20.f, kC2_3 = 1.f / 90.f;
 
-constexpr int kTileY = 64;        // rows per tile: one wavefront, one row per lane
+constexpr int kTileY = 64;        // CPML tiles: rows per tile, one row per lane
 constexpr int kBlock = 256;       // 4 wavefronts per workgroup
 constexpr int kMaxTables = 1024;  // max CPML cells per side (dPML + 6)
+constexpr int kStreamRows = 248;  // interior stream items: rows owned per wavefront
+constexpr int kIntPad = 4;        // front rows of padding of the interior arrays
 
 // Snapshot storage formats.  FP16/BF16 snapshots carry one power-of-two scale per
-// (snapshot, tile): the tile's max-abs value is mapped to 2^14 (FP16) or 1 (BF16), so
-// the only error is the 16-bit rounding of the scaled value.
+// (snapshot, interior column, 248-row stream tile): the max-abs value there is mapped
+// to 2^14 (FP16) or 1 (BF16), so the only error is the 16-bit rounding of the scaled
+// value.
 enum SnapFmt { kSnapF32 = 0, kSnapF16 = 1, kSnapBF16 = 2 };
 
 // Everything one time step of one shot needs.  Layout in HBM:
-//   wavefields  p[i*ld + j], i in [0,nxt), j in [0,nyt), ld = nyt rounded up to 64
+//   wavefields  p[i*ld + j], i in [0,nxt), j in [0,nyt): the pointer is offset into its
+//               buffer (state_off) so that row dn-4 of every column is 128-byte aligned
 //   x-CPML mem  x[c*ld + j], c in [0,2dn): c < dn left strip column c, else right
 //               strip column c-dn (grid column nx+c); all nyt rows (strips+corners)
 //   y-CPML mem  y[i*2dn + r], r in [0,2dn): r < dn top row r, else bottom row ny+r;
 //               all nxt columns (strips+corners)
 //   c2          the unpadded window c2w[(i-dn)*ldc + (j-dn)], read with clamped
-//               indices: the reference replicates edge velocities into the pads
-//   snapshots   F[s][i][j] over the interior (ny x nx), s = snapshot slot
-// Region-aligned tiling: tile rows cover the top CPML [0,dn), the interior [dn,dn+ny)
-// and the bottom CPML separately (64 rows each, the last one of a region partial);
-// tile columns likewise with TX columns.  Every tile is then wholly interior, x strip,
-// y strip or corner.  Tiles are numbered column-major: t = C * nrt + R.
+//               indices by the CPML cells (the reference replicates edge velocities
+//               into the pads); the pointer is kIntPad rows into its buffer
+//   snapshots   F[s][(i-dn)*lds + (j-dn)] over the interior (ny x nx), s = snapshot slot;
+//   image       same layout (kIntPad front rows in the buffer, like c2)
+// Work of one step:
+//   CPML tiles  64 rows x TX columns of the region-aligned grid: tile rows cover the
+//               top CPML [0,dn), the interior [dn,dn+ny) and the bottom CPML apart,
+//               columns likewise, so each tile is x strip, y strip or corner.  Only
+//               the non-interior tiles of that grid are used: first the top/bottom
+//               strips (2*rt_pml row tiles x nct column tiles, c = C*2rt_pml + r), then
+//               the x strips over the interior rows (2*ct_pml x rt_int, c = cc*rt_int + r).
+//   stream items  the interior in wavefront-sized pieces: kStreamRows rows (4 per lane,
+//               one 4-row halo lane at each end) x sw columns, walked column by column
+//               with a 7-column register window (ri row tiles x cs column segments,
+//               u = CS*ri + R); four items per workgroup.
 struct Tiling {
-    int rt_pml, rt_int, ct_pml, ct_int, nrt, nct, ntiles;
+    int rt_pml, rt_int, ct_pml, ct_int, nrt, nct;  // the region-aligned 64 x TX grid
+    int ncpml_a, ncpml;                            // CPML tiles: top/bottom strips, all
+    int ri, cs, sw, nstream, nsblk;                // stream items and their workgroups
+    int nitems;                                    // workgroups of one step: ncpml + nsblk
 };
 
 __host__ __device__ inline int cdiv(int a, int b) { return (a + b - 1) / b; }
-// Leading dimension of the interior arrays (snapshots, image, c2): whole 64-float
-// column segments, so a wavefront's 64 rows are one aligned 256-byte access.
-__host__ __device__ inline int interior_ld(int ny) { return cdiv(ny, 64) * 64; }
+// Leading dimension of the interior arrays (snapshots, image, c2): a multiple of 64
+// holding the kIntPad front rows and every row a stream item reads (its halo lanes).
+__host__ __device__ inline int interior_ld(int ny) { return cdiv((ny > kStreamRows ? ny : kStreamRows) + 8, 64) * 64; }
+// Row offset of the wavefield pointer into its buffer: row dn-4 lands on a 32-float
+// (128-byte) boundary, so every stream item's 256-row column reads are 16-byte aligned.
+__host__ __device__ inline int state_off(int dn) { return ((4 - dn) % 32 + 32) % 32; }
+// Leading dimension of the wavefield arrays: holds state_off + the nyt grid rows and
+// every row a stream item reads.
+__host__ __device__ inline int state_ld(int ny, int dn) {
+    const int rows = ny + 2 * dn > dn + (ny > kStreamRows ? ny : kStreamRows) + 8
+                         ? ny + 2 * dn
+                         : dn + (ny > kStreamRows ? ny : kStreamRows) + 8;
+    return cdiv(state_off(dn) + rows, 64) * 64;
+}
+// First interior row (j - dn) a stream item of row tile R reads at lane 1: 248 R, except
+// that the last tile is moved up (to a multiple of 4) so it ends at or below ny + 3.
+__host__ __device__ inline int stream_origin(int R, int ny) {
+    if (ny <= kStreamRows) return 0;
+    const int last = cdiv(ny - kStreamRows, 4) * 4;
+    return R * kStreamRows < last ? R * kStreamRows : last;
+}
 
-__host__ __device__ inline Tiling make_tiling(int ny, int nx, int dn, int tx) {
+__host__ __device__ inline Tiling make_tiling(int ny, int nx, int dn, int tx, int sw) {
     Tiling T;
     T.rt_pml = cdiv(dn, kTileY);
     T.rt_int = cdiv(ny, kTileY);
@@ -52,7 +88,14 @@ __host__ __device__ inline Tiling make_tiling(int ny, int nx, int dn, int tx) {
     T.ct_int = cdiv(nx, tx);
     T.nrt = 2 * T.rt_pml + T.rt_int;
     T.nct = 2 * T.ct_pml + T.ct_int;
-    T.ntiles = T.nrt * T.nct;
+    T.ncpml_a = 2 * T.rt_pml * T.nct;
+    T.ncpml = T.ncpml_a + 2 * T.ct_pml * T.rt_int;
+    T.ri = cdiv(ny, kStreamRows);
+    T.sw = sw;
+    T.cs = cdiv(nx, sw);
+    T.nstream = T.ri * T.cs;
+    T.nsblk = cdiv(T.nstream, 4);
+    T.nitems = T.ncpml + T.nsblk;
     return T;
 }
 
@@ -60,11 +103,12 @@ __host__ __device__ inline Tiling make_tiling(int ny, int nx, int dn, int tx) {
 // along one axis with `npml` CPML tiles on each side sharing its `n_pml` cells evenly
 // (so every CPML tile spans at least 4 cells when n_pml >= 7 and w >= 8: the clamped
 // edge stencils then stay inside the tile's staged halo), and `nint` interior tiles of
-// `w` cells over `n_int` cells.
-__host__ __device__ inline int axis_tile(int r, int npml, int nint, int n_pml, int n_int, int w, int *lo, int *hi) {
+// `w` cells over `n_int` cells.  `fpml` divides by npml.
+__host__ __device__ inline int axis_tile(int r, int npml, int nint, int n_pml, int n_int, int w, const FastDiv &fpml,
+                                         int *lo, int *hi) {
     if (r < npml) {
-        *lo = r * n_pml / npml;
-        *hi = (r + 1) * n_pml / npml;
+        *lo = fdiv(r * n_pml, fpml);
+        *hi = fdiv((r + 1) * n_pml, fpml);
         return 1;
     }
     r -= npml;
@@ -74,32 +118,22 @@ __host__ __device__ inline int axis_tile(int r, int npml, int nint, int n_pml, i
         return 0;
     }
     r -= nint;
-    *lo = n_pml + n_int + r * n_pml / npml;
-    *hi = n_pml + n_int + (r + 1) * n_pml / npml;
+    *lo = n_pml + n_int + fdiv(r * n_pml, fpml);
+    *hi = n_pml + n_int + fdiv((r + 1) * n_pml, fpml);
     return 2;
 }
 
-// Tile index along one axis of cell c (inverse of axis_tile).
-__host__ __device__ inline int axis_tile_of(int c, int npml, int n_pml, int n_int, int w) {
-    if (c >= n_pml && c < n_pml + n_int) return npml + (c - n_pml) / w;
-    const int side = c < n_pml ? 0 : 1;
-    const int cl = side ? c - n_pml - n_int : c;
-    int r = 0;
-    while ((r + 1) * n_pml / npml <= cl) ++r;
-    return side ? npml + cdiv(n_int, w) + r : r;
-}
-
-// Tile index of padded cell (j, i).
-__host__ __device__ inline int tile_of_cell(const Tiling &T, int j, int i, int ny, int nx, int dn, int tx) {
-    const int R = axis_tile_of(j, T.rt_pml, dn, ny, kTileY);
-    const int C = axis_tile_of(i, T.ct_pml, dn, nx, tx);
-    return C * T.nrt + R;
+// Stream item that owns interior cell (jw, iw) (0-based inside the window).
+__host__ __device__ inline int stream_item_of_cell(const Tiling &T, int jw, int iw) {
+    return (iw / T.sw) * T.ri + jw / kStreamRows;
 }
 
 struct StepArgs {
     int ny, nx, dn, nyt, nxt, ld;
-    int ntile_y, ntiles, per_xcd;  // ntile_y = Tiling::nrt
-    int rt_pml, rt_int, ct_pml, ct_int;
+    int nitems, per_xcd;  // workgroups of one step (Tiling::nitems); per XCD
+    int rt_pml, rt_int, ct_pml, ct_int, nct;
+    int ncpml_a, ncpml, ri, cs, sw, nstream;
+    FastDiv fd_2rp, fd_rint, fd_ri, fd_rpml, fd_cpml;  // division by 2*rt_pml, rt_int, ri, rt_pml, ct_pml
     int fs, ram;      // free surface flag; ram = 1 -> _ram semantics, 0 -> _disk
     int k;            // time step index of the reference loop, 2..nt
     int mode;         // 0 forward (snapshots), 1 backward (imaging)
@@ -119,16 +153,17 @@ struct StepArgs {
     const float *c2;
     int ldc;
     const float *fA, *fB;  // 1D damping tables (dn), index = distance into the CPML - 1
-    const int *ev_ptr;     // per tile [ev_ptr[t], ev_ptr[t+1]) injection events
+    const int *ev_ptr;     // per stream item [ev_ptr[u], ev_ptr[u+1]) injection events, by column
     const int *ev_cell;    // event cell j + i*nyt
     const int *ev_inj;     // column of inj_val
     const float *inj_val;  // (nt-1) x ninj, row k-2 holds the values injected at step k
     int ninj;
     void *snaps;           // forward: snapshot base (format snap_fmt); backward: read
-    float *snap_scale;     // FP16/BF16: scale[slot * ntiles + tile]
+    float *snap_scale;     // FP16/BF16: scale[slot * scale_stride + R * nx + (i - dn)]
+    int scale_stride;      // ri * nx
     float *image;          // backward: kernel(ny,nx), leading dimension lds
     float *image_comp;     // Kahan compensation (kahan != 0)
-    int lds;               // leading dimension of snapshots, image and c2: ny rounded up to 64
+    int lds;               // leading dimension of snapshots and image (interior_ld)
     size_t snap_stride;    // elements per snapshot (lds*nx)
     const int *rec_cell;   // receivers: cell j + i*ld, or -1 outside the grid
     float *rec;            // Rec_point(nt, NumRec) column-major, leading dimension rec_ld = nt
@@ -136,6 +171,27 @@ struct StepArgs {
     int nrec;
     int rec_row;           // row of rec that p_c (step k-1) fills, -1 = none
 };
+
+// Copies a tiling into the step arguments (host side).
+inline void set_tiling(StepArgs &a, const Tiling &T) {
+    a.nitems = T.nitems;
+    a.rt_pml = T.rt_pml;
+    a.rt_int = T.rt_int;
+    a.ct_pml = T.ct_pml;
+    a.ct_int = T.ct_int;
+    a.nct = T.nct;
+    a.ncpml_a = T.ncpml_a;
+    a.ncpml = T.ncpml;
+    a.ri = T.ri;
+    a.cs = T.cs;
+    a.sw = T.sw;
+    a.nstream = T.nstream;
+    a.fd_2rp = make_fastdiv(2 * T.rt_pml);
+    a.fd_rint = make_fastdiv(T.rt_int);
+    a.fd_ri = make_fastdiv(T.ri);
+    a.fd_rpml = make_fastdiv(T.rt_pml);
+    a.fd_cpml = make_fastdiv(T.ct_pml);
+}
 
 // One slot of the batched engine: a shot resident on the device for its forward and
 // backward phases.  `a` holds the static fields; the per-step pointers and flags are
@@ -145,6 +201,7 @@ struct SlotDesc {
     float *pA, *pB, *phxA, *phxB, *phyA, *phyB, *hsA, *hsB;
     long long g0;  // global step of the slot's first forward step (k = 2); < 0: slot idle
     int nt, ss, nsnap;
+    FastDiv fd_ss;  // division by ss
     const int *ev_ptr_f, *ev_cell_f, *ev_inj_f;
     const float *inj_f;
     int ninj_f;
@@ -156,7 +213,8 @@ struct SlotDesc {
 struct BatchArgs {
     const SlotDesc *slots;
     int nslots;
-    int cap_tiles;           // work items per slot (tiles of the slot capacity)
+    int cap_tiles;           // work items (workgroups) per slot: nitems of the slot capacity
+    FastDiv fd_cap;          // division by cap_tiles
     int nwork;               // nslots * cap_tiles
     int per_xcd;             // work items per XCD
     const long long *gbase;  // global step of the chunk's first step

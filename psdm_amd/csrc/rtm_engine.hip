@@ -105,7 +105,8 @@ double now_s() {
 struct psdm_rtm_engine {
     psdm_rtm_ctx *c = nullptr;
     psdm_rtm_engine_cfg cfg{};
-    int dn = 0, ld = 0, lds = 0, nyt = 0, nxt_max = 0, ntile_y = 0, cap_tiles = 0, nwork = 0, per_xcd = 0, grid = 0;
+    int dn = 0, ld = 0, lds = 0, off = 0, sw = 16, nyt = 0, nxt_max = 0, cap_tiles = 0, nwork = 0, per_xcd = 0, grid = 0;
+    size_t win = 0;  // per-slot length of the c2 / image / comp buffers
     size_t n_p = 0, n_x = 0, n_y = 0, state_stride = 0, esize = 4;
     // device
     SlotDesc *d_slots = nullptr;
@@ -150,7 +151,7 @@ int issue_chunk(psdm_rtm_engine *e) {
         HIP_TRY(c, hipGraphLaunch(e->exec, c->stream));
     } else {
         for (int g = 0; g < C; g++) {
-            BatchArgs b{e->d_slots, e->cfg.nslots, e->cap_tiles, e->nwork, e->per_xcd, e->d_gbase, g};
+            BatchArgs b{e->d_slots, e->cfg.nslots, e->cap_tiles, make_fastdiv(e->cap_tiles), e->nwork, e->per_xcd, e->d_gbase, g};
             launch_batch(b, e->cfg.tile_x, e->grid, c->stream);
         }
         launch_advance(e->d_gbase, C, c->stream);
@@ -163,7 +164,7 @@ int capture_graph(psdm_rtm_engine *e) {
     psdm_rtm_ctx *c = e->c;
     HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
     for (int g = 0; g < e->cfg.chunk_steps; g++) {
-        BatchArgs b{e->d_slots, e->cfg.nslots, e->cap_tiles, e->nwork, e->per_xcd, e->d_gbase, g};
+        BatchArgs b{e->d_slots, e->cfg.nslots, e->cap_tiles, make_fastdiv(e->cap_tiles), e->nwork, e->per_xcd, e->d_gbase, g};
         launch_batch(b, e->cfg.tile_x, e->grid, c->stream);
     }
     launch_advance(e->d_gbase, e->cfg.chunk_steps, c->stream);
@@ -172,12 +173,15 @@ int capture_graph(psdm_rtm_engine *e) {
     return 0;
 }
 
+// Snapshot store of one shot: nsnap snapshots of lds x nx_w (plus the front padding and
+// a read margin), then for 16-bit formats one scale per (snapshot, stream row tile, column).
+size_t snap_data_bytes(const psdm_rtm_engine *e, int nx_w, int nsnap) {
+    return (((size_t)nsnap * e->lds * nx_w + 64) * e->esize + 255) & ~(size_t)255;
+}
 size_t snap_bytes_of(const psdm_rtm_engine *e, int nx_w, int nt) {
     const int nsnap = (nt - 1) / e->cfg.store_snap;
-    const int ntiles = make_tiling(e->cfg.ny, nx_w, e->dn, e->cfg.tile_x).ntiles;
-    size_t b = (size_t)nsnap * e->lds * nx_w * e->esize;
-    b = (b + 255) & ~(size_t)255;
-    if (e->cfg.snap_format != kSnapF32) b += (size_t)nsnap * ntiles * sizeof(float);
+    size_t b = snap_data_bytes(e, nx_w, nsnap);
+    if (e->cfg.snap_format != kSnapF32) b += (size_t)nsnap * cdiv(e->cfg.ny, kStreamRows) * nx_w * sizeof(float);
     return b;
 }
 
@@ -187,32 +191,28 @@ int admit(psdm_rtm_engine *e, int s, Job &job, size_t arena_off) {
     const psdm_rtm_job &J = job.j;
     const int ny = e->cfg.ny, nx = J.nx_w, nt = J.nt, dn = e->dn, TX = e->cfg.tile_x;
     const int nxt = nx + 2 * dn;
-    const Tiling T = make_tiling(ny, nx, dn, TX);
-    const int ntiles = T.ntiles;
+    const Tiling T = make_tiling(ny, nx, dn, TX, e->sw);
     Slot &sl = e->slots[s];
 
     // injection events: forward source (1 point), backward receivers (last writer wins)
     auto build = [&](const int *px, const int *py, int n, std::vector<int> &ptr, std::vector<int> &cell,
                      std::vector<int> &inj, std::vector<int> &cols) {
-        std::map<int, int> last;
+        std::map<int, int> last;  // (iw << 16 | jw) -> trace column; ordered by column, then row
         for (int q = 0; q < n; q++) {
             const int jw = py[q] - 1, iw = px[q] - 1;
             if (jw < 0 || jw >= ny || iw < 0 || iw >= nx) continue;
-            last[(dn + jw) + (dn + iw) * e->nyt] = q;
+            last[(iw << 16) | jw] = q;
         }
-        std::vector<std::vector<std::pair<int, int>>> per(ntiles);
-        for (auto &kv : last) {
-            const int j = kv.first % e->nyt, i = kv.first / e->nyt;
-            per[tile_of_cell(T, j, i, ny, nx, dn, TX)].push_back(kv);
-        }
-        ptr.assign(ntiles + 1, 0);
-        for (int t = 0; t < ntiles; t++) {
-            for (auto &kv : per[t]) {
+        std::vector<std::vector<std::pair<int, int>>> per(T.nstream);
+        for (auto &kv : last) per[stream_item_of_cell(T, kv.first & 0xffff, kv.first >> 16)].push_back(kv);
+        ptr.assign(T.nstream + 1, 0);
+        for (int u = 0; u < T.nstream; u++) {
+            for (auto &kv : per[u]) {
                 cell.push_back(kv.first);
                 inj.push_back((int)cols.size());
                 cols.push_back(kv.second);
             }
-            ptr[t + 1] = (int)cell.size();
+            ptr[u + 1] = (int)cell.size();
         }
     };
     const int sx = J.nxSou, sy = J.nySou;
@@ -245,7 +245,7 @@ int admit(psdm_rtm_engine *e, int s, Job &job, size_t arena_off) {
     float *inj_f = sl.d_inj, *inj_b = sl.d_inj + (size_t)(nt - 1) * ninj_f;
     launch_gather(J.d_wavelet, nt, sl.d_ints + ofcol, ninj_f, inj_f, 0, c->stream);
     launch_gather(J.d_data, nt, sl.d_ints + obcol, ninj_b, inj_b, 1, c->stream);
-    float *c2 = e->d_c2 + (size_t)s * e->lds * e->cfg.max_nx;
+    float *c2 = e->d_c2 + (size_t)s * e->win + kIntPad;
     launch_c2(e->cfg.d_vel + (size_t)(J.x_i - 1) * ny, c2, ny, nx, e->lds, e->cfg.dt, e->cfg.dmodel, c->stream);
     // the host vectors above are read by the queued copy: wait before they go away
     HIP_TRY(c, hipStreamSynchronize(c->stream));
@@ -259,13 +259,8 @@ int admit(psdm_rtm_engine *e, int s, Job &job, size_t arena_off) {
     a.nyt = e->nyt;
     a.nxt = nxt;
     a.ld = e->ld;
-    a.ntile_y = T.nrt;
-    a.ntiles = ntiles;
+    set_tiling(a, T);
     a.per_xcd = 0;
-    a.rt_pml = T.rt_pml;
-    a.rt_int = T.rt_int;
-    a.ct_pml = T.ct_pml;
-    a.ct_int = T.ct_int;
     a.fs = e->cfg.fs != 0;
     a.ram = e->cfg.variant == 0;
     a.snap_fmt = e->cfg.snap_format;
@@ -277,17 +272,18 @@ int admit(psdm_rtm_engine *e, int s, Job &job, size_t arena_off) {
     a.lds = e->lds;
     a.fA = e->d_tabs;
     a.fB = e->d_tabs + dn;
-    a.snaps = e->d_arena + arena_off;
+    a.snaps = e->d_arena + arena_off + kIntPad * e->esize;
     const int nsnap = (nt - 1) / e->cfg.store_snap;
-    const size_t data_b = ((size_t)nsnap * e->lds * nx * e->esize + 255) & ~(size_t)255;
-    a.snap_scale = e->cfg.snap_format != kSnapF32 ? (float *)(e->d_arena + arena_off + data_b) : nullptr;
-    a.image = e->d_img + (size_t)s * e->lds * e->cfg.max_nx;
-    a.image_comp = e->d_comp + (size_t)s * e->lds * e->cfg.max_nx;
+    a.snap_scale = e->cfg.snap_format != kSnapF32 ? (float *)(e->d_arena + arena_off + snap_data_bytes(e, nx, nsnap))
+                                                  : nullptr;
+    a.scale_stride = T.ri * nx;
+    a.image = e->d_img + (size_t)s * e->win + kIntPad;
+    a.image_comp = e->d_comp + (size_t)s * e->win + kIntPad;
     a.snap_stride = (size_t)e->lds * nx;
     a.rec_row = -1;
     float *st = e->d_state + (size_t)s * e->state_stride;
-    d.pA = st;
-    d.pB = st + e->n_p;
+    d.pA = st + e->off;
+    d.pB = st + e->n_p + e->off;
     d.phxA = st + 2 * e->n_p;
     d.phxB = d.phxA + e->n_x;
     a.chx = d.phxB + e->n_x;
@@ -299,6 +295,7 @@ int admit(psdm_rtm_engine *e, int s, Job &job, size_t arena_off) {
     d.g0 = e->G;
     d.nt = nt;
     d.ss = e->cfg.store_snap;
+    d.fd_ss = make_fastdiv(e->cfg.store_snap);
     d.nsnap = nsnap;
     d.ev_ptr_f = sl.d_ints + ofp;
     d.ev_cell_f = sl.d_ints + ofc;
@@ -324,8 +321,8 @@ int retire(psdm_rtm_engine *e, int s) {
     Slot &sl = e->slots[s];
     const psdm_rtm_job &J = sl.job.j;
     const int ny = e->cfg.ny;
-    float *img = e->d_img + (size_t)s * e->lds * e->cfg.max_nx;
-    float *comp = e->d_comp + (size_t)s * e->lds * e->cfg.max_nx;
+    float *img = e->d_img + (size_t)s * e->win + kIntPad;
+    float *comp = e->d_comp + (size_t)s * e->win + kIntPad;
     if (J.d_image_out)
         HIP_TRY(c, hipMemcpy2DAsync(J.d_image_out, (size_t)ny * sizeof(float), img, (size_t)e->lds * sizeof(float),
                                     (size_t)ny * sizeof(float), J.nx_w, hipMemcpyDeviceToDevice, c->stream));
@@ -400,11 +397,13 @@ int psdm_rtm_engine_create(psdm_rtm_ctx *c, const psdm_rtm_engine_cfg *cfg, psdm
     if (cfg->nslots < 1 || cfg->nslots > 256) return fail(c, PSDM_RTM_EINVAL, "nslots must be in 1..256");
     if (cfg->ny < 1 || cfg->max_nx < 1 || cfg->max_nt < 2 || cfg->nx_prop < 1)
         return fail(c, PSDM_RTM_EINVAL, "bad engine capacity");
+    if (cfg->ny >= 65536 || cfg->max_nx >= 32768) return fail(c, PSDM_RTM_EINVAL, "window exceeds 65535 x 32767 cells");
     if (cfg->dPML < 2 || cfg->dPML + 6 > kMaxTables) return fail(c, PSDM_RTM_EINVAL, "dPML out of range");
     if (cfg->store_snap < 1) return fail(c, PSDM_RTM_EINVAL, "store_snap must be >= 1");
     if (cfg->snap_format < 0 || cfg->snap_format > 2) return fail(c, PSDM_RTM_EINVAL, "snap_format must be 0, 1 or 2");
     if (cfg->tile_x != 8 && cfg->tile_x != 16 && cfg->tile_x != 32)
         return fail(c, PSDM_RTM_EINVAL, "tile_x must be 8, 16 or 32");
+    if (cfg->stream_w < 0 || cfg->stream_w > 256) return fail(c, PSDM_RTM_EINVAL, "stream_w must be in 0..256");
     if (cfg->chunk_steps < 1 || cfg->chunk_steps > 4096) return fail(c, PSDM_RTM_EINVAL, "chunk_steps out of range");
     if (!cfg->d_vel) return fail(c, PSDM_RTM_EINVAL, "null velocity model");
     if (!(cfg->dt > 0.f) || !(cfg->dmodel > 0.f)) return fail(c, PSDM_RTM_EINVAL, "dt and dmodel must be > 0");
@@ -413,24 +412,23 @@ int psdm_rtm_engine_create(psdm_rtm_ctx *c, const psdm_rtm_engine_cfg *cfg, psdm
     e->cfg = *cfg;
     e->dn = cfg->dPML + 6;
     e->nyt = cfg->ny + 2 * e->dn;
-    e->ld = (e->nyt + 63) / 64 * 64;
+    e->ld = state_ld(cfg->ny, e->dn);
     e->lds = psdm::interior_ld(cfg->ny);
+    e->off = state_off(e->dn);
+    e->sw = cfg->stream_w > 0 ? cfg->stream_w : 4;
     e->nxt_max = cfg->max_nx + 2 * e->dn;
-    {
-        const Tiling T = make_tiling(cfg->ny, cfg->max_nx, e->dn, cfg->tile_x);
-        e->ntile_y = T.nrt;
-        e->cap_tiles = T.ntiles;
-    }
+    e->cap_tiles = make_tiling(cfg->ny, cfg->max_nx, e->dn, cfg->tile_x, e->sw).nitems;
+    e->win = (size_t)e->lds * cfg->max_nx + 64;
     e->nwork = cfg->nslots * e->cap_tiles;
     e->per_xcd = (e->nwork + 7) / 8;
     e->grid = cfg->grid > 0 ? std::min(cfg->grid, e->per_xcd * 8) : e->per_xcd * 8;
     e->esize = cfg->snap_format == kSnapF32 ? 4 : 2;
     const size_t dn2 = 2 * e->dn;
-    e->n_p = (size_t)e->ld * e->nxt_max;
+    e->n_p = ((size_t)e->ld * e->nxt_max + 32 + 63) / 64 * 64;  // + room for the row offset
     e->n_x = (size_t)e->ld * dn2;
     e->n_y = dn2 * e->nxt_max;
     e->state_stride = (2 * e->n_p + 3 * e->n_x + 3 * e->n_y + 2 * dn2 + 63) / 64 * 64;
-    const size_t S = cfg->nslots, win = (size_t)psdm::interior_ld(cfg->ny) * cfg->max_nx;
+    const size_t S = cfg->nslots, win = e->win;
     size_t budget = cfg->hbm_budget > 0 ? (size_t)cfg->hbm_budget : S * snap_bytes_of(e, cfg->max_nx, cfg->max_nt);
     auto bail = [&](int rc) {
         psdm_rtm_engine_destroy(e);

@@ -77,143 +77,257 @@ __device__ __forceinline__ void stage_p(gptr<const float> pc, int ld, int nyt, i
     }
 }
 
-// Interior tile: stencil update, source injection, snapshot store (forward) or
-// zero-lag imaging (backward).
-template <int TX>
-__device__ __forceinline__ void interior_tile(const StepArgs &a, int t, int j0, int j1, int i0, int i1, float *tile,
-                                              float *srcv, float *red) {
-    constexpr int TY = kTileY, HY = TY + 6, CPW = TX / 4;
-    const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // w: wave-uniform
-    const int dn = a.dn, ld = a.ld, lds = a.lds;
-    const int j = j0 + lane, jw = j - dn;
-    const bool rowok = j < j1;
-    gptr<float> pn = gp(a.pn);
-    gptr<const float> c2 = gp(a.c2);
+// Native 4 x float / 2 x uint vectors (16- and 8-byte lane accesses).
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef unsigned v2u __attribute__((ext_vector_type(2)));
 
-    // loads that do not depend on the staged tile are issued before the barrier
-    float p1v[CPW], c2v[CPW], Fv[CPW], imv[CPW], cmv[CPW];
+// Lane L receives lane L-1's value (DPP wave_shr:1; lane 0 receives 0).
+__device__ __forceinline__ float from_prev_lane(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xf, 0xf, false));
+}
+// Lane L receives lane L+1's value (DPP wave_shl:1; lane 63 receives 0).
+__device__ __forceinline__ float from_next_lane(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xf, 0xf, false));
+}
+
+// D2 of one row of the 6th-order stencil, in the reference's order (solver_space_derivs_clean.f90:1-69):
+// ((((p+p)*c0 + (a1+b1)*c1) + (a2+b2)*c2) + (a3+b3)*c3)
+__device__ __forceinline__ float d2(float p, float a1, float b1, float a2, float b2, float a3, float b3) {
+    float d = (p + p) * kC2_0;
+    d = d + (a1 + b1) * kC2_1;
+    d = d + (a2 + b2) * kC2_2;
+    d = d + (a3 + b3) * kC2_3;
+    return d;
+}
+
+// Interior stream item u: interior rows o-4 .. o+251 of one stream row tile (4 per lane;
+// lanes 0 and 63 hold the 4-row halos) and columns [c0, cend), walked left to right with
+// p(k-1) columns i-3 .. i+3 in registers.  x neighbours come from the register window,
+// y neighbours from the lane's own 4 rows and its two neighbour lanes (DPP).  Per column:
+// update (:443-444, source injection :348-358), free surface (:463-466), then the
+// snapshot (forward, :476-500) or the imaging term (backward,
+// solver_ac_backward_ram.f90:455-481).  Only the rows the item owns are stored.
+__device__ __forceinline__ void stream_item(const StepArgs &a, int u) {
+    const int lane = threadIdx.x & 63;
+    const int dn = a.dn;
+    const int CS = fdiv(u, a.fd_ri), R = u - CS * a.ri;
+    const int o = stream_origin(R, a.ny);
+    const int own0 = R * kStreamRows, own1 = own0 + kStreamRows < a.ny ? own0 + kStreamRows : a.ny;
+    const int c0 = dn + CS * a.sw, cend = c0 + a.sw < dn + a.nx ? c0 + a.sw : dn + a.nx;
+    const int jl = o - 4 + 4 * lane;  // interior row of this lane's first value
+    const bool ow0 = jl >= own0 && jl < own1, ow1 = jl + 1 >= own0 && jl + 1 < own1;
+    const bool ow2 = jl + 2 >= own0 && jl + 2 < own1, ow3 = jl + 3 >= own0 && jl + 3 < own1;
+    const bool full = ow0 && ow3, any = ow0 || ow1 || ow2 || ow3;
+    const int ld4 = a.ld >> 2, lc4 = a.ldc >> 2;
+    const int prow = dn + o - 4;  // padded row of lane 0's first value
+    gptr<const v4f> P = gp((const v4f *)(a.pc + prow)) + lane;
+    gptr<v4f> PN = gp((v4f *)(a.pn + prow)) + lane;
+    gptr<const v4f> C2 = gp((const v4f *)(a.c2 + (o - 4))) + lane;
+    const v4f z4 = v4f{0.f, 0.f, 0.f, 0.f};
+
+    // injection events of this item, ordered by column then row: cell = iw << 16 | jw
+    int e = gp(a.ev_ptr)[u];
+    const int ev_e = gp(a.ev_ptr)[u + 1];
+    gptr<const float> inj = gp(a.inj_val) + (size_t)(a.k - 2) * a.ninj;
+
     const bool imaging = a.mode == 1 && a.snap_slot >= 0;
+    const bool snap = a.mode == 0 && a.snap_slot >= 0;
     const size_t sbase = (size_t)(a.snap_slot < 0 ? 0 : a.snap_slot) * a.snap_stride;
-#pragma unroll
-    for (int q = 0; q < CPW; ++q) {
-        const int i = i0 + w * CPW + q;
-        const bool ok = rowok && i < i1;
-        const int o = (i - dn) * lds + jw;
-        p1v[q] = (ok && !a.zp) ? pn[i * ld + j] : 0.f;
-        c2v[q] = ok ? c2[(i - dn) * a.ldc + jw] : 0.f;
-        Fv[q] = imv[q] = cmv[q] = 0.f;
-        if (imaging && ok) {
-            if (a.snap_fmt == kSnapF32)
-                Fv[q] = gp((const float *)a.snaps)[sbase + o];
-            else if (a.snap_fmt == kSnapF16)
-                Fv[q] = (float)gp((const _Float16 *)a.snaps)[sbase + o];
-            else
-                Fv[q] = bf16_to_f32(gp((const unsigned short *)a.snaps)[sbase + o]);
-            imv[q] = gp(a.image)[o];
-            if (a.kahan) cmv[q] = gp(a.image_comp)[o];
-        }
-    }
-    // injection: the tile's events (unique cells, the last writer resolved on the host)
-    // are scattered into a dense LDS array; cells without an event read 0
-    const int ev_s = gp(a.ev_ptr)[t], ev_e = gp(a.ev_ptr)[t + 1];
-    const bool has_src = ev_s < ev_e;
-    if (has_src) {
-#pragma unroll
-        for (int q = 0; q < CPW; ++q) srcv[(w * CPW + q) * TY + lane] = 0.f;
-    }
-    stage_p<TX>(gp(a.pc), ld, a.nyt, a.nxt, j0, i0, a.zc, tile);
-    __syncthreads();
-    if (has_src) {
-        for (int e = ev_s + (int)threadIdx.x; e < ev_e; e += kBlock) {
-            const int cell = gp(a.ev_cell)[e];
-            const int cj = cell % a.nyt - j0, ci = cell / a.nyt - i0;
-            srcv[ci * TY + cj] = gp(a.inj_val)[(size_t)(a.k - 2) * a.ninj + gp(a.ev_inj)[e]];
-        }
-        __syncthreads();
-    }
+    const size_t scbase = (size_t)(a.snap_slot < 0 ? 0 : a.snap_slot) * a.scale_stride + (size_t)R * a.nx;
+    const int srow = (o - 4) + 4 * lane;  // interior row of this lane's first value (array index)
 
-    // d = p(k) - p(k-2) of this thread's cells, for the snapshot / imaging pass
-    float dv[CPW];
-    bool dok[CPW];
-#pragma unroll
-    for (int q = 0; q < CPW; ++q) {
-        const int i = i0 + w * CPW + q;
-        dv[q] = 0.f;
-        dok[q] = rowok && i < i1;
-        if (!dok[q]) continue;
-        const float *T = tile + (i - i0 + 3) * HY + (lane + 3);  // T[di*HY + dj]
-        const float p = T[0], p1 = p1v[q];
-        float d2x = (p + p) * kC2_0;
-        d2x = d2x + (T[HY] + T[-HY]) * kC2_1;
-        d2x = d2x + (T[2 * HY] + T[-2 * HY]) * kC2_2;
-        d2x = d2x + (T[3 * HY] + T[-3 * HY]) * kC2_3;
-        float d2y = (p + p) * kC2_0;
-        d2y = d2y + (T[1] + T[-1]) * kC2_1;
-        d2y = d2y + (T[2] + T[-2]) * kC2_2;
-        d2y = d2y + (T[3] + T[-3]) * kC2_3;
-        float lap = d2x + d2y;
-        if (has_src) lap = lap + srcv[(w * CPW + q) * TY + lane];  // (d2x + d2y + source), :443
-        float v = (-p1 + 2.f * p) + lap * c2v[q];  // :443-444
-        if (a.fs && j == dn) v = 0.f;              // :463-466
-        pn[i * ld + j] = v;
-        dv[q] = v - p1;
+    v4f w0, w1, w2, w3, w4, w5, w6;
+    if (a.zc) {
+        w0 = w1 = w2 = w3 = w4 = w5 = w6 = z4;
+    } else {
+        w0 = P[(c0 - 3) * ld4];
+        w1 = P[(c0 - 2) * ld4];
+        w2 = P[(c0 - 1) * ld4];
+        w3 = P[c0 * ld4];
+        w4 = P[(c0 + 1) * ld4];
+        w5 = P[(c0 + 2) * ld4];
+        w6 = P[(c0 + 3) * ld4];
     }
-
-    if (a.snap_slot < 0) return;  // uniform over the workgroup
-    if (a.mode == 0) {
-        // snapshot F = ((p(k) - p(k-2)) * 0.5) / dt (:485)
-        float fv[CPW];
-#pragma unroll
-        for (int q = 0; q < CPW; ++q) fv[q] = dok[q] ? (dv[q] * 0.5f) / a.dt : 0.f;
-        if (a.snap_fmt == kSnapF32) {
-            gptr<float> S = gp((float *)a.snaps) + sbase;
-#pragma unroll
-            for (int q = 0; q < CPW; ++q)
-                if (dok[q]) S[(i0 + w * CPW + q - dn) * lds + jw] = fv[q];
-            return;
-        }
-        // 16-bit snapshot: power-of-two scale from the tile's max-abs
-        float m = 0.f;
-#pragma unroll
-        for (int q = 0; q < CPW; ++q) m = fmaxf(m, fabsf(fv[q]));
-        m = wave_max(m);
-        if (lane == 0) red[w] = m;
-        __syncthreads();
-        m = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
-        __syncthreads();  // red is reused by the next tile
-        const int target = a.snap_fmt == kSnapF16 ? 14 : 0;
-        const int e = m > 0.f ? ilogbf(m) - target : 0;
-        if (threadIdx.x == 0) gp(a.snap_scale)[(size_t)a.snap_slot * a.ntiles + t] = ldexpf(1.f, e);
-#pragma unroll
-        for (int q = 0; q < CPW; ++q)
-            if (dok[q]) {
-                const size_t so = sbase + (size_t)((i0 + w * CPW + q - dn) * lds + jw);
-                const float s = ldexpf(fv[q], -e);
-                if (a.snap_fmt == kSnapF16)
-                    gp((_Float16 *)a.snaps)[so] = (_Float16)s;
-                else
-                    gp((unsigned short *)a.snaps)[so] = f32_to_bf16(s);
-            }
-        return;
-    }
-    // imaging: kernel += 0.5*(q(k)-q(k-2)) * F(snap) * store_snap (solver_ac_backward_ram.f90:465)
-    const float scale = a.snap_fmt == kSnapF32 ? 1.f : gp(a.snap_scale)[(size_t)a.snap_slot * a.ntiles + t];
-    gptr<float> img = gp(a.image);
-#pragma unroll
-    for (int q = 0; q < CPW; ++q)
-        if (dok[q]) {
-            const int o = (i0 + w * CPW + q - dn) * lds + jw;
-            const float F = a.snap_fmt == kSnapF32 ? Fv[q] : Fv[q] * scale;
-            const float term = ((0.5f * dv[q]) * F) * a.ss_f;
-            if (a.kahan) {
-                const float y = term - cmv[q];
-                const float tt = imv[q] + y;
-                gp(a.image_comp)[o] = (tt - imv[q]) - y;
-                img[o] = tt;
+    for (int i = c0; i < cend; ++i) {
+        const int iw = i - dn;
+        const v4f p1 = a.zp ? z4 : PN[i * ld4];
+        const v4f cc = C2[iw * lc4];
+        const v4f nw = a.zc ? z4 : P[(i + 4) * ld4];
+        v4f F4 = z4, im4 = z4, cm4 = z4;
+        float scale = 1.f;
+        if (imaging) {
+            const size_t so = sbase + (size_t)iw * a.lds + srow;
+            if (a.snap_fmt == kSnapF32) {
+                F4 = *gp((const v4f *)((const float *)a.snaps + so));
             } else {
-                img[o] = imv[q] + term;
+                const v2u h = *gp((const v2u *)((const unsigned short *)a.snaps + so));
+                if (a.snap_fmt == kSnapF16) {
+                    F4.x = (float)__builtin_bit_cast(_Float16, (unsigned short)(h.x & 0xffffu));
+                    F4.y = (float)__builtin_bit_cast(_Float16, (unsigned short)(h.x >> 16));
+                    F4.z = (float)__builtin_bit_cast(_Float16, (unsigned short)(h.y & 0xffffu));
+                    F4.w = (float)__builtin_bit_cast(_Float16, (unsigned short)(h.y >> 16));
+                } else {
+                    F4.x = bf16_to_f32((unsigned short)(h.x & 0xffffu));
+                    F4.y = bf16_to_f32((unsigned short)(h.x >> 16));
+                    F4.z = bf16_to_f32((unsigned short)(h.y & 0xffffu));
+                    F4.w = bf16_to_f32((unsigned short)(h.y >> 16));
+                }
+                scale = gp(a.snap_scale)[scbase + iw];
+            }
+            im4 = *gp((const v4f *)(a.image + (size_t)iw * a.lds + srow));
+            if (a.kahan) cm4 = *gp((const v4f *)(a.image_comp + (size_t)iw * a.lds + srow));
+        }
+        // y neighbours: the previous lane's rows 1..3 and the next lane's rows 0..2
+        const float b1 = from_prev_lane(w3.y), b2 = from_prev_lane(w3.z), b3 = from_prev_lane(w3.w);
+        const float n0 = from_next_lane(w3.x), n1 = from_next_lane(w3.y), n2 = from_next_lane(w3.z);
+        v4f lap;
+        lap.x = d2(w3.x, w4.x, w2.x, w5.x, w1.x, w6.x, w0.x) + d2(w3.x, w3.y, b3, w3.z, b2, w3.w, b1);
+        lap.y = d2(w3.y, w4.y, w2.y, w5.y, w1.y, w6.y, w0.y) + d2(w3.y, w3.z, w3.x, w3.w, b3, n0, b2);
+        lap.z = d2(w3.z, w4.z, w2.z, w5.z, w1.z, w6.z, w0.z) + d2(w3.z, w3.w, w3.y, n0, w3.x, n1, b3);
+        lap.w = d2(w3.w, w4.w, w2.w, w5.w, w1.w, w6.w, w0.w) + d2(w3.w, n0, w3.z, n1, w3.y, n2, w3.x);
+        // (d2x + d2y + source) at the injection cells of this column (:443)
+        while (e < ev_e) {
+            const int cell = gp(a.ev_cell)[e];
+            if ((cell >> 16) != iw) break;
+            const int r = (cell & 0xffff) - (o - 4);
+            const float val = inj[gp(a.ev_inj)[e]];
+            if (lane == (r >> 2)) {
+                const int q = r & 3;
+                if (q == 0) lap.x = lap.x + val;
+                else if (q == 1) lap.y = lap.y + val;
+                else if (q == 2) lap.z = lap.z + val;
+                else lap.w = lap.w + val;
+            }
+            ++e;
+        }
+        v4f v;
+        v.x = (-p1.x + 2.f * w3.x) + lap.x * cc.x;  // :443-444
+        v.y = (-p1.y + 2.f * w3.y) + lap.y * cc.y;
+        v.z = (-p1.z + 2.f * w3.z) + lap.z * cc.z;
+        v.w = (-p1.w + 2.f * w3.w) + lap.w * cc.w;
+        if (a.fs && o == 0) {  // interior row 0 is lane 1's first value (:463-466)
+            if (lane == 1) v.x = 0.f;
+        }
+        if (full) {
+            PN[i * ld4] = v;
+        } else if (any) {
+            float *q = a.pn + prow + 4 * lane + (size_t)i * a.ld;
+            if (ow0) gp(q)[0] = v.x;
+            if (ow1) gp(q)[1] = v.y;
+            if (ow2) gp(q)[2] = v.z;
+            if (ow3) gp(q)[3] = v.w;
+        }
+        v4f dv;
+        dv.x = v.x - p1.x;
+        dv.y = v.y - p1.y;
+        dv.z = v.z - p1.z;
+        dv.w = v.w - p1.w;
+        if (snap) {
+            // F = ((p(k) - p(k-2)) * 0.5) / dt (:485)
+            v4f f;
+            f.x = (dv.x * 0.5f) / a.dt;
+            f.y = (dv.y * 0.5f) / a.dt;
+            f.z = (dv.z * 0.5f) / a.dt;
+            f.w = (dv.w * 0.5f) / a.dt;
+            const size_t so = sbase + (size_t)iw * a.lds + srow;
+            if (a.snap_fmt == kSnapF32) {
+                float *q = (float *)a.snaps + so;
+                if (full) {
+                    *gp((v4f *)q) = f;
+                } else if (any) {
+                    if (ow0) gp(q)[0] = f.x;
+                    if (ow1) gp(q)[1] = f.y;
+                    if (ow2) gp(q)[2] = f.z;
+                    if (ow3) gp(q)[3] = f.w;
+                }
+            } else {
+                // 16-bit: power-of-two scale from the column's max-abs over the owned rows
+                float m = 0.f;
+                if (ow0) m = fmaxf(m, fabsf(f.x));
+                if (ow1) m = fmaxf(m, fabsf(f.y));
+                if (ow2) m = fmaxf(m, fabsf(f.z));
+                if (ow3) m = fmaxf(m, fabsf(f.w));
+                m = wave_max(m);
+                const int target = a.snap_fmt == kSnapF16 ? 14 : 0;
+                const int ex = m > 0.f ? ilogbf(m) - target : 0;
+                if (lane == 0) gp(a.snap_scale)[scbase + iw] = ldexpf(1.f, ex);
+                unsigned short hs[4];
+                const float fs4[4] = {f.x, f.y, f.z, f.w};
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    const float sv = ldexpf(fs4[c], -ex);
+                    hs[c] = a.snap_fmt == kSnapF16 ? __builtin_bit_cast(unsigned short, (_Float16)sv) : f32_to_bf16(sv);
+                }
+                unsigned short *q = (unsigned short *)a.snaps + so;
+                if (full) {
+                    *gp((v2u *)q) = v2u{(unsigned)hs[0] | ((unsigned)hs[1] << 16), (unsigned)hs[2] | ((unsigned)hs[3] << 16)};
+                } else if (any) {
+                    if (ow0) gp(q)[0] = hs[0];
+                    if (ow1) gp(q)[1] = hs[1];
+                    if (ow2) gp(q)[2] = hs[2];
+                    if (ow3) gp(q)[3] = hs[3];
+                }
+            }
+        } else if (imaging) {
+            // kernel += 0.5*(q(k)-q(k-2)) * F(snap) * store_snap (solver_ac_backward_ram.f90:465)
+            v4f F = F4;
+            if (a.snap_fmt != kSnapF32) {
+                F.x = F.x * scale;
+                F.y = F.y * scale;
+                F.z = F.z * scale;
+                F.w = F.w * scale;
+            }
+            v4f term, img, cmp;
+            term.x = ((0.5f * dv.x) * F.x) * a.ss_f;
+            term.y = ((0.5f * dv.y) * F.y) * a.ss_f;
+            term.z = ((0.5f * dv.z) * F.z) * a.ss_f;
+            term.w = ((0.5f * dv.w) * F.w) * a.ss_f;
+            if (a.kahan) {
+                v4f y;
+                y.x = term.x - cm4.x;
+                y.y = term.y - cm4.y;
+                y.z = term.z - cm4.z;
+                y.w = term.w - cm4.w;
+                img.x = im4.x + y.x;
+                img.y = im4.y + y.y;
+                img.z = im4.z + y.z;
+                img.w = im4.w + y.w;
+                cmp.x = (img.x - im4.x) - y.x;
+                cmp.y = (img.y - im4.y) - y.y;
+                cmp.z = (img.z - im4.z) - y.z;
+                cmp.w = (img.w - im4.w) - y.w;
+            } else {
+                img.x = im4.x + term.x;
+                img.y = im4.y + term.y;
+                img.z = im4.z + term.z;
+                img.w = im4.w + term.w;
+            }
+            float *qi = a.image + (size_t)iw * a.lds + srow;
+            float *qc = a.image_comp + (size_t)iw * a.lds + srow;
+            if (full) {
+                *gp((v4f *)qi) = img;
+                if (a.kahan) *gp((v4f *)qc) = cmp;
+            } else if (any) {
+                if (ow0) gp(qi)[0] = img.x;
+                if (ow1) gp(qi)[1] = img.y;
+                if (ow2) gp(qi)[2] = img.z;
+                if (ow3) gp(qi)[3] = img.w;
+                if (a.kahan) {
+                    if (ow0) gp(qc)[0] = cmp.x;
+                    if (ow1) gp(qc)[1] = cmp.y;
+                    if (ow2) gp(qc)[2] = cmp.z;
+                    if (ow3) gp(qc)[3] = cmp.w;
+                }
             }
         }
+        w0 = w1;
+        w1 = w2;
+        w2 = w3;
+        w3 = w4;
+        w4 = w5;
+        w5 = w6;
+        w6 = nw;
+    }
 }
 
 // CPML tile: x strip (xs != 0, ys == 0), y strip (xs == 0, ys != 0) or corner.  xs / ys:
@@ -373,23 +487,36 @@ __device__ __forceinline__ void cpml_tile(const StepArgs &a, int xs, int ys, int
     }
 }
 
-// One time step of one tile (tile index t, region-aligned tiling) of one shot.  The
-// branch is uniform over the workgroup; all its threads take part in the barriers.
+// One work item of one time step of one shot: t < ncpml is a CPML tile (the whole
+// workgroup, LDS-staged), otherwise a group of four interior stream items (one per
+// wavefront, no workgroup barrier).  The branch is uniform over the workgroup.
 template <int TX>
-__device__ __forceinline__ void step_tile(const StepArgs &a, int t, float *tile, float *aux, float *red) {
+__device__ __forceinline__ void step_item(const StepArgs &a, int t, float *tile, float *aux) {
+    if (t >= a.ncpml) {
+        const int u = 4 * (t - a.ncpml) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+        if (u < a.nstream) stream_item(a, u);
+        return;
+    }
+    int R, C;
+    if (t < a.ncpml_a) {  // top / bottom strips
+        C = fdiv(t, a.fd_2rp);
+        const int r = t - C * 2 * a.rt_pml;
+        R = r < a.rt_pml ? r : r + a.rt_int;
+    } else {  // x strips over the interior rows
+        const int t2 = t - a.ncpml_a;
+        const int cc = fdiv(t2, a.fd_rint);
+        R = a.rt_pml + (t2 - cc * a.rt_int);
+        C = cc < a.ct_pml ? cc : cc + a.ct_int;
+    }
     int j0, j1, i0, i1;
-    const int ys = axis_tile(t % a.ntile_y, a.rt_pml, a.rt_int, a.dn, a.ny, kTileY, &j0, &j1);
-    const int xs = axis_tile(t / a.ntile_y, a.ct_pml, a.ct_int, a.dn, a.nx, TX, &i0, &i1);
-    if ((xs | ys) == 0)
-        interior_tile<TX>(a, t, j0, j1, i0, i1, tile, aux, red);
-    else
-        cpml_tile<TX>(a, xs, ys, j0, j1, i0, i1, tile, aux);
+    const int ys = axis_tile(R, a.rt_pml, a.rt_int, a.dn, a.ny, kTileY, a.fd_rpml, &j0, &j1);
+    const int xs = axis_tile(C, a.ct_pml, a.ct_int, a.dn, a.nx, TX, a.fd_cpml, &i0, &i1);
+    cpml_tile<TX>(a, xs, ys, j0, j1, i0, i1, tile, aux);
 }
 
 template <int TX>
 __global__ __launch_bounds__(kBlock) void rtm_step_kernel(StepArgs a) {
     __shared__ float tile[(TX + 6) * (kTileY + 6)];
-    __shared__ float red[4];
     __shared__ float aux[(TX + 6) * kTileY];
     const int b = blockIdx.x;
     // traces of the previous step (:567-571), read from p(k-1) before it is overwritten
@@ -401,8 +528,8 @@ __global__ __launch_bounds__(kBlock) void rtm_step_kernel(StepArgs a) {
     // XCD-aware order: the 8 XCDs take blocks round-robin, so XCD x gets the
     // contiguous tile range [x*per_xcd, (x+1)*per_xcd) and shares halos in its own L2.
     const int t = (b & 7) * a.per_xcd + (b >> 3);
-    if (t >= a.ntiles) return;
-    step_tile<TX>(a, t, tile, aux, red);
+    if (t >= a.nitems) return;
+    step_item<TX>(a, t, tile, aux);
 }
 
 // Per-step fields of a slot at global step G (forward steps k = 2..nt, then backward
@@ -431,10 +558,8 @@ __device__ __forceinline__ bool resolve_step(const SlotDesc &sd, long long G, St
     a.rec_row = -1;
     int slot = -1;
     if (fwd) {
-        if ((k - 1) % sd.ss == 0) {
-            const int s = (k - 1) / sd.ss;
-            if (s <= sd.nsnap) slot = s - 1;
-        }
+        const int s = fdiv(k - 1, sd.fd_ss);
+        if (s * sd.ss == k - 1 && s <= sd.nsnap) slot = s - 1;
         a.ev_ptr = sd.ev_ptr_f;
         a.ev_cell = sd.ev_cell_f;
         a.ev_inj = sd.ev_inj_f;
@@ -442,10 +567,8 @@ __device__ __forceinline__ bool resolve_step(const SlotDesc &sd, long long G, St
         a.ninj = sd.ninj_f;
     } else {
         // backward step k pairs with the forward snapshot taken at k_f = nt + 2 - k
-        if ((nt + 1 - k) % sd.ss == 0) {
-            const int s = (nt + 1 - k) / sd.ss;
-            if (s >= 1 && s <= sd.nsnap) slot = s - 1;
-        }
+        const int s = fdiv(nt + 1 - k, sd.fd_ss);
+        if (s * sd.ss == nt + 1 - k && s >= 1 && s <= sd.nsnap) slot = s - 1;
         a.ev_ptr = sd.ev_ptr_b;
         a.ev_cell = sd.ev_cell_b;
         a.ev_inj = sd.ev_inj_b;
@@ -464,18 +587,17 @@ template <int TX>
 __global__ __launch_bounds__(kBlock) void rtm_batch_kernel(BatchArgs b, const SlotDesc *__restrict__ slots,
                                                            const long long *__restrict__ gbase) {
     __shared__ float tile[(TX + 6) * (kTileY + 6)];
-    __shared__ float red[4];
     __shared__ float aux[(TX + 6) * kTileY];
     const long long G = *gbase + b.goff;
     const int x = blockIdx.x & 7, nbx = gridDim.x >> 3;
     for (int m = blockIdx.x >> 3; m < b.per_xcd; m += nbx) {
         const int wk = x * b.per_xcd + m;
         if (wk >= b.nwork) break;
-        const int s = wk / b.cap_tiles, t = wk - s * b.cap_tiles;
+        const int s = fdiv(wk, b.fd_cap), t = wk - s * b.cap_tiles;
         const SlotDesc &sd = slots[s];
         StepArgs a = sd.a;
-        if (!resolve_step(sd, G, a) || t >= a.ntiles) continue;
-        step_tile<TX>(a, t, tile, aux, red);
+        if (!resolve_step(sd, G, a) || t >= a.nitems) continue;
+        step_item<TX>(a, t, tile, aux);
         __syncthreads();  // the LDS buffers are reused by the next item
     }
 }

@@ -151,7 +151,7 @@ def test_engine_hbm_budget_admission(ctx):
     """A budget that holds one FP32 snapshot store at a time runs the shots one after
     the other in a 3-slot engine, with the same results."""
     vel = model(3)
-    lds = -(-NY // 64) * 64  # snapshot rows are padded to a multiple of 64
+    lds = -(-(max(NY, 248) + 8) // 64) * 64  # snapshot column length (interior_ld)
     one = (SHOTS[1][2] - 1) * lds * SHOTS[1][1] * 4 + 4096
     say("[gpu] engine hbm budget admission")
     a, _, sa = run_engine(ctx, vel, SHOTS, [1, 1, 1], nslots=3, hbm_budget=one, chunk_steps=16)
