@@ -445,7 +445,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=64, help="shots timed (per form)")
     ap.add_argument("--warmup", type=int, default=1, help="untimed shots before the plain timed region")
-    ap.add_argument("--tile-x", type=int, default=16, help="plain form tile width (8, 16 or 32)")
+    ap.add_argument("--tile-x", type=int, default=8, help="plain form CPML tile width (8, 16 or 32)")
     ap.add_argument("--timing-every", type=int, default=16, help="sample every n-th step kernel with HIP events")
     ap.add_argument("--cpu-procs", type=int, default=16)
     ap.add_argument("--cpu-steps", type=int, default=200)

@@ -120,6 +120,9 @@ long long psdm_rtm_device_bytes(const psdm_rtm_ctx *ctx);
  * hipGraphs of `graph_steps` steps.  timing != 0 records HIP events around every step
  * kernel; psdm_rtm_step_stats returns their count and mean duration (ms). */
 int psdm_rtm_set_launch(psdm_rtm_ctx *ctx, int tile_x, int use_graph, int graph_steps);
+/* Interior columns per wavefront work item of the one-shot solver (default 2; shorter
+ * items spread one shot over more wavefronts). */
+int psdm_rtm_set_stream_width(psdm_rtm_ctx *ctx, int stream_w);
 int psdm_rtm_set_timing(psdm_rtm_ctx *ctx, int timing);
 int psdm_rtm_step_stats(psdm_rtm_ctx *ctx, long long *n_steps, double *mean_ms);
 

@@ -31,6 +31,7 @@ SIGNATURES = {
     "psdm_rtm_normalize": ([_P, _P, _P, _I, _I, _P], _I),
     "psdm_rtm_device_bytes": ([_P], _LL),
     "psdm_rtm_set_launch": ([_P, _I, _I, _I], _I),
+    "psdm_rtm_set_stream_width": ([_P, _I], _I),
     "psdm_rtm_set_timing": ([_P, _I], _I),
     "psdm_rtm_step_stats": ([_P, C.POINTER(_LL), C.POINTER(C.c_double)], _I),
     "psdm_rtm_engine_create": ([_P, _P, C.POINTER(_P)], _I),
@@ -120,6 +121,9 @@ class Context:
     # launch controls ------------------------------------------------------------
     def set_launch(self, tile_x: int = 8, use_graph: bool = False, graph_steps: int = 64):
         self.check(self._L.psdm_rtm_set_launch(self.h, tile_x, int(use_graph), graph_steps), "set_launch")
+
+    def set_stream_width(self, stream_w: int):
+        self.check(self._L.psdm_rtm_set_stream_width(self.h, stream_w), "set_stream_width")
 
     def set_timing(self, every: int):
         self.check(self._L.psdm_rtm_set_timing(self.h, every), "set_timing")

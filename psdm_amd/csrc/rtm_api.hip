@@ -17,10 +17,6 @@ using psdm::StepArgs;
 
 namespace {
 
-// Interior columns per stream item of the one-shot solver: short items, so that one
-// shot's interior spreads over enough wavefronts to fill the chip.
-constexpr int kPlainStreamW = 4;
-
 // Geometry of one call, 0-based padded-grid quantities.
 struct Layout {
     int ny, nx, nt, dn, nyt, nxt, ld, lds, off, fs, ram, ss, nt_snap;
@@ -145,7 +141,7 @@ int check_geom(psdm_rtm_ctx *c, const psdm_rtm_geom *g) {
     return 0;
 }
 
-Layout make_layout(const psdm_rtm_geom *g, int tile_x) {
+Layout make_layout(const psdm_rtm_geom *g, int tile_x, int stream_w) {
     Layout L;
     L.ny = g->ny;
     L.nx = g->nx;
@@ -163,7 +159,7 @@ Layout make_layout(const psdm_rtm_geom *g, int tile_x) {
     L.dx = g->dmodel;
     L.dt = g->dt;
     L.tile_x = tile_x;
-    L.T = psdm::make_tiling(L.ny, L.nx, L.dn, tile_x, kPlainStreamW);
+    L.T = psdm::make_tiling(L.ny, L.nx, L.dn, tile_x, stream_w);
     L.per_xcd = (L.T.nitems + 7) / 8;
     return L;
 }
@@ -367,7 +363,7 @@ int run_loop(psdm_rtm_ctx *c, const Layout &L, Prepared &P, int mode, int nslots
 int fwd_device(psdm_rtm_ctx *c, const psdm_rtm_geom *g, const float *d_ctmp, const float *d_S, float *d_rec,
                long long key) {
     if (int rc = check_geom(c, g)) return rc;
-    const Layout L = make_layout(g, c->tile_x);
+    const Layout L = make_layout(g, c->tile_x, c->stream_w);
     Prepared P;
     if (int rc = prepare(c, L, d_ctmp, d_S, g->NumSou, g->nxSou, g->nySou, g->NumSou, g->nxRec, g->nyRec, g->NumRec, P))
         return rc;
@@ -392,7 +388,7 @@ int bwd_device(psdm_rtm_ctx *c, const psdm_rtm_geom *g, const float *d_ctmp, con
     if (it == c->snaps.end()) return fail(c, PSDM_RTM_ENOENT, "no device-resident snapshots under this key");
     if (it->second.ny != g->ny || it->second.nx != g->nx)
         return fail(c, PSDM_RTM_EINVAL, "snapshot window differs from the backward window");
-    const Layout L = make_layout(g, c->tile_x);
+    const Layout L = make_layout(g, c->tile_x, c->stream_w);
     Prepared P;
     if (int rc = prepare(c, L, d_ctmp, d_Sadj, g->NumRec, g->nxRec, g->nyRec, g->NumRec, nullptr, nullptr, 0, P))
         return rc;
@@ -657,6 +653,13 @@ int psdm_rtm_set_launch(psdm_rtm_ctx *c, int tile_x, int use_graph, int graph_st
     c->tile_x = tile_x;
     c->use_graph = use_graph;
     c->graph_steps = graph_steps;
+    return 0;
+}
+
+int psdm_rtm_set_stream_width(psdm_rtm_ctx *c, int stream_w) {
+    if (!c) return PSDM_RTM_EINVAL;
+    if (stream_w < 1 || stream_w > 256) return fail(c, PSDM_RTM_EINVAL, "stream_w must be in 1..256");
+    c->stream_w = stream_w;
     return 0;
 }
 

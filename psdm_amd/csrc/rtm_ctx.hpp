@@ -35,6 +35,7 @@ struct psdm_rtm_ctx {
     std::map<long long, psdm::host::SnapStore> snaps;
     std::vector<std::pair<void *, size_t>> spare;  // released snapshot buffers
     int tile_x = 8;
+    int stream_w = 2;  // interior columns per stream item of the one-shot solver
     int use_graph = 0, graph_steps = 64;
     int timing = 0;                                 // sample every `timing`-th step kernel (0 = off)
     std::vector<hipEvent_t> ev_free;                // recycled events

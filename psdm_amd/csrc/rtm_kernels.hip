@@ -372,21 +372,46 @@ __device__ __forceinline__ void cpml_tile(const StepArgs &a, int xs, int ys, int
             phs[li * TY + lane] = ok ? ph[mc * ld + j] : 0.f;
         }
     }
-    float p1v[CPW], c2v[CPW];
+    // every global operand of the update is loaded before the barrier, so a tile costs
+    // one memory round trip: p(k-2), c2, the damping factors, the strips' chi (x) and
+    // the y memory terms, the _ram strip history
+    gptr<const float> fA = gp(a.fA), fB = gp(a.fB);
+    const int jc = clampi(j, 3, a.nyt - 4);
+    const int disty = ys == 0 ? 0 : (ys == 1 ? dn - j : j - (dn + a.ny) + 1);  // 1 next to the interior
+    const bool own = ys == 0 && a.ram && a.fs && j == dn;  // x strip row dn+1 keeps its own history
+    float p1v[CPW], c2v[CPW], Av[CPW], Bv[CPW], chx_o[CPW], dphy[CPW], phy_o[CPW], chy_o[CPW], hsc[CPW], hsn[CPW];
 #pragma unroll
     for (int q = 0; q < CPW; ++q) {
         const int i = i0 + w * CPW + q;
         const bool ok = rowok && i < i1;
         p1v[q] = (ok && !a.zp) ? pn[i * ld + j] : 0.f;
         c2v[q] = ok ? c2_at(a, j, i) : 0.f;
+        const int distx = xs == 0 ? 0 : (xs == 1 ? dn - i : i - (dn + a.nx) + 1);
+        // corners use F(max(dist_x, dist_y)) for both directions (solver_PML_coeffs.f90:288-322)
+        const int dist = distx > disty ? distx : disty;
+        Av[q] = ok ? fA[dist - 1] : 0.f;
+        Bv[q] = ok ? fB[dist - 1] : 0.f;
+        chx_o[q] = (ok && xs != 0 && !a.zm) ? gp((const float *)a.chx)[(i - xoff) * ld + j] : 0.f;
+        dphy[q] = phy_o[q] = chy_o[q] = 0.f;
+        if (ok && ys != 0 && !a.zm) {
+            const int jl = ys == 1 ? j : j - (dn + a.ny);
+            const int yr = ys == 1 ? j : j - a.ny;  // row of the y memory arrays
+            const int yrc = yr - jl + clampi(jl, 3, dn - 4);
+            gptr<const float> ph = gp(a.phy_o) + i * dn2;
+            float d = (ph[yrc + 1] - ph[yrc - 1]) * kC1_1;
+            d = d + (ph[yrc + 2] - ph[yrc - 2]) * kC1_2;
+            d = d + (ph[yrc + 3] - ph[yrc - 3]) * kC1_3;
+            dphy[q] = d;
+            phy_o[q] = ph[yr];
+            chy_o[q] = gp((const float *)a.chy)[i * dn2 + yr];
+        }
+        hsc[q] = (ok && own && !a.zc) ? gp(a.hs_c)[i - xoff] : 0.f;
+        hsn[q] = (ok && own && !a.zp) ? gp((const float *)a.hs_n)[i - xoff] : 0.f;
     }
     stage_p<TX>(gp(a.pc), ld, a.nyt, a.nxt, j0, i0, a.zc, tile);
     __syncthreads();
     if (!rowok) return;
 
-    gptr<const float> fA = gp(a.fA), fB = gp(a.fB);
-    const int jc = clampi(j, 3, a.nyt - 4);
-    const int disty = ys == 0 ? 0 : (ys == 1 ? dn - j : j - (dn + a.ny) + 1);  // 1 next to the interior
 #pragma unroll
     for (int q = 0; q < CPW; ++q) {
         const int i = i0 + w * CPW + q;
@@ -415,14 +440,11 @@ __device__ __forceinline__ void cpml_tile(const StepArgs &a, int xs, int ys, int
         d2y = d2y + (yp2 + ym2) * kC2_2;
         d2y = d2y + (yp3 + ym3) * kC2_3;
         const float p = tile[(i - i0 + 3) * HY + (lane + 3)];
-        const int distx = xs == 0 ? 0 : (xs == 1 ? dn - i : i - (dn + a.nx) + 1);
-        // corners use F(max(dist_x, dist_y)) for both directions (solver_PML_coeffs.f90:288-322)
-        const int dist = distx > disty ? distx : disty;
-        const float A = fA[dist - 1], B = fB[dist - 1];
+        const float A = Av[q], B = Bv[q];
         // x-direction memory (strips xl/xr and the corners), pml_matrices_x/_xycorner(1)
         float propx = 0.f;
         if (xs != 0) {
-            const int xc = i - xoff;                          // memory column of this cell
+            const int xc = i - xoff;                            // memory column of this cell
             const int xcc = xlo + clampi(xc - xlo, 3, dn - 4);  // clamped derivative column
             const float *ph = phs + (xcc - xm0 + 3) * TY + lane;
             float dphi = (ph[TY] - ph[-TY]) * kC1_1;
@@ -430,7 +452,7 @@ __device__ __forceinline__ void cpml_tile(const StepArgs &a, int xs, int ys, int
             dphi = dphi + (ph[3 * TY] - ph[-3 * TY]) * kC1_3;
             const float pho = phs[(xc - xm0 + 3) * TY + lane];
             const int xo = xc * ld + j;
-            const float cho = a.zm ? 0.f : gp((const float *)a.chx)[xo];
+            const float cho = chx_o[q];
             float d1x = (xp1 - xm1) * kC1_1;
             d1x = d1x + (xp2 - xm2) * kC1_2;
             d1x = d1x + (xp3 - xm3) * kC1_3;
@@ -442,24 +464,14 @@ __device__ __forceinline__ void cpml_tile(const StepArgs &a, int xs, int ys, int
         }
         float propy = 0.f;
         if (ys != 0) {
-            const int jl = ys == 1 ? j : j - (dn + a.ny);
             const int yr = ys == 1 ? j : j - a.ny;  // row of the y memory arrays
-            const int yrc = yr - jl + clampi(jl, 3, dn - 4);
-            float dphi = 0.f, pho = 0.f, cho = 0.f;
             const int yo = i * dn2 + yr;
-            if (!a.zm) {
-                gptr<const float> ph = gp(a.phy_o) + i * dn2;
-                dphi = (ph[yrc + 1] - ph[yrc - 1]) * kC1_1;
-                dphi = dphi + (ph[yrc + 2] - ph[yrc - 2]) * kC1_2;
-                dphi = dphi + (ph[yrc + 3] - ph[yrc - 3]) * kC1_3;
-                pho = ph[yr];
-                cho = gp((const float *)a.chy)[yo];
-            }
+            const float dphi = dphy[q];
             float d1y = (yp1 - ym1) * kC1_1;
             d1y = d1y + (yp2 - ym2) * kC1_2;
             d1y = d1y + (yp3 - ym3) * kC1_3;
-            const float phn = A * pho + B * d1y;
-            const float chn = A * cho + B * (d2y + dphi);
+            const float phn = A * phy_o[q] + B * d1y;
+            const float chn = A * chy_o[q] + B * (d2y + dphi);
             gp(a.phy_n)[yo] = phn;
             gp(a.chy)[yo] = chn;
             propy = dphi + chn;
@@ -468,13 +480,11 @@ __device__ __forceinline__ void cpml_tile(const StepArgs &a, int xs, int ys, int
         if (ys == 0) {
             // x strip, interior rows (:376-382).  With a free surface the _ram form
             // zeroes row dn+1 of the grid but the strip keeps its own history there.
-            const bool own = a.ram && a.fs && j == dn;
-            const int hsi = i - xoff;
-            const float ps = own ? (a.zc ? 0.f : gp(a.hs_c)[hsi]) : p;
-            const float p1s = own ? (a.zp ? 0.f : gp((const float *)a.hs_n)[hsi]) : p1;
+            const float ps = own ? hsc[q] : p;
+            const float p1s = own ? hsn[q] : p1;
             v = (-p1s + 2.f * ps) + c2 * ((d2x + d2y) + propx);
             if (own) {
-                gp(a.hs_n)[hsi] = v;
+                gp(a.hs_n)[i - xoff] = v;
                 v = 0.f;
             }
         } else if (xs == 0) {
