@@ -177,8 +177,9 @@ def run_batched(ctx, torch, dev, W, d_mig, d_src, obs, steps, warmup, cfg):
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     st = eng.stats()
+    launch_ms, n_launch = eng.timing()
     eng.close()
-    return dict(elapsed=elapsed, stats=st, psdm_sum=psdm_sum, fold=fold)
+    return dict(elapsed=elapsed, stats=st, launch_ms=launch_ms, n_launch=n_launch, psdm_sum=psdm_sum, fold=fold)
 
 
 def job_of(W, k, d_src, obs):
@@ -534,7 +535,7 @@ def main():
         log(f"[bench] plain {plain['shots_per_s']} shots/s")
 
     if "batched" in forms:
-        cfg = dict(nslots=16, tile_x=32, stream_w=4, grid=0, chunk_steps=64)
+        cfg = dict(nslots=16, tile_x=16, stream_w=4, grid=0, chunk_steps=64)
         tuned = None
         if not args.no_tune:
             from psdm_amd.autotune import tune
@@ -546,7 +547,7 @@ def main():
             tuned = {"best": best, "trials": trials}
         B = run_batched(ctx, torch, dev, W, d_mig, d_src, obs, args.steps, args.warmup, cfg)
         st = B["stats"]
-        per_launch_ms = 1e3 * B["elapsed"] / max(st["steps_issued"], 1)
+        per_launch_ms = B["launch_ms"]  # device time per step launch (events around the captured chunks)
         bytes_launch = step_b * st["slot_steps"] / max(st["steps_issued"], 1)
         ach = bytes_launch / (per_launch_ms * 1e-3) / 1e9
         batched = {"shots_per_s": round(args.steps / B["elapsed"], 4),
@@ -554,9 +555,10 @@ def main():
                    "config": cfg, "autotune": tuned,
                    "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                 "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "rtm_batch_kernel",
-                                "mean_kernel_ms": round(per_launch_ms, 5),
+                                "mean_kernel_ms": round(per_launch_ms, 5), "timed_launches": B["n_launch"],
                                 "algorithmic_bytes_per_launch": round(bytes_launch),
-                                "note": "per-launch time = timed region / launches (includes graph-node gaps)"}}
+                                "wall_ms_per_launch": round(1e3 * B["elapsed"] / max(st["steps_issued"], 1), 5),
+                                "note": "mean_kernel_ms: HIP events around each captured chunk of launches"}}
         if plain is not None:
             a = B["psdm_sum"].cpu().numpy()
             b = P["psdm_sum"].cpu().numpy()
@@ -564,13 +566,16 @@ def main():
             batched["fold_equal"] = bool(torch.equal(B["fold"], P["fold"]))
         log(f"[bench] batched {batched['shots_per_s']} shots/s")
 
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic_rtm_step_kernel_mcs2001.md")
-    if os.path.exists(tpath):
+    def traffic_of(name):
+        """HBM bytes per launch measured with rocprofv3 counters (profiles/<name>)."""
         import re
 
+        tpath = os.path.join(ROOT, "profiles", name)
+        if not os.path.exists(tpath):
+            return None
         mt = re.search(r"^traffic_bytes_per_launch = (\d+)", open(tpath).read(), re.M)
-        traffic = int(mt.group(1)) if mt else None
+        return int(mt.group(1)) if mt else None
+
 
     acc = None
     if not args.no_accuracy:
@@ -579,9 +584,11 @@ def main():
 
     head = batched or plain
     form = "batched" if batched else "plain"
+    if plain is not None:
+        plain["roofline"]["traffic"] = traffic_of("traffic_rtm_step_kernel_mcs2001.md")
+    if batched is not None:
+        batched["roofline"]["traffic"] = traffic_of("traffic_rtm_batch_kernel_mcs2001.md")
     roof = dict(head["roofline"])
-    if form == "plain":
-        roof["traffic"] = traffic
     out = {
         "metric": "migrated shots/s + Gpt-updates/s, batched vs plain, at rel-L2<=1e-4; p99 latency",
         "value": head["shots_per_s"],

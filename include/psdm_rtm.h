@@ -184,6 +184,9 @@ int psdm_rtm_engine_poll(psdm_rtm_engine *engine, long long *ids, double *t_done
 int psdm_rtm_engine_drain(psdm_rtm_engine *engine);
 /* Device pointers of the engine's stack: PSDM_sum (ny, nx_prop) FP64, Cov_sum int32. */
 int psdm_rtm_engine_image(psdm_rtm_engine *engine, double **d_psdm_sum, int **d_cov_sum);
+/* Mean device time of one step launch (all resident shots), from events around each
+ * captured chunk of chunk_steps launches (the chunk's step-counter update included). */
+int psdm_rtm_engine_timing(psdm_rtm_engine *engine, double *mean_launch_ms, long long *n_launches);
 int psdm_rtm_engine_stats(psdm_rtm_engine *engine, long long *steps_issued, long long *slot_steps,
                           long long *hbm_used, long long *hbm_budget);
 

@@ -41,6 +41,7 @@ SIGNATURES = {
     "psdm_rtm_engine_poll": ([_P, C.POINTER(_LL), C.POINTER(C.c_double), _I, C.POINTER(_I)], _I),
     "psdm_rtm_engine_drain": ([_P], _I),
     "psdm_rtm_engine_image": ([_P, C.POINTER(_P), C.POINTER(_P)], _I),
+    "psdm_rtm_engine_timing": ([_P, C.POINTER(C.c_double), C.POINTER(_LL)], _I),
     "psdm_rtm_engine_stats": ([_P, C.POINTER(_LL), C.POINTER(_LL), C.POINTER(_LL), C.POINTER(_LL)], _I),
 }
 

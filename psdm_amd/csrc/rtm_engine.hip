@@ -128,7 +128,9 @@ struct psdm_rtm_engine {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     static constexpr int kRing = 256;
-    hipEvent_t ev_chunk[kRing] = {};
+    hipEvent_t ev_chunk[kRing] = {};   // recorded after each chunk
+    hipEvent_t ev_cstart[kRing] = {};  // recorded before each chunk (device time of the chunks)
+    long long timed_upto = 0;           // chunks whose device time is in chunk_ms_sum
     hipEvent_t ev_origin = nullptr;
     double t_origin = 0.0;
     std::vector<DoneRec> done_wait;  // retired, waiting for their stack to complete
@@ -364,6 +366,25 @@ int upload_slots(psdm_rtm_engine *e) {
 // Reports jobs whose last chunk has completed.  Entries whose chunk is at or before
 // `block_upto` are waited for (their ring event is about to be reused); the rest are
 // only queried.
+// Folds the device time of completed chunks (start/stop events around each graph of
+// chunk_steps launches) into chunk_ms_sum, waiting for the chunks up to `block_upto`.
+int harvest_chunk_times(psdm_rtm_engine *e, long long block_upto) {
+    psdm_rtm_ctx *c = e->c;
+    while (e->timed_upto < e->chunks) {
+        const long long q = e->timed_upto;
+        hipEvent_t e0 = e->ev_cstart[q % psdm_rtm_engine::kRing], e1 = e->ev_chunk[q % psdm_rtm_engine::kRing];
+        const hipError_t st = q <= block_upto ? hipEventSynchronize(e1) : hipEventQuery(e1);
+        if (st == hipErrorNotReady) break;
+        if (st != hipSuccess) return fail(c, PSDM_RTM_EDEVICE, std::string("chunk event: ") + hipGetErrorString(st));
+        float ms = 0.f;
+        HIP_TRY(c, hipEventElapsedTime(&ms, e0, e1));
+        e->chunk_ms_sum += ms;
+        e->chunk_ms_n++;
+        e->timed_upto++;
+    }
+    return 0;
+}
+
 // Moves retired shots whose image stack has completed on the device to done_ready
 // (waiting for them when `block`), with the completion time on the host steady clock.
 int collect(psdm_rtm_engine *e, bool block) {
@@ -470,6 +491,7 @@ int psdm_rtm_engine_create(psdm_rtm_ctx *c, const psdm_rtm_engine_cfg *cfg, psdm
     for (size_t s = 0; s < 2 * S; s++) e->h_slots[s].g0 = -1;
     for (int q = 0; q < 2; q++) ETRY(hipEventCreate(&e->h_ring_ev[q]));
     for (int q = 0; q < psdm_rtm_engine::kRing; q++) ETRY(hipEventCreate(&e->ev_chunk[q]));
+    for (int q = 0; q < psdm_rtm_engine::kRing; q++) ETRY(hipEventCreate(&e->ev_cstart[q]));
     ETRY(hipEventCreate(&e->ev_origin));
     e->slots.resize(S);
     e->arena.init(budget);
@@ -507,6 +529,8 @@ int psdm_rtm_engine_destroy(psdm_rtm_engine *e) {
     for (auto ev : e->h_ring_ev)
         if (ev) (void)hipEventDestroy(ev);
     for (auto ev : e->ev_chunk)
+        if (ev) (void)hipEventDestroy(ev);
+    for (auto ev : e->ev_cstart)
         if (ev) (void)hipEventDestroy(ev);
     if (e->ev_origin) (void)hipEventDestroy(e->ev_origin);
     for (auto ev : e->ev_pool) (void)hipEventDestroy(ev);
@@ -574,6 +598,9 @@ int psdm_rtm_engine_run(psdm_rtm_engine *e, int max_chunks, int *n_active, int *
             hipEvent_t old = e->ev_chunk[(e->chunks - inflight) % psdm_rtm_engine::kRing];
             HIP_TRY(c, hipEventSynchronize(old));
         }
+        // the ring entry about to be re-recorded must have its device time folded in
+        if (int rc = harvest_chunk_times(e, e->chunks - psdm_rtm_engine::kRing)) return rc;
+        HIP_TRY(c, hipEventRecord(e->ev_cstart[e->chunks % psdm_rtm_engine::kRing], c->stream));
         if (int rc = issue_chunk(e)) return rc;
         for (auto &sl : e->slots)
             if (sl.busy) e->active_slot_steps += std::min<long long>(C, sl.g_end - e->G);
@@ -617,6 +644,15 @@ int psdm_rtm_engine_image(psdm_rtm_engine *e, double **d_psdm_sum, int **d_cov_s
     if (!e) return PSDM_RTM_EINVAL;
     if (d_psdm_sum) *d_psdm_sum = e->d_sum;
     if (d_cov_sum) *d_cov_sum = e->d_fold;
+    return 0;
+}
+
+int psdm_rtm_engine_timing(psdm_rtm_engine *e, double *mean_launch_ms, long long *n_launches) {
+    if (!e) return PSDM_RTM_EINVAL;
+    if (int rc = harvest_chunk_times(e, e->chunks - 1)) return rc;
+    const long long n = e->chunk_ms_n * e->cfg.chunk_steps;
+    if (n_launches) *n_launches = n;
+    if (mean_launch_ms) *mean_launch_ms = n > 0 ? e->chunk_ms_sum / (double)n : 0.0;
     return 0;
 }
 

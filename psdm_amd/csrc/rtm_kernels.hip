@@ -158,9 +158,9 @@ __device__ __forceinline__ void stream_item(const StepArgs &a, int u) {
         if (imaging) {
             const size_t so = sbase + (size_t)iw * a.lds + srow;
             if (a.snap_fmt == kSnapF32) {
-                F4 = *gp((const v4f *)((const float *)a.snaps + so));
+                F4 = __builtin_nontemporal_load(gp((const v4f *)((const float *)a.snaps + so)));
             } else {
-                const v2u h = *gp((const v2u *)((const unsigned short *)a.snaps + so));
+                const v2u h = __builtin_nontemporal_load(gp((const v2u *)((const unsigned short *)a.snaps + so)));
                 if (a.snap_fmt == kSnapF16) {
                     F4.x = (float)__builtin_bit_cast(_Float16, (unsigned short)(h.x & 0xffffu));
                     F4.y = (float)__builtin_bit_cast(_Float16, (unsigned short)(h.x >> 16));
@@ -223,7 +223,8 @@ __device__ __forceinline__ void stream_item(const StepArgs &a, int u) {
         dv.z = v.z - p1.z;
         dv.w = v.w - p1.w;
         if (snap) {
-            // F = ((p(k) - p(k-2)) * 0.5) / dt (:485)
+            // F = ((p(k) - p(k-2)) * 0.5) / dt (:485); snapshots are written once and read
+            // once, in the backward phase, so they bypass the caches (non-temporal)
             v4f f;
             f.x = (dv.x * 0.5f) / a.dt;
             f.y = (dv.y * 0.5f) / a.dt;
@@ -233,7 +234,7 @@ __device__ __forceinline__ void stream_item(const StepArgs &a, int u) {
             if (a.snap_fmt == kSnapF32) {
                 float *q = (float *)a.snaps + so;
                 if (full) {
-                    *gp((v4f *)q) = f;
+                    __builtin_nontemporal_store(f, gp((v4f *)q));
                 } else if (any) {
                     if (ow0) gp(q)[0] = f.x;
                     if (ow1) gp(q)[1] = f.y;
@@ -260,7 +261,7 @@ __device__ __forceinline__ void stream_item(const StepArgs &a, int u) {
                 }
                 unsigned short *q = (unsigned short *)a.snaps + so;
                 if (full) {
-                    *gp((v2u *)q) = v2u{(unsigned)hs[0] | ((unsigned)hs[1] << 16), (unsigned)hs[2] | ((unsigned)hs[3] << 16)};
+                    __builtin_nontemporal_store(v2u{(unsigned)hs[0] | ((unsigned)hs[1] << 16), (unsigned)hs[2] | ((unsigned)hs[3] << 16)}, gp((v2u *)q));
                 } else if (any) {
                     if (ow0) gp(q)[0] = hs[0];
                     if (ow1) gp(q)[1] = hs[1];

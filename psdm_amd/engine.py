@@ -119,6 +119,12 @@ class Engine:
     def drain(self):
         self.ctx.check(self._L.psdm_rtm_engine_drain(self.h), "psdm_rtm_engine_drain")
 
+    def timing(self):
+        """(mean device ms of one step launch, launches timed), from the chunk events."""
+        ms, n = C.c_double(), _LL()
+        self.ctx.check(self._L.psdm_rtm_engine_timing(self.h, C.byref(ms), C.byref(n)), "psdm_rtm_engine_timing")
+        return ms.value, n.value
+
     def stats(self):
         a, b, c, d = _LL(), _LL(), _LL(), _LL()
         self.ctx.check(self._L.psdm_rtm_engine_stats(self.h, C.byref(a), C.byref(b), C.byref(c), C.byref(d)),

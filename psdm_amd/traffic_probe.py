@@ -3,7 +3,7 @@ counters per dispatch).
 
     python -m psdm_amd.traffic_probe copy    # 256 MiB device-to-device copy, 4 times
     python -m psdm_amd.traffic_probe plain   # one 601 x 715 (dPML 40) shot, 200 forward + 200 backward steps
-    python -m psdm_amd.traffic_probe batched # 16 such shots resident in the engine, 128 steps
+    python -m psdm_amd.traffic_probe batched # 16 such shots resident in the engine, forward + backward
 
 `copy` calibrates the counters: FETCH_SIZE / WRITE_SIZE of a copy of known size.
 """
@@ -55,14 +55,14 @@ def main(what: str):
         from psdm_amd.engine import Engine
 
         eng = Engine(ctx, d_mig, m.nz, m.nx, nslots=16, max_nx=sh.nx_w, max_nt=nt, dPML=W["dPML"], fs=W["fs"],
-                     dmodel=m.dx, dt=W["dt"], tile_x=16, chunk_steps=64)
+                     dmodel=m.dx, dt=W["dt"], tile_x=16, stream_w=4, chunk_steps=64)
         for _ in range(16):
             eng.submit(sh.x_i, sh.nx_w, nt, sh.nxSou - sh.x_i + 1, sh.nySou, sh.nxRec - sh.x_i + 1, sh.nyRec, src[0],
                        data)
-        eng.run(2)
+        eng.drain()
         ctx.sync()
         eng.close()
-        print("batched: 16 slots x 128 steps", flush=True)
+        print(f"batched: 16 slots x {2 * (nt - 1)} steps (forward + backward)", flush=True)
     ctx.close()
 
 
