@@ -278,7 +278,7 @@ def bench_uneven(args, mixed=False):
     log(f"[bench] uneven line: {len(shots)} shots, windows {min(s.nx_w for s in shots)}-"
         f"{max(s.nx_w for s in shots)}, {upd / 1e12:.2f} T updates, snapshot budget {args.hbm_gb} GB")
 
-    cfg = dict(nslots=args.slots, tile_x=32, grid=0, chunk_steps=64)
+    cfg = dict(nslots=args.slots, tile_x=16, stream_w=4, grid=0, chunk_steps=64)
     forms = [("f32", 1, False)] + ([("f32", 4, True), ("f16", 4, True), ("bf16", 4, True)] if mixed else [])
     results = {}
     for fmt, ss, kahan in forms:
@@ -339,7 +339,7 @@ def bench_uneven(args, mixed=False):
         "config": {"workload": f"uneven line: {len(shots)} shots on a 40 km 601-deep model @10 m, windows "
                                f"600-2000 columns, nt in {{4000,6000,8000}}, dt 0.8 ms, dPML 40",
                    "form": "batched" + (" FP16 snapshots every 4th step" if mixed else ""),
-                   "slots": args.slots, "tile_x": 32, "snapshot_budget_gb": args.hbm_gb},
+                   "slots": args.slots, "tile_x": 16, "stream_w": 4, "snapshot_budget_gb": args.hbm_gb},
         "gpt_updates_per_s": head["gpt_updates_per_s"],
         "forms": results, "plain": plain,
         "speedup_batched_vs_plain": round(base["shots_per_s"] / plain["shots_per_s"], 3) if plain else None,
@@ -402,10 +402,10 @@ def bench_served(args):
 
     def engines():
         e = {"mcs": Engine(ctx, d_vm, 601, 4001, nslots=mcs_cfg["nslots"], max_nx=mcs_cfg["max_nx"],
-                           max_nt=nt_m, dPML=40, fs=0, dmodel=dx, dt=dt_m, tile_x=32, snap_format="f32",
+                           max_nt=nt_m, dPML=40, fs=0, dmodel=dx, dt=dt_m, tile_x=16, snap_format="f32",
                            store_snap=1, hbm_budget=mcs_cfg["budget"]),
              "was": Engine(ctx, d_vw, 801, 6001, nslots=was_cfg["nslots"], max_nx=was_cfg["max_nx"], max_nt=nt_w,
-                           dPML=40, fs=0, dmodel=dx, dt=dt_w, tile_x=32, snap_format="f16", store_snap=4,
+                           dPML=40, fs=0, dmodel=dx, dt=dt_w, tile_x=16, snap_format="f16", store_snap=4,
                            kahan=True, hbm_budget=was_cfg["budget"])}
         return e
 
@@ -542,7 +542,7 @@ def main():
 
             jobs = [job_of(W, k, d_src, obs) for k in range(nshots)]
             best, trials = tune(ctx, d_mig, ny, m.nx, jobs, max(s.nx_w for s in shots), nt, dPML, W["fs"], m.dx, dt,
-                                log=log)
+                                nshots=args.steps, log=log)
             cfg.update(nslots=best["nslots"], tile_x=best["tile_x"], stream_w=best["stream_w"], grid=best["grid"])
             tuned = {"best": best, "trials": trials}
         B = run_batched(ctx, torch, dev, W, d_mig, d_src, obs, args.steps, args.warmup, cfg)
