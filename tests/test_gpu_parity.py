@@ -128,3 +128,38 @@ def test_gpu_large_window_against_oracle(ctx):
     np.testing.assert_array_equal(Rg, Ro)
     np.testing.assert_array_equal(Fg, Fo)
     np.testing.assert_array_equal(kg, ko)
+
+
+# Interior layouts the small golden cases do not reach: two and three 248-row stream tiles
+# (the last one moved up over its neighbour's rows, ny not a multiple of 4), windows
+# narrower and wider than a stream item, sources and receivers in the rows two stream
+# tiles both compute, a free surface, and stream widths 1..8.
+STREAM_CASES = {
+    "two_tiles_overlap_fs": dict(ny=301, nx=37, nt=140, dPML=8, seed=11, fs=1, src=(17, 250),
+                                 rec_x=[3, 9, 20, 33, 20, 21], rec_y=[100, 247, 248, 260, 300, 1]),
+    "three_tiles_disk": dict(ny=530, nx=23, nt=120, dPML=6, seed=12, variant=1, src=(11, 497),
+                             rec_x=np.arange(2, 23, 2), rec_y=495),
+    "narrow_window": dict(ny=259, nx=16, nt=110, dPML=9, seed=13, src=(5, 252), rec_x=[1, 16, 7], rec_y=[255, 256, 3]),
+}
+
+
+@pytest.mark.parametrize("stream_w", [1, 3, 8])
+@pytest.mark.parametrize("name", list(STREAM_CASES))
+def test_stream_tiles_bit_exact_vs_oracle(ctx, name, stream_w):
+    from tests.cases import make_case
+
+    c = make_case(**STREAM_CASES[name])
+    say(f"[gpu] stream case={name} {c['ny']}x{c['nx']} nt={c['nt']} stream_w={stream_w}")
+    ctx.set_launch(tile_x=16)
+    ctx.set_stream_width(stream_w)
+    try:
+        g = gpu_rtm(ctx, c, host_fields=True)
+    finally:
+        ctx.set_stream_width(2)
+        ctx.set_launch(tile_x=8)
+    o = oracle_rtm(c)
+    np.testing.assert_array_equal(g[0], o[0])
+    if c["variant"] == 0:
+        np.testing.assert_array_equal(g[1], o[1])
+    np.testing.assert_array_equal(g[2], o[2])
+    assert np.linalg.norm(o[2]) > 0
