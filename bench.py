@@ -163,7 +163,8 @@ def run_batched(ctx, torch, dev, W, d_mig, d_src, obs, steps, warmup, cfg):
     torch.cuda.synchronize()
     eng = Engine(ctx, d_mig, ny, m.nx, nslots=cfg["nslots"], max_nx=max(s.nx_w for s in shots), max_nt=nt,
                  dPML=W["dPML"], fs=W["fs"], dmodel=m.dx, dt=W["dt"], tile_x=cfg["tile_x"], grid=cfg["grid"],
-                 stream_w=cfg.get("stream_w", 0), snap_format=cfg.get("snap_format", "f32"),
+                 stream_w=cfg.get("stream_w", 0), groups=cfg.get("groups", 0),
+                 snap_format=cfg.get("snap_format", "f32"),
                  store_snap=cfg.get("store_snap", 1), kahan=cfg.get("kahan", False),
                  chunk_steps=cfg.get("chunk_steps", 64), d_psdm_sum=psdm_sum, d_cov_sum=fold)
     for k in range(warmup, warmup + steps):  # the shots the plain form times
@@ -231,6 +232,7 @@ def run_engine_jobs(ctx, torch, dev, d_mig, ny, nx_prop, jobs, W, cfg, budget, i
     eng = Engine(ctx, d_mig, ny, nx_prop, nslots=cfg["nslots"], max_nx=max(j["nx_w"] for j in jobs),
                  max_nt=max(j["nt"] for j in jobs), dPML=W["dPML"], fs=W["fs"], dmodel=W["model"].dx, dt=W["dt"],
                  tile_x=cfg["tile_x"], stream_w=cfg.get("stream_w", 0), grid=cfg.get("grid", 0),
+                 groups=cfg.get("groups", 0),
                  snap_format=cfg.get("snap_format", "f32"),
                  store_snap=cfg.get("store_snap", 1), kahan=cfg.get("kahan", False),
                  chunk_steps=cfg.get("chunk_steps", 64), hbm_budget=budget)
@@ -278,7 +280,7 @@ def bench_uneven(args, mixed=False):
     log(f"[bench] uneven line: {len(shots)} shots, windows {min(s.nx_w for s in shots)}-"
         f"{max(s.nx_w for s in shots)}, {upd / 1e12:.2f} T updates, snapshot budget {args.hbm_gb} GB")
 
-    cfg = dict(nslots=args.slots, tile_x=16, stream_w=4, grid=0, chunk_steps=64)
+    cfg = dict(nslots=args.slots, tile_x=16, stream_w=4, grid=0, groups=2, chunk_steps=64)
     forms = [("f32", 1, False)] + ([("f32", 4, True), ("f16", 4, True), ("bf16", 4, True)] if mixed else [])
     results = {}
     for fmt, ss, kahan in forms:
@@ -535,7 +537,7 @@ def main():
         log(f"[bench] plain {plain['shots_per_s']} shots/s")
 
     if "batched" in forms:
-        cfg = dict(nslots=16, tile_x=16, stream_w=4, grid=0, chunk_steps=64)
+        cfg = dict(nslots=16, tile_x=16, stream_w=4, grid=0, groups=2, chunk_steps=64)
         tuned = None
         if not args.no_tune:
             from psdm_amd.autotune import tune
@@ -543,11 +545,14 @@ def main():
             jobs = [job_of(W, k, d_src, obs) for k in range(nshots)]
             best, trials = tune(ctx, d_mig, ny, m.nx, jobs, max(s.nx_w for s in shots), nt, dPML, W["fs"], m.dx, dt,
                                 nshots=args.steps, log=log)
-            cfg.update(nslots=best["nslots"], tile_x=best["tile_x"], stream_w=best["stream_w"], grid=best["grid"])
+            cfg.update(nslots=best["nslots"], tile_x=best["tile_x"], stream_w=best["stream_w"], grid=best["grid"],
+                       groups=best["groups"])
             tuned = {"best": best, "trials": trials}
         B = run_batched(ctx, torch, dev, W, d_mig, d_src, obs, args.steps, args.warmup, cfg)
         st = B["stats"]
-        per_launch_ms = B["launch_ms"]  # device time per step launch (events around the captured chunks)
+        # device time per batched step (events around the captured chunks); one step is `groups`
+        # concurrent dispatches of the batch kernel, one per slot group
+        per_launch_ms = B["launch_ms"]
         bytes_launch = step_b * st["slot_steps"] / max(st["steps_issued"], 1)
         ach = bytes_launch / (per_launch_ms * 1e-3) / 1e9
         batched = {"shots_per_s": round(args.steps / B["elapsed"], 4),
@@ -556,9 +561,12 @@ def main():
                    "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                 "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "rtm_batch_kernel",
                                 "mean_kernel_ms": round(per_launch_ms, 5), "timed_launches": B["n_launch"],
+                                "concurrent_dispatches_per_launch": cfg["groups"],
                                 "algorithmic_bytes_per_launch": round(bytes_launch),
                                 "wall_ms_per_launch": round(1e3 * B["elapsed"] / max(st["steps_issued"], 1), 5),
-                                "note": "mean_kernel_ms: HIP events around each captured chunk of launches"}}
+                                "note": "one launch = one step of the resident shots, issued as concurrent_dispatches_"
+                                        "per_launch dispatches (slot groups); mean_kernel_ms: HIP events around each "
+                                        "captured chunk of launches / launches per chunk"}}
         if plain is not None:
             a = B["psdm_sum"].cpu().numpy()
             b = P["psdm_sum"].cpu().numpy()

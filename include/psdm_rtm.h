@@ -153,6 +153,8 @@ typedef struct {
     int tile_x;              /* 8, 16 or 32 columns per CPML tile */
     int stream_w;            /* interior columns per wavefront work item (0 = 4) */
     int grid;                /* persistent grid size in workgroups (0 = one per work item) */
+    int groups;              /* slots split into this many concurrent launch chains per captured
+                                chunk, so one chain's next step fills another's drain (0 = 2; 1..4) */
     int use_graph;           /* replay captured graphs (1) or launch each step (0) */
     int max_inflight;        /* chunks queued ahead of the host (>= 1) */
     long long hbm_budget;    /* bytes for snapshot stores (0 = nslots x the largest shot) */

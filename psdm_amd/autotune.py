@@ -2,7 +2,7 @@
 
 For a workload (model, shot windows, record length) it times short runs of the
 engine for each candidate (CPML tile width, interior stream-item width, grid shape,
-resident slots) and keeps the one with the lowest expected time per shot.  A trial
+concurrent slot groups, resident slots) and keeps the one with the lowest expected time per shot.  A trial
 admits `nslots` shots with their record length cut to a few chunks (the cost of a step
 does not depend on the record length), lets one chunk run (admission and first-step
 effects), then times the following chunks, which cover forward and backward steps.
@@ -39,23 +39,24 @@ def _trial(ctx, d_vel, ny, nx_prop, jobs, nslots, max_nx, nt, dPML, fs, dx, dt, 
 
 
 def tune(ctx, d_vel, ny, nx_prop, jobs, max_nx, nt, dPML, fs, dx, dt, snap_format="f32", store_snap=1, kahan=False,
-         slots=(16, 24, 32), tiles=(16, 32), stream_ws=(4, 8), grids=(0,), chunk_steps=32, chunks=5, nshots=None,
-         log=None):
+         slots=(16, 24, 32), tiles=(16, 32), stream_ws=(4, 8), grids=(0,), groups=(2,), chunk_steps=32, chunks=5,
+         nshots=None, log=None):
     """jobs: list of Engine.submit keyword dicts.  Returns (best config dict, trials)."""
     trials = []
-    for nslots, tile_x, sw, grid in itertools.product(slots, tiles, stream_ws, grids):
-        cfg = dict(tile_x=tile_x, stream_w=sw, grid=grid, snap_format=snap_format, store_snap=store_snap, kahan=kahan)
+    for nslots, tile_x, sw, grid, ng in itertools.product(slots, tiles, stream_ws, grids, groups):
+        cfg = dict(tile_x=tile_x, stream_w=sw, grid=grid, groups=ng, snap_format=snap_format, store_snap=store_snap,
+                   kahan=kahan)
         try:
             t = _trial(ctx, d_vel, ny, nx_prop, jobs, nslots, max_nx, nt, dPML, fs, dx, dt, cfg, chunk_steps, chunks)
         except Exception as exc:  # e.g. a slot count whose snapshot stores exceed HBM
             if log:
-                log(f"[autotune] slots={nslots} tile_x={tile_x} stream_w={sw} grid={grid}: skipped ({exc})")
+                log(f"[autotune] slots={nslots} tile_x={tile_x} stream_w={sw} grid={grid} groups={ng}: skipped ({exc})")
             continue
         fill = nslots * -(-nshots // nslots) / nshots if nshots else 1.0
-        trials.append(dict(nslots=nslots, tile_x=tile_x, stream_w=sw, grid=grid, us_per_slot_step=1e6 * t,
+        trials.append(dict(nslots=nslots, tile_x=tile_x, stream_w=sw, grid=grid, groups=ng, us_per_slot_step=1e6 * t,
                            score=1e6 * t * fill))
         if log:
-            log(f"[autotune] slots={nslots} tile_x={tile_x} stream_w={sw} grid={grid}: {1e6 * t:.3f} us per slot-step"
+            log(f"[autotune] slots={nslots} tile_x={tile_x} stream_w={sw} grid={grid} groups={ng}: {1e6 * t:.3f} us per slot-step"
                 f" (x{fill:.3f} last-round fill)")
     best = min(trials, key=lambda r: r["score"])
     return best, trials

@@ -35,6 +35,11 @@ int fail(psdm_rtm_ctx *c, int code, const std::string &msg) {
     return code;
 }
 
+int hip_fail(psdm_rtm_ctx *c, hipError_t e, const char *expr) {
+    (void)hipGetLastError();
+    return fail(c, PSDM_RTM_EDEVICE, std::string(expr) + ": " + hipGetErrorString(e));
+}
+
 int get_buf(psdm_rtm_ctx *c, const char *name, size_t bytes, void **out) {
     DevBuf &b = c->bufs[name];
     if (b.n < bytes) {

@@ -131,8 +131,8 @@ __host__ __device__ inline int stream_item_of_cell(const Tiling &T, int jw, int 
 struct StepArgs {
     int ny, nx, dn, nyt, nxt, ld;
     int nitems, per_xcd;  // workgroups of one step (Tiling::nitems); per XCD
-    int rt_pml, rt_int, ct_pml, ct_int, nct;
-    int ncpml_a, ncpml, ri, cs, sw, nstream;
+    int rt_pml, rt_int, ct_pml, ct_int;
+    int ncpml_a, ncpml, ri, sw, nstream;
     FastDiv fd_2rp, fd_rint, fd_ri, fd_rpml, fd_cpml;  // division by 2*rt_pml, rt_int, ri, rt_pml, ct_pml
     int fs, ram;      // free surface flag; ram = 1 -> _ram semantics, 0 -> _disk
     int k;            // time step index of the reference loop, 2..nt
@@ -179,11 +179,9 @@ inline void set_tiling(StepArgs &a, const Tiling &T) {
     a.rt_int = T.rt_int;
     a.ct_pml = T.ct_pml;
     a.ct_int = T.ct_int;
-    a.nct = T.nct;
     a.ncpml_a = T.ncpml_a;
     a.ncpml = T.ncpml;
     a.ri = T.ri;
-    a.cs = T.cs;
     a.sw = T.sw;
     a.nstream = T.nstream;
     a.fd_2rp = make_fastdiv(2 * T.rt_pml);

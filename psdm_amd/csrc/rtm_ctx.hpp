@@ -48,6 +48,9 @@ namespace psdm {
 namespace host {
 
 int fail(psdm_rtm_ctx *c, int code, const std::string &msg);
+// Failed HIP call: records it and clears the thread's last-error state, so that a caller
+// checking hipGetLastError after its own next launch does not see the library's error.
+int hip_fail(psdm_rtm_ctx *c, hipError_t e, const char *expr);
 // Grow-only named device buffer of the context.
 int get_buf(psdm_rtm_ctx *c, const char *name, size_t bytes, void **out);
 int set_device(psdm_rtm_ctx *c);
@@ -61,5 +64,5 @@ void pml_profile(int dPML, double dt, double d, std::vector<float> &A, std::vect
     do {                                                                                                     \
         hipError_t e_ = (expr);                                                                              \
         if (e_ != hipSuccess)                                                                                \
-            return psdm::host::fail(ctx, PSDM_RTM_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+            return psdm::host::hip_fail(ctx, e_, #expr);                                                     \
     } while (0)

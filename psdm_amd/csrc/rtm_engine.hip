@@ -91,6 +91,8 @@ struct Slot {
     size_t n_inj = 0;
 };
 
+constexpr int kMaxGroups = 4;  // concurrent launch chains of one captured chunk
+
 struct DoneRec {
     long long id;
     hipEvent_t ev;  // recorded after the shot's image is stacked
@@ -107,6 +109,11 @@ struct psdm_rtm_engine {
     psdm_rtm_engine_cfg cfg{};
     int dn = 0, ld = 0, lds = 0, off = 0, sw = 16, nyt = 0, nxt_max = 0, cap_tiles = 0, nwork = 0, per_xcd = 0, grid = 0;
     size_t win = 0;  // per-slot length of the c2 / image / comp buffers
+    // slots split into `groups` independent launch chains inside a captured chunk, so one
+    // group's next step can start while the other's drains
+    int groups = 1;
+    hipStream_t side[kMaxGroups - 1] = {};
+    hipEvent_t ev_fork = nullptr, ev_join[kMaxGroups - 1] = {};
     size_t n_p = 0, n_x = 0, n_y = 0, state_stride = 0, esize = 4;
     // device
     SlotDesc *d_slots = nullptr;
@@ -146,16 +153,27 @@ namespace {
 int efail(psdm_rtm_engine *e, int code, const std::string &m) { return fail(e->c, code, m); }
 
 // Launches one chunk of time steps (graph replay or eager launches).
+// Launch arguments of slot group q (slots [q*n0, min((q+1)*n0, nslots)), n0 = ceil(nslots/groups)).
+BatchArgs group_args(const psdm_rtm_engine *e, int q, int goff, int *grid) {
+    const int n0 = cdiv(e->cfg.nslots, e->groups);
+    const int s0 = q * n0, n = std::min(n0, e->cfg.nslots - s0);
+    const int nwork = n * e->cap_tiles, per_xcd = cdiv(nwork, 8);
+    *grid = e->cfg.grid > 0 ? std::min(e->cfg.grid, per_xcd * 8) : per_xcd * 8;
+    return BatchArgs{e->d_slots + s0, n, e->cap_tiles, make_fastdiv(e->cap_tiles), nwork, per_xcd, e->d_gbase, goff};
+}
+
 int issue_chunk(psdm_rtm_engine *e) {
     psdm_rtm_ctx *c = e->c;
     const int C = e->cfg.chunk_steps;
     if (e->exec) {
         HIP_TRY(c, hipGraphLaunch(e->exec, c->stream));
     } else {
-        for (int g = 0; g < C; g++) {
-            BatchArgs b{e->d_slots, e->cfg.nslots, e->cap_tiles, make_fastdiv(e->cap_tiles), e->nwork, e->per_xcd, e->d_gbase, g};
-            launch_batch(b, e->cfg.tile_x, e->grid, c->stream);
-        }
+        for (int g = 0; g < C; g++)
+            for (int q = 0; q < e->groups; q++) {
+                int grid;
+                const BatchArgs b = group_args(e, q, g, &grid);
+                launch_batch(b, e->cfg.tile_x, grid, c->stream);
+            }
         launch_advance(e->d_gbase, C, c->stream);
     }
     HIP_TRY(c, hipGetLastError());
@@ -165,9 +183,19 @@ int issue_chunk(psdm_rtm_engine *e) {
 int capture_graph(psdm_rtm_engine *e) {
     psdm_rtm_ctx *c = e->c;
     HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
-    for (int g = 0; g < e->cfg.chunk_steps; g++) {
-        BatchArgs b{e->d_slots, e->cfg.nslots, e->cap_tiles, make_fastdiv(e->cap_tiles), e->nwork, e->per_xcd, e->d_gbase, g};
-        launch_batch(b, e->cfg.tile_x, e->grid, c->stream);
+    if (e->groups > 1) {  // the other chains fork off the capture stream and join before the advance
+        HIP_TRY(c, hipEventRecord(e->ev_fork, c->stream));
+        for (int q = 1; q < e->groups; q++) HIP_TRY(c, hipStreamWaitEvent(e->side[q - 1], e->ev_fork, 0));
+    }
+    for (int g = 0; g < e->cfg.chunk_steps; g++)
+        for (int q = 0; q < e->groups; q++) {
+            int grid;
+            const BatchArgs b = group_args(e, q, g, &grid);
+            launch_batch(b, e->cfg.tile_x, grid, q == 0 ? c->stream : e->side[q - 1]);
+        }
+    for (int q = 1; q < e->groups; q++) {
+        HIP_TRY(c, hipEventRecord(e->ev_join[q - 1], e->side[q - 1]));
+        HIP_TRY(c, hipStreamWaitEvent(c->stream, e->ev_join[q - 1], 0));
     }
     launch_advance(e->d_gbase, e->cfg.chunk_steps, c->stream);
     HIP_TRY(c, hipStreamEndCapture(c->stream, &e->graph));
@@ -374,8 +402,11 @@ int harvest_chunk_times(psdm_rtm_engine *e, long long block_upto) {
         const long long q = e->timed_upto;
         hipEvent_t e0 = e->ev_cstart[q % psdm_rtm_engine::kRing], e1 = e->ev_chunk[q % psdm_rtm_engine::kRing];
         const hipError_t st = q <= block_upto ? hipEventSynchronize(e1) : hipEventQuery(e1);
-        if (st == hipErrorNotReady) break;
-        if (st != hipSuccess) return fail(c, PSDM_RTM_EDEVICE, std::string("chunk event: ") + hipGetErrorString(st));
+        if (st == hipErrorNotReady) {
+            (void)hipGetLastError();  // a pending event is not an error of the caller's
+            break;
+        }
+        if (st != hipSuccess) return hip_fail(c, st, "chunk event");
         float ms = 0.f;
         HIP_TRY(c, hipEventElapsedTime(&ms, e0, e1));
         e->chunk_ms_sum += ms;
@@ -398,9 +429,10 @@ int collect(psdm_rtm_engine *e, bool block) {
             e->done_ready.emplace_back(d.id, e->t_origin + 1e-3 * ms);
             e->ev_pool.push_back(d.ev);
         } else if (q == hipErrorNotReady) {
+            (void)hipGetLastError();
             keep.push_back(d);
         } else {
-            return fail(c, PSDM_RTM_EDEVICE, std::string("chunk event: ") + hipGetErrorString(q));
+            return hip_fail(c, q, "shot completion event");
         }
     }
     e->done_wait.swap(keep);
@@ -425,6 +457,7 @@ int psdm_rtm_engine_create(psdm_rtm_ctx *c, const psdm_rtm_engine_cfg *cfg, psdm
     if (cfg->tile_x != 8 && cfg->tile_x != 16 && cfg->tile_x != 32)
         return fail(c, PSDM_RTM_EINVAL, "tile_x must be 8, 16 or 32");
     if (cfg->stream_w < 0 || cfg->stream_w > 256) return fail(c, PSDM_RTM_EINVAL, "stream_w must be in 0..256");
+    if (cfg->groups < 0 || cfg->groups > kMaxGroups) return fail(c, PSDM_RTM_EINVAL, "groups must be in 0..4");
     if (cfg->chunk_steps < 1 || cfg->chunk_steps > 4096) return fail(c, PSDM_RTM_EINVAL, "chunk_steps out of range");
     if (!cfg->d_vel) return fail(c, PSDM_RTM_EINVAL, "null velocity model");
     if (!(cfg->dt > 0.f) || !(cfg->dmodel > 0.f)) return fail(c, PSDM_RTM_EINVAL, "dt and dmodel must be > 0");
@@ -439,6 +472,9 @@ int psdm_rtm_engine_create(psdm_rtm_ctx *c, const psdm_rtm_engine_cfg *cfg, psdm
     e->sw = cfg->stream_w > 0 ? cfg->stream_w : 4;
     e->nxt_max = cfg->max_nx + 2 * e->dn;
     e->cap_tiles = make_tiling(cfg->ny, cfg->max_nx, e->dn, cfg->tile_x, e->sw).nitems;
+    e->groups = cfg->groups > 0 ? cfg->groups : 2;
+    if (!cfg->use_graph) e->groups = 1;
+    e->groups = cdiv(cfg->nslots, cdiv(cfg->nslots, e->groups));  // every group holds at least one slot
     e->win = (size_t)e->lds * cfg->max_nx + 64;
     e->nwork = cfg->nslots * e->cap_tiles;
     e->per_xcd = (e->nwork + 7) / 8;
@@ -459,8 +495,13 @@ int psdm_rtm_engine_create(psdm_rtm_ctx *c, const psdm_rtm_engine_cfg *cfg, psdm
     do {                                                                                          \
         hipError_t e_ = (expr);                                                                   \
         if (e_ != hipSuccess)                                                                     \
-            return bail(fail(c, PSDM_RTM_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_))); \
+            return bail(hip_fail(c, e_, #expr));                                                  \
     } while (0)
+    if (e->groups > 1) ETRY(hipEventCreateWithFlags(&e->ev_fork, hipEventDisableTiming));
+    for (int q = 1; q < e->groups; q++) {
+        ETRY(hipStreamCreateWithFlags(&e->side[q - 1], hipStreamNonBlocking));
+        ETRY(hipEventCreateWithFlags(&e->ev_join[q - 1], hipEventDisableTiming));
+    }
     ETRY(hipMalloc(&e->d_slots, S * sizeof(SlotDesc)));
     ETRY(hipMalloc(&e->d_gbase, sizeof(long long)));
     ETRY(hipMalloc(&e->d_tabs, 2 * e->dn * sizeof(float)));
@@ -534,6 +575,11 @@ int psdm_rtm_engine_destroy(psdm_rtm_engine *e) {
         if (ev) (void)hipEventDestroy(ev);
     if (e->ev_origin) (void)hipEventDestroy(e->ev_origin);
     for (auto ev : e->ev_pool) (void)hipEventDestroy(ev);
+    if (e->ev_fork) (void)hipEventDestroy(e->ev_fork);
+    for (int q = 0; q < kMaxGroups - 1; q++) {
+        if (e->ev_join[q]) (void)hipEventDestroy(e->ev_join[q]);
+        if (e->side[q]) (void)hipStreamDestroy(e->side[q]);
+    }
     for (auto &d : e->done_wait) (void)hipEventDestroy(d.ev);
     delete e;
     return 0;

@@ -26,7 +26,7 @@ class EngineCfg(C.Structure):
         ("nslots", _I), ("ny", _I), ("nx_prop", _I), ("d_vel", _P), ("max_nx", _I), ("max_nt", _I),
         ("dPML", _I), ("fs", _I), ("variant", _I), ("dmodel", _F), ("dt", _F), ("store_snap", _I),
         ("snap_format", _I), ("kahan", _I), ("chunk_steps", _I), ("tile_x", _I), ("stream_w", _I), ("grid", _I),
-        ("use_graph", _I),
+        ("groups", _I), ("use_graph", _I),
         ("max_inflight", _I), ("hbm_budget", _LL), ("d_psdm_sum", _P), ("d_cov_sum", _P),
     ]
 
@@ -52,7 +52,7 @@ class Engine:
     def __init__(self, ctx: Context, d_vel, ny: int, nx_prop: int, nslots: int, max_nx: int, max_nt: int,
                  dPML: int, fs: int, dmodel: float, dt: float, variant: int = 0, store_snap: int = 1,
                  snap_format: str = "f32", kahan: bool = False, chunk_steps: int = 64, tile_x: int = 32,
-                 stream_w: int = 0, grid: int = 0, use_graph: bool = True, max_inflight: int = 2, hbm_budget: int = 0,
+                 stream_w: int = 0, grid: int = 0, groups: int = 0, use_graph: bool = True, max_inflight: int = 2, hbm_budget: int = 0,
                  d_psdm_sum=None, d_cov_sum=None):
         self.ctx = ctx
         self._L = ctx._L
@@ -63,7 +63,7 @@ class Engine:
         cfg.dmodel, cfg.dt, cfg.store_snap = dmodel, dt, store_snap
         cfg.snap_format, cfg.kahan = SNAP_FORMATS[snap_format], int(kahan)
         cfg.chunk_steps, cfg.tile_x, cfg.grid, cfg.use_graph = chunk_steps, tile_x, grid, int(use_graph)
-        cfg.stream_w = stream_w
+        cfg.stream_w, cfg.groups = stream_w, groups
         cfg.max_inflight, cfg.hbm_budget = max_inflight, hbm_budget
         cfg.d_psdm_sum = d_psdm_sum.data_ptr() if d_psdm_sum is not None else None
         cfg.d_cov_sum = d_cov_sum.data_ptr() if d_cov_sum is not None else None

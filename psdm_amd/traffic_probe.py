@@ -3,7 +3,8 @@ counters per dispatch).
 
     python -m psdm_amd.traffic_probe copy    # 256 MiB device-to-device copy, 4 times
     python -m psdm_amd.traffic_probe plain   # one 601 x 715 (dPML 40) shot, 200 forward + 200 backward steps
-    python -m psdm_amd.traffic_probe batched # 16 such shots resident in the engine, forward + backward
+    python -m psdm_amd.traffic_probe batched # 16 such shots resident in the engine (two slot groups of 8,
+                                             # two concurrent dispatches per step), forward + backward
 
 `copy` calibrates the counters: FETCH_SIZE / WRITE_SIZE of a copy of known size.
 """
@@ -55,7 +56,7 @@ def main(what: str):
         from psdm_amd.engine import Engine
 
         eng = Engine(ctx, d_mig, m.nz, m.nx, nslots=16, max_nx=sh.nx_w, max_nt=nt, dPML=W["dPML"], fs=W["fs"],
-                     dmodel=m.dx, dt=W["dt"], tile_x=16, stream_w=4, chunk_steps=64)
+                     dmodel=m.dx, dt=W["dt"], tile_x=16, stream_w=4, groups=2, chunk_steps=64)
         for _ in range(16):
             eng.submit(sh.x_i, sh.nx_w, nt, sh.nxSou - sh.x_i + 1, sh.nySou, sh.nxRec - sh.x_i + 1, sh.nyRec, src[0],
                        data)

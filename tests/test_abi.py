@@ -54,6 +54,24 @@ def test_python_binding_covers_every_symbol():
         getattr(L, s)
 
 
+def _struct_fields(name):
+    """Member names, in order, of `typedef struct { ... } name;` in the header."""
+    text = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
+    body = re.search(r"typedef struct \{([^}]*)\}\s*" + name + r"\s*;", text).group(1)
+    names = []
+    for decl in body.split(";"):
+        decl = re.sub(r"^\s*(const\s+)?(long\s+long|\w+)", "", decl)  # drop the type
+        names += [d.strip(" *\n") for d in decl.split(",") if d.strip()]
+    return names
+
+
+def test_python_structs_mirror_the_header():
+    from psdm_amd.engine import EngineCfg, Job
+
+    assert [f[0] for f in EngineCfg._fields_] == _struct_fields("psdm_rtm_engine_cfg")
+    assert [f[0] for f in Job._fields_] == _struct_fields("psdm_rtm_job")
+
+
 @pytest.mark.skipif(gpu_available(), reason="checks the no-device error path")
 def test_context_without_device_fails_loudly():
     from psdm_amd._native import Context, RtmError

@@ -147,6 +147,19 @@ def test_engine_graph_equals_eager_and_grid_shapes(ctx):
         np.testing.assert_array_equal(a[q], d[q])
 
 
+def test_engine_launch_chains_bit_identical(ctx):
+    """Slots split into 1, 2 or 3 concurrent launch chains per captured chunk (5 slots:
+    uneven 2+2+1 split with 3) give the same images and stack, bit for bit."""
+    vel = model(5)
+    say("[gpu] engine launch chains 1 / 2 / 3")
+    runs = [run_engine(ctx, vel, SHOTS, range(5), nslots=5, groups=g, chunk_steps=16) for g in (1, 2, 3)]
+    for imgs, (psum, fold), _ in runs[1:]:
+        for q in range(5):
+            np.testing.assert_array_equal(imgs[q], runs[0][0][q])
+        np.testing.assert_array_equal(psum, runs[0][1][0])
+        np.testing.assert_array_equal(fold, runs[0][1][1])
+
+
 def test_engine_hbm_budget_admission(ctx):
     """A budget that holds one FP32 snapshot store at a time runs the shots one after
     the other in a 3-slot engine, with the same results."""
@@ -180,6 +193,24 @@ def test_engine_rejects_oversized_and_out_of_model(ctx):
     with pytest.raises(RtmError, match="outside the velocity model"):
         eng.submit(140, 40, 200, 5, 3, [3, 6, 9], [2, 2, 2], w, d)
     eng.close()
+
+
+def test_engine_out_of_memory_leaves_no_stale_error(ctx):
+    """A snapshot arena larger than the device fails engine creation with an error, and
+    the caller's next launch (torch checks the thread's last HIP error after it) works."""
+    import torch
+
+    from psdm_amd._native import RtmError
+    from psdm_amd.engine import Engine
+
+    dev = torch.device("cuda:0")
+    d_vel = torch.from_numpy(np.ascontiguousarray(model(0).T)).to(dev)
+    with pytest.raises(RtmError, match="out of memory"):
+        Engine(ctx, d_vel, NY, NXP, nslots=1, max_nx=60, max_nt=200, dPML=DPML, fs=0, dmodel=DX, dt=DT,
+               hbm_budget=1 << 44)
+    x = torch.zeros((NXP, NY), dtype=torch.float64, device=dev) + 1.0
+    torch.cuda.synchronize()
+    assert float(x.sum()) == NXP * NY
 
 
 @pytest.mark.parametrize("fmt,bar", [("f32", 1e-5), ("f16", 2e-4), ("bf16", 2e-3)])
