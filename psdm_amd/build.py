@@ -19,10 +19,12 @@ SOURCES = ["rtm_kernels.hip", "rtm_api.hip", "rtm_engine.hip"]
 # -ffp-contract=off: no fused multiply-add, so each product and sum rounds separately
 #   in the reference's order (the FP32 parity contract).
 # -packed-fp32-ops off: plain scalar FP32 VALU ops only.
+# -Werror=shorten-64-to-32: a 64-bit value (an address) silently narrowed to 32 bits is a
+#   build error (the lane intrinsics, e.g. readfirstlane, take and return int).
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    "-Wall", "-Wno-unused-function", "-Wno-unused-result",
+    "-Wall", "-Wno-unused-function", "-Wno-unused-result", "-Werror=shorten-64-to-32",
 ]
 
 

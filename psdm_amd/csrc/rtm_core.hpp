@@ -19,6 +19,7 @@ constexpr int kTileY = 64;        // CPML tiles: rows per tile, one row per lane
 constexpr int kBlock = 256;       // 4 wavefronts per workgroup
 constexpr int kMaxTables = 1024;  // max CPML cells per side (dPML + 6)
 constexpr int kStreamRows = 248;  // interior stream items: rows owned per wavefront
+constexpr int kStageCols = 4;     // FP32 snapshot columns a stream item stages in LDS
 constexpr int kIntPad = 4;        // front rows of padding of the interior arrays
 
 // Snapshot storage formats.  FP16/BF16 snapshots carry one power-of-two scale per
@@ -26,6 +27,10 @@ constexpr int kIntPad = 4;        // front rows of padding of the interior array
 // to 2^14 (FP16) or 1 (BF16), so the only error is the 16-bit rounding of the scaled
 // value.
 enum SnapFmt { kSnapF32 = 0, kSnapF16 = 1, kSnapBF16 = 2 };
+// Snapshot / imaging variant a step kernel is compiled for (the interior carries only the
+// registers its variant needs): FP32 or 16-bit snapshots, plain or Kahan imaging.
+enum SnapVariant { kSvF32 = 0, kSvF32Kahan = 1, kSv16 = 2, kSv16Kahan = 3 };
+constexpr int snap_variant(int snap_fmt, int kahan) { return (snap_fmt != kSnapF32 ? 2 : 0) + (kahan ? 1 : 0); }
 
 // Everything one time step of one shot needs.  Layout in HBM:
 //   wavefields  p[i*ld + j], i in [0,nxt), j in [0,nyt): the pointer is offset into its
@@ -220,7 +225,8 @@ struct BatchArgs {
 };
 
 void launch_step(const StepArgs &a, int tile_x, hipStream_t s, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
-void launch_batch(const BatchArgs &b, int tile_x, int grid, hipStream_t s, hipEvent_t e0 = nullptr,
+// sv: the SnapVariant of every slot (the engine's snapshot format and Kahan setting)
+void launch_batch(const BatchArgs &b, int tile_x, int sv, int grid, hipStream_t s, hipEvent_t e0 = nullptr,
                   hipEvent_t e1 = nullptr);
 void launch_advance(long long *gbase, long long by, hipStream_t s);
 void launch_record(const float *p, const int *rec_cell, float *rec, int rec_ld, int row, int nrec, hipStream_t s);

@@ -172,7 +172,7 @@ int issue_chunk(psdm_rtm_engine *e) {
             for (int q = 0; q < e->groups; q++) {
                 int grid;
                 const BatchArgs b = group_args(e, q, g, &grid);
-                launch_batch(b, e->cfg.tile_x, grid, c->stream);
+                launch_batch(b, e->cfg.tile_x, snap_variant(e->cfg.snap_format, e->cfg.kahan), grid, c->stream);
             }
         launch_advance(e->d_gbase, C, c->stream);
     }
@@ -191,7 +191,8 @@ int capture_graph(psdm_rtm_engine *e) {
         for (int q = 0; q < e->groups; q++) {
             int grid;
             const BatchArgs b = group_args(e, q, g, &grid);
-            launch_batch(b, e->cfg.tile_x, grid, q == 0 ? c->stream : e->side[q - 1]);
+            launch_batch(b, e->cfg.tile_x, snap_variant(e->cfg.snap_format, e->cfg.kahan), grid,
+                         q == 0 ? c->stream : e->side[q - 1]);
         }
     for (int q = 1; q < e->groups; q++) {
         HIP_TRY(c, hipEventRecord(e->ev_join[q - 1], e->side[q - 1]));

@@ -107,7 +107,22 @@ __device__ __forceinline__ float d2(float p, float a1, float b1, float a2, float
 // update (:443-444, source injection :348-358), free surface (:463-466), then the
 // snapshot (forward, :476-500) or the imaging term (backward,
 // solver_ac_backward_ram.f90:455-481).  Only the rows the item owns are stored.
-__device__ __forceinline__ void stream_item(const StepArgs &a, int u) {
+// Lane address = wave-uniform base + this lane's 32-bit byte offset (global loads and
+// stores then take the base in scalar registers and one offset register).
+// (__builtin_amdgcn_readfirstlane is int -> int: the 64-bit base is read as two halves)
+template <class T>
+__device__ __forceinline__ gptr<T> lanep(const void *base, unsigned off) {
+    const unsigned long long v = (unsigned long long)base;
+    const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)v);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(v >> 32));
+    return gp((T *)((const char *)(((unsigned long long)hi << 32) | lo) + off));
+}
+__device__ __forceinline__ v4f ldv(const float *base, unsigned off) { return *lanep<const v4f>(base, off); }
+
+template <int SV>
+__device__ __forceinline__ void stream_item(const StepArgs &a, int u, v4f *stage) {
+    constexpr bool kF32 = SV == kSvF32 || SV == kSvF32Kahan;  // else 16-bit (FP16 or BF16)
+    constexpr bool kKahan = SV == kSvF32Kahan || SV == kSv16Kahan;
     const int lane = threadIdx.x & 63;
     const int dn = a.dn;
     const int CS = fdiv(u, a.fd_ri), R = u - CS * a.ri;
@@ -118,49 +133,77 @@ __device__ __forceinline__ void stream_item(const StepArgs &a, int u) {
     const bool ow0 = jl >= own0 && jl < own1, ow1 = jl + 1 >= own0 && jl + 1 < own1;
     const bool ow2 = jl + 2 >= own0 && jl + 2 < own1, ow3 = jl + 3 >= own0 && jl + 3 < own1;
     const bool full = ow0 && ow3, any = ow0 || ow1 || ow2 || ow3;
-    const int ld4 = a.ld >> 2, lc4 = a.ldc >> 2;
+    // Every access is a wave-uniform column base (scalar registers) plus this lane's
+    // 32-bit byte offset, so addresses cost one vector register for all arrays.
+    const unsigned lo = 16u * (unsigned)lane, lo2 = 8u * (unsigned)lane;
     const int prow = dn + o - 4;  // padded row of lane 0's first value
-    gptr<const v4f> P = gp((const v4f *)(a.pc + prow)) + lane;
-    gptr<v4f> PN = gp((v4f *)(a.pn + prow)) + lane;
-    gptr<const v4f> C2 = gp((const v4f *)(a.c2 + (o - 4))) + lane;
+    const float *P = a.pc + prow;
+    float *PN = a.pn + prow;
+    const float *C2 = a.c2 + (o - 4);
+    const size_t ld = a.ld;
     const v4f z4 = v4f{0.f, 0.f, 0.f, 0.f};
 
-    // injection events of this item, ordered by column then row: cell = iw << 16 | jw
+    // injection events of this item, ordered by column then row: cell = iw << 16 | jw.
+    // The first 64 are loaded one per lane (cell and value) together with the window, so
+    // the column loop reads them with readlane instead of dependent scalar loads.
     int e = gp(a.ev_ptr)[u];
     const int ev_e = gp(a.ev_ptr)[u + 1];
     gptr<const float> inj = gp(a.inj_val) + (size_t)(a.k - 2) * a.ninj;
+    const int e_first = e, e_reg = ev_e < e + 64 ? ev_e : e + 64;
+    int ev_c = 0;
+    float ev_v = 0.f;
+    if (e + lane < e_reg) {
+        ev_c = gp(a.ev_cell)[e + lane];
+        ev_v = inj[gp(a.ev_inj)[e + lane]];
+    }
 
     const bool imaging = a.mode == 1 && a.snap_slot >= 0;
     const bool snap = a.mode == 0 && a.snap_slot >= 0;
     const size_t sbase = (size_t)(a.snap_slot < 0 ? 0 : a.snap_slot) * a.snap_stride;
     const size_t scbase = (size_t)(a.snap_slot < 0 ? 0 : a.snap_slot) * a.scale_stride + (size_t)R * a.nx;
-    const int srow = (o - 4) + 4 * lane;  // interior row of this lane's first value (array index)
+    const float *SN = (const float *)a.snaps + sbase + (o - 4);  // FP32 snapshot, lane 0's first row
+    const unsigned short *SH = (const unsigned short *)a.snaps + sbase + (o - 4);
+    float *IM = a.image + (o - 4), *CM = a.image_comp + (o - 4);
+
+    // FP32 imaging with at most kStageCols columns: the item's snapshot columns are copied
+    // straight into this wavefront's LDS slice (global_load_lds, no registers held) at the
+    // start, so their HBM latency is paid once per item instead of once per column.  The
+    // compiler does not order these copies against LDS reads; the column loop waits for
+    // them (vmcnt 0) before its first read.
+    const bool staged = kF32 && imaging && cend - c0 <= kStageCols;
+    if (staged)
+        for (int q = 0; q < cend - c0; ++q)
+            __builtin_amdgcn_global_load_lds((const void *)lanep<const v4f>(SN + (size_t)(c0 - dn + q) * a.lds, lo),
+                                             (__attribute__((address_space(3))) void *)(stage + q * 64), 16, 0, 0);
 
     v4f w0, w1, w2, w3, w4, w5, w6;
     if (a.zc) {
         w0 = w1 = w2 = w3 = w4 = w5 = w6 = z4;
     } else {
-        w0 = P[(c0 - 3) * ld4];
-        w1 = P[(c0 - 2) * ld4];
-        w2 = P[(c0 - 1) * ld4];
-        w3 = P[c0 * ld4];
-        w4 = P[(c0 + 1) * ld4];
-        w5 = P[(c0 + 2) * ld4];
-        w6 = P[(c0 + 3) * ld4];
+        w0 = ldv(P + (c0 - 3) * ld, lo);
+        w1 = ldv(P + (c0 - 2) * ld, lo);
+        w2 = ldv(P + (c0 - 1) * ld, lo);
+        w3 = ldv(P + c0 * ld, lo);
+        w4 = ldv(P + (c0 + 1) * ld, lo);
+        w5 = ldv(P + (c0 + 2) * ld, lo);
+        w6 = ldv(P + (c0 + 3) * ld, lo);
     }
     for (int i = c0; i < cend; ++i) {
         const int iw = i - dn;
-        const v4f p1 = a.zp ? z4 : PN[i * ld4];
-        const v4f cc = C2[iw * lc4];
-        const v4f nw = a.zc ? z4 : P[(i + 4) * ld4];
+        const v4f p1 = a.zp ? z4 : ldv(PN + i * ld, lo);
+        const v4f cc = ldv(C2 + (size_t)iw * a.ldc, lo);
+        const v4f nw = a.zc ? z4 : ldv(P + (i + 4) * ld, lo);
+        const size_t co = (size_t)iw * a.lds;  // column offset in the interior arrays
         v4f F4 = z4, im4 = z4, cm4 = z4;
         float scale = 1.f;
         if (imaging) {
-            const size_t so = sbase + (size_t)iw * a.lds + srow;
-            if (a.snap_fmt == kSnapF32) {
-                F4 = __builtin_nontemporal_load(gp((const v4f *)((const float *)a.snaps + so)));
+            if (staged) {
+                if (i == c0) __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
+                F4 = stage[(i - c0) * 64 + lane];
+            } else if (kF32) {
+                F4 = __builtin_nontemporal_load(lanep<const v4f>(SN + co, lo));
             } else {
-                const v2u h = __builtin_nontemporal_load(gp((const v2u *)((const unsigned short *)a.snaps + so)));
+                const v2u h = __builtin_nontemporal_load(lanep<const v2u>(SH + co, lo2));
                 if (a.snap_fmt == kSnapF16) {
                     F4.x = (float)__builtin_bit_cast(_Float16, (unsigned short)(h.x & 0xffffu));
                     F4.y = (float)__builtin_bit_cast(_Float16, (unsigned short)(h.x >> 16));
@@ -174,8 +217,8 @@ __device__ __forceinline__ void stream_item(const StepArgs &a, int u) {
                 }
                 scale = gp(a.snap_scale)[scbase + iw];
             }
-            im4 = *gp((const v4f *)(a.image + (size_t)iw * a.lds + srow));
-            if (a.kahan) cm4 = *gp((const v4f *)(a.image_comp + (size_t)iw * a.lds + srow));
+            im4 = ldv(IM + co, lo);
+            if (kKahan) cm4 = ldv(CM + co, lo);
         }
         // y neighbours: the previous lane's rows 1..3 and the next lane's rows 0..2
         const float b1 = from_prev_lane(w3.y), b2 = from_prev_lane(w3.z), b3 = from_prev_lane(w3.w);
@@ -187,10 +230,18 @@ __device__ __forceinline__ void stream_item(const StepArgs &a, int u) {
         lap.w = d2(w3.w, w4.w, w2.w, w5.w, w1.w, w6.w, w0.w) + d2(w3.w, n0, w3.z, n1, w3.y, n2, w3.x);
         // (d2x + d2y + source) at the injection cells of this column (:443)
         while (e < ev_e) {
-            const int cell = gp(a.ev_cell)[e];
-            if ((cell >> 16) != iw) break;
+            int cell;
+            float val;
+            if (e < e_reg) {
+                cell = __builtin_amdgcn_readlane(ev_c, e - e_first);
+                if ((cell >> 16) != iw) break;
+                val = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, ev_v), e - e_first));
+            } else {  // events past the first 64 of the item
+                cell = gp(a.ev_cell)[e];
+                if ((cell >> 16) != iw) break;
+                val = inj[gp(a.ev_inj)[e]];
+            }
             const int r = (cell & 0xffff) - (o - 4);
-            const float val = inj[gp(a.ev_inj)[e]];
             if (lane == (r >> 2)) {
                 const int q = r & 3;
                 if (q == 0) lap.x = lap.x + val;
@@ -209,13 +260,13 @@ __device__ __forceinline__ void stream_item(const StepArgs &a, int u) {
             if (lane == 1) v.x = 0.f;
         }
         if (full) {
-            PN[i * ld4] = v;
+            *lanep<v4f>(PN + i * ld, lo) = v;
         } else if (any) {
-            float *q = a.pn + prow + 4 * lane + (size_t)i * a.ld;
-            if (ow0) gp(q)[0] = v.x;
-            if (ow1) gp(q)[1] = v.y;
-            if (ow2) gp(q)[2] = v.z;
-            if (ow3) gp(q)[3] = v.w;
+            gptr<float> q = lanep<float>(PN + i * ld, lo);
+            if (ow0) q[0] = v.x;
+            if (ow1) q[1] = v.y;
+            if (ow2) q[2] = v.z;
+            if (ow3) q[3] = v.w;
         }
         v4f dv;
         dv.x = v.x - p1.x;
@@ -230,16 +281,15 @@ __device__ __forceinline__ void stream_item(const StepArgs &a, int u) {
             f.y = (dv.y * 0.5f) / a.dt;
             f.z = (dv.z * 0.5f) / a.dt;
             f.w = (dv.w * 0.5f) / a.dt;
-            const size_t so = sbase + (size_t)iw * a.lds + srow;
-            if (a.snap_fmt == kSnapF32) {
-                float *q = (float *)a.snaps + so;
+            if (kF32) {
                 if (full) {
-                    __builtin_nontemporal_store(f, gp((v4f *)q));
+                    __builtin_nontemporal_store(f, lanep<v4f>(SN + co, lo));
                 } else if (any) {
-                    if (ow0) gp(q)[0] = f.x;
-                    if (ow1) gp(q)[1] = f.y;
-                    if (ow2) gp(q)[2] = f.z;
-                    if (ow3) gp(q)[3] = f.w;
+                    gptr<float> q = lanep<float>(SN + co, lo);
+                    if (ow0) q[0] = f.x;
+                    if (ow1) q[1] = f.y;
+                    if (ow2) q[2] = f.z;
+                    if (ow3) q[3] = f.w;
                 }
             } else {
                 // 16-bit: power-of-two scale from the column's max-abs over the owned rows
@@ -259,20 +309,20 @@ __device__ __forceinline__ void stream_item(const StepArgs &a, int u) {
                     const float sv = ldexpf(fs4[c], -ex);
                     hs[c] = a.snap_fmt == kSnapF16 ? __builtin_bit_cast(unsigned short, (_Float16)sv) : f32_to_bf16(sv);
                 }
-                unsigned short *q = (unsigned short *)a.snaps + so;
                 if (full) {
-                    __builtin_nontemporal_store(v2u{(unsigned)hs[0] | ((unsigned)hs[1] << 16), (unsigned)hs[2] | ((unsigned)hs[3] << 16)}, gp((v2u *)q));
+                    __builtin_nontemporal_store(v2u{(unsigned)hs[0] | ((unsigned)hs[1] << 16), (unsigned)hs[2] | ((unsigned)hs[3] << 16)}, lanep<v2u>(SH + co, lo2));
                 } else if (any) {
-                    if (ow0) gp(q)[0] = hs[0];
-                    if (ow1) gp(q)[1] = hs[1];
-                    if (ow2) gp(q)[2] = hs[2];
-                    if (ow3) gp(q)[3] = hs[3];
+                    gptr<unsigned short> q = lanep<unsigned short>(SH + co, lo2);
+                    if (ow0) q[0] = hs[0];
+                    if (ow1) q[1] = hs[1];
+                    if (ow2) q[2] = hs[2];
+                    if (ow3) q[3] = hs[3];
                 }
             }
         } else if (imaging) {
             // kernel += 0.5*(q(k)-q(k-2)) * F(snap) * store_snap (solver_ac_backward_ram.f90:465)
             v4f F = F4;
-            if (a.snap_fmt != kSnapF32) {
+            if (!kF32) {
                 F.x = F.x * scale;
                 F.y = F.y * scale;
                 F.z = F.z * scale;
@@ -283,7 +333,7 @@ __device__ __forceinline__ void stream_item(const StepArgs &a, int u) {
             term.y = ((0.5f * dv.y) * F.y) * a.ss_f;
             term.z = ((0.5f * dv.z) * F.z) * a.ss_f;
             term.w = ((0.5f * dv.w) * F.w) * a.ss_f;
-            if (a.kahan) {
+            if (kKahan) {
                 v4f y;
                 y.x = term.x - cm4.x;
                 y.y = term.y - cm4.y;
@@ -303,21 +353,21 @@ __device__ __forceinline__ void stream_item(const StepArgs &a, int u) {
                 img.z = im4.z + term.z;
                 img.w = im4.w + term.w;
             }
-            float *qi = a.image + (size_t)iw * a.lds + srow;
-            float *qc = a.image_comp + (size_t)iw * a.lds + srow;
             if (full) {
-                *gp((v4f *)qi) = img;
-                if (a.kahan) *gp((v4f *)qc) = cmp;
+                *lanep<v4f>(IM + co, lo) = img;
+                if (kKahan) *lanep<v4f>(CM + co, lo) = cmp;
             } else if (any) {
-                if (ow0) gp(qi)[0] = img.x;
-                if (ow1) gp(qi)[1] = img.y;
-                if (ow2) gp(qi)[2] = img.z;
-                if (ow3) gp(qi)[3] = img.w;
-                if (a.kahan) {
-                    if (ow0) gp(qc)[0] = cmp.x;
-                    if (ow1) gp(qc)[1] = cmp.y;
-                    if (ow2) gp(qc)[2] = cmp.z;
-                    if (ow3) gp(qc)[3] = cmp.w;
+                gptr<float> qi = lanep<float>(IM + co, lo);
+                if (ow0) qi[0] = img.x;
+                if (ow1) qi[1] = img.y;
+                if (ow2) qi[2] = img.z;
+                if (ow3) qi[3] = img.w;
+                if (kKahan) {
+                    gptr<float> qc = lanep<float>(CM + co, lo);
+                    if (ow0) qc[0] = cmp.x;
+                    if (ow1) qc[1] = cmp.y;
+                    if (ow2) qc[2] = cmp.z;
+                    if (ow3) qc[3] = cmp.w;
                 }
             }
         }
@@ -498,14 +548,25 @@ __device__ __forceinline__ void cpml_tile(const StepArgs &a, int xs, int ys, int
     }
 }
 
+// LDS of a step workgroup: a CPML tile's wavefield tile and x-memory staging, or the
+// four stream items' snapshot staging (kStageCols columns x 64 lanes x 16 B each).
+constexpr int cpml_tile_floats(int TX) { return (TX + 6) * (kTileY + 6); }
+constexpr int lds_floats(int TX) {
+    return cpml_tile_floats(TX) + (TX + 6) * kTileY > 4 * kStageCols * 64 * 4 ? cpml_tile_floats(TX) + (TX + 6) * kTileY
+                                                                              : 4 * kStageCols * 64 * 4;
+}
+
 // One work item of one time step of one shot: t < ncpml is a CPML tile (the whole
 // workgroup, LDS-staged), otherwise a group of four interior stream items (one per
-// wavefront, no workgroup barrier).  The branch is uniform over the workgroup.
-template <int TX>
-__device__ __forceinline__ void step_item(const StepArgs &a, int t, float *tile, float *aux) {
+// wavefront, no workgroup barrier).  The branch is uniform over the workgroup.  SV: the
+// snapshot / imaging variant the interior is compiled for (SnapVariant).
+template <int TX, int SV>
+__device__ __forceinline__ void step_item(const StepArgs &a, int t, float *lds) {
+    float *tile = lds, *aux = lds + cpml_tile_floats(TX);
     if (t >= a.ncpml) {
-        const int u = 4 * (t - a.ncpml) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-        if (u < a.nstream) stream_item(a, u);
+        const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+        const int u = 4 * (t - a.ncpml) + w;
+        if (u < a.nstream) stream_item<SV>(a, u, (v4f *)lds + w * kStageCols * 64);
         return;
     }
     int R, C;
@@ -527,8 +588,7 @@ __device__ __forceinline__ void step_item(const StepArgs &a, int t, float *tile,
 
 template <int TX>
 __global__ __launch_bounds__(kBlock) void rtm_step_kernel(StepArgs a) {
-    __shared__ float tile[(TX + 6) * (kTileY + 6)];
-    __shared__ float aux[(TX + 6) * kTileY];
+    __shared__ __attribute__((aligned(16))) float lds[lds_floats(TX)];
     const int b = blockIdx.x;
     // traces of the previous step (:567-571), read from p(k-1) before it is overwritten
     if (a.rec_row >= 0)
@@ -540,7 +600,7 @@ __global__ __launch_bounds__(kBlock) void rtm_step_kernel(StepArgs a) {
     // contiguous tile range [x*per_xcd, (x+1)*per_xcd) and shares halos in its own L2.
     const int t = (b & 7) * a.per_xcd + (b >> 3);
     if (t >= a.nitems) return;
-    step_item<TX>(a, t, tile, aux);
+    step_item<TX, kSvF32>(a, t, lds);  // the one-shot solver: FP32 snapshots, plain imaging
 }
 
 // Per-step fields of a slot at global step G (forward steps k = 2..nt, then backward
@@ -594,11 +654,10 @@ __device__ __forceinline__ bool resolve_step(const SlotDesc &sd, long long G, St
 // list; each XCD takes a contiguous range of items (neighbouring tiles of one shot).
 // The slot descriptors are read-only during a launch (the host rewrites them between
 // chunks), so their words are fetched with scalar loads.
-template <int TX>
+template <int TX, int SV>
 __global__ __launch_bounds__(kBlock) void rtm_batch_kernel(BatchArgs b, const SlotDesc *__restrict__ slots,
                                                            const long long *__restrict__ gbase) {
-    __shared__ float tile[(TX + 6) * (kTileY + 6)];
-    __shared__ float aux[(TX + 6) * kTileY];
+    __shared__ __attribute__((aligned(16))) float lds[lds_floats(TX)];
     const long long G = *gbase + b.goff;
     const int x = blockIdx.x & 7, nbx = gridDim.x >> 3;
     for (int m = blockIdx.x >> 3; m < b.per_xcd; m += nbx) {
@@ -608,7 +667,7 @@ __global__ __launch_bounds__(kBlock) void rtm_batch_kernel(BatchArgs b, const Sl
         const SlotDesc &sd = slots[s];
         StepArgs a = sd.a;
         if (!resolve_step(sd, G, a) || t >= a.nitems) continue;
-        step_item<TX>(a, t, tile, aux);
+        step_item<TX, SV>(a, t, lds);
         __syncthreads();  // the LDS buffers are reused by the next item
     }
 }
@@ -688,22 +747,33 @@ void launch_step(const StepArgs &a, int tile_x, hipStream_t s, hipEvent_t e0, hi
         launch_step_tx<8>(a, s, e0, e1);
 }
 
-template <int TX>
+template <int TX, int SV>
 void launch_batch_tx(const BatchArgs &b, int grid, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
     if (!e0 && !e1)  // plain launch: also valid under stream capture
-        hipLaunchKernelGGL(rtm_batch_kernel<TX>, dim3(grid), dim3(kBlock), 0, s, b, b.slots, b.gbase);
+        hipLaunchKernelGGL((rtm_batch_kernel<TX, SV>), dim3(grid), dim3(kBlock), 0, s, b, b.slots, b.gbase);
     else
-        hipExtLaunchKernelGGL(rtm_batch_kernel<TX>, dim3(grid), dim3(kBlock), 0, s, e0, e1, 0, b, b.slots, b.gbase);
+        hipExtLaunchKernelGGL((rtm_batch_kernel<TX, SV>), dim3(grid), dim3(kBlock), 0, s, e0, e1, 0, b, b.slots,
+                              b.gbase);
 }
 
-void launch_batch(const BatchArgs &b, int tile_x, int grid, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
-    grid = (grid + 7) / 8 * 8;
+template <int SV>
+void launch_batch_sv(const BatchArgs &b, int tile_x, int grid, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
     if (tile_x == 32)
-        launch_batch_tx<32>(b, grid, s, e0, e1);
+        launch_batch_tx<32, SV>(b, grid, s, e0, e1);
     else if (tile_x == 16)
-        launch_batch_tx<16>(b, grid, s, e0, e1);
+        launch_batch_tx<16, SV>(b, grid, s, e0, e1);
     else
-        launch_batch_tx<8>(b, grid, s, e0, e1);
+        launch_batch_tx<8, SV>(b, grid, s, e0, e1);
+}
+
+void launch_batch(const BatchArgs &b, int tile_x, int sv, int grid, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+    grid = (grid + 7) / 8 * 8;
+    switch (sv) {
+    case kSvF32Kahan: launch_batch_sv<kSvF32Kahan>(b, tile_x, grid, s, e0, e1); break;
+    case kSv16: launch_batch_sv<kSv16>(b, tile_x, grid, s, e0, e1); break;
+    case kSv16Kahan: launch_batch_sv<kSv16Kahan>(b, tile_x, grid, s, e0, e1); break;
+    default: launch_batch_sv<kSvF32>(b, tile_x, grid, s, e0, e1); break;
+    }
 }
 
 void launch_advance(long long *gbase, long long by, hipStream_t s) {

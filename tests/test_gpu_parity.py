@@ -133,13 +133,17 @@ def test_gpu_large_window_against_oracle(ctx):
 # Interior layouts the small golden cases do not reach: two and three 248-row stream tiles
 # (the last one moved up over its neighbour's rows, ny not a multiple of 4), windows
 # narrower and wider than a stream item, sources and receivers in the rows two stream
-# tiles both compute, a free surface, and stream widths 1..8.
+# tiles both compute, a free surface, dense receiver lines, and stream widths 1..8.
 STREAM_CASES = {
     "two_tiles_overlap_fs": dict(ny=301, nx=37, nt=140, dPML=8, seed=11, fs=1, src=(17, 250),
                                  rec_x=[3, 9, 20, 33, 20, 21], rec_y=[100, 247, 248, 260, 300, 1]),
     "three_tiles_disk": dict(ny=530, nx=23, nt=120, dPML=6, seed=12, variant=1, src=(11, 497),
                              rec_x=np.arange(2, 23, 2), rec_y=495),
     "narrow_window": dict(ny=259, nx=16, nt=110, dPML=9, seed=13, src=(5, 252), rec_x=[1, 16, 7], rec_y=[255, 256, 3]),
+    # a receiver on every cell of 12 rows: 96 injection events per 8-column item, past the
+    # 64 an item holds one per lane
+    "dense_receivers": dict(ny=120, nx=20, nt=100, dPML=6, seed=14, src=(10, 45), rec_x=np.tile(np.arange(1, 21), 12),
+                            rec_y=np.repeat(np.arange(30, 42), 20)),
 }
 
 
