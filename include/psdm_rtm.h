@@ -192,6 +192,44 @@ int psdm_rtm_engine_timing(psdm_rtm_engine *engine, double *mean_launch_ms, long
 int psdm_rtm_engine_stats(psdm_rtm_engine *engine, long long *steps_issued, long long *slot_steps,
                           long long *hbm_used, long long *hbm_budget);
 
+/* ------------------------------------------------------------------------------
+ * Gather conditioning before migration: the per-shot steps of the driver between
+ * reading a gather and migrating it (calculate_PSDM_RTM.f90:299-358), on the device.
+ *   1. frequency filter when f_init > 0 and f_final > 0 (freq_filter, auxiliar.f90:
+ *      498-531 -> filters_function1D, filters.f90:50-234): the 6th-order Butterworth
+ *      of the reference applied to the trace zero-padded x24; typef 1 low-pass at
+ *      f_final, 2 high-pass at 40 Hz, 3 band-pass f_init..f_final;
+ *   2. resampling from dt_data to dt_prop when they differ (interpolation_akima.f90:
+ *      1-31 -> INTRPL, akima433.f90:6-170), else the first nt_prop2 samples;
+ *   3. cut_sf: samples 1..nt_sf(i) set to 0 (nt_sf != NULL);
+ *   4. nt_cut == NULL: with taper_option != 0 the half-cosine taper over the N_taper
+ *      samples ending at nt_prop2 (taper_pad_data, filters.f90:1-48); nt_cut != NULL
+ *      (cut_multiple > 0): samples nt_cut(i)..nt_taper set to 0 and the taper ending at
+ *      nt_cut(i) (taper_pad_trace);
+ *   rows nt_prop2+1..nt_taper of the output are 0 (Data_target is zero-filled, :294-296).
+ * d_field: device Field_Data(nt_data, NumRec) (column-major, not modified); d_target:
+ * device Data_target(nt_taper, NumRec).  Steps 2-4 reproduce the reference's FP32
+ * arithmetic bit for bit; the filter is the same linear operator evaluated as a
+ * convolution (taps from the reference's transfer function in FP64), within 1e-5 rel-L2
+ * of the reference's FFTPACK evaluation.  Synchronous on the context's stream.
+ * ------------------------------------------------------------------------------ */
+typedef struct {
+    int nt_data;             /* samples per raw trace */
+    float dt_data;           /* raw sample interval (s) */
+    int typef;               /* filter type: 1 low-pass, 2 high-pass, 3 band-pass */
+    float f_init, f_final;   /* filter corners (Hz); no filter unless both > 0 */
+    int nt_prop2;            /* resampled samples kept */
+    float dt_prop;           /* propagation time step (s) */
+    int nt_taper;            /* output rows (>= nt_prop2) */
+    int taper_option;        /* != 0: taper at nt_prop2 (only when nt_cut == NULL) */
+    int N_taper, N_zeros;    /* taper length (>= 1 when a taper applies), zero padding */
+    const int *nt_sf;        /* host (NumRec) or NULL */
+    const int *nt_cut;       /* host (NumRec) or NULL; 1 <= nt_cut(i) <= nt_taper */
+} psdm_rtm_cond;
+
+int psdm_rtm_condition(psdm_rtm_ctx *ctx, const psdm_rtm_cond *cond, int NumRec, const float *d_field,
+                       float *d_target);
+
 #ifdef __cplusplus
 }
 #endif

@@ -139,4 +139,39 @@ contains
     end select
   end subroutine
 
+  ! Data conditioning of a gather before migration (calculate_PSDM_RTM.f90:299-358): the
+  ! reference's own frequency filter (auxiliar.f90:498-531 -> filters.f90:50-234 with
+  ! FFTPACK), Akima resampling (interpolation_akima.f90:1-31 -> akima433.f90) and the
+  ! half-cosine taper with zero padding (filters.f90:20-48).
+  subroutine ref_freq_filter(nt, dt, Num, typef, f1, f2, data) bind(C, name="ref_freq_filter")
+    integer(c_int), value :: nt, Num, typef
+    real(c_float), value :: dt, f1, f2
+    real(c_float) :: data(nt, Num)
+    integer :: nt_, num_, tf
+    real :: dt_, fa, fb
+    nt_ = nt; num_ = Num; tf = typef; dt_ = dt; fa = f1; fb = f2
+    call freq_filter(data, nt_, dt_, num_, tf, fa, fb)
+  end subroutine
+
+  subroutine ref_interpolation_akima(nt_data, dt_data, NumRec, data, nt, dt, out) &
+      bind(C, name="ref_interpolation_akima")
+    integer(c_int), value :: nt_data, NumRec, nt
+    real(c_float), value :: dt_data, dt
+    real(c_float) :: data(nt_data, NumRec), out(nt, NumRec)
+    logical, external :: interpolation_akima
+    logical :: ok
+    integer :: ntd, nr, ntt
+    real :: dtd, dtt
+    ntd = nt_data; nr = NumRec; ntt = nt; dtd = dt_data; dtt = dt
+    ok = interpolation_akima(data, dtd, ntd, out, nr, dtt, ntt)
+  end subroutine
+
+  subroutine ref_taper_pad_trace(trace, nt_prop, N_taper, N_zeros, nt_new) bind(C, name="ref_taper_pad_trace")
+    integer(c_int), value :: nt_prop, N_taper, N_zeros, nt_new
+    real(c_float) :: trace(nt_new)
+    integer :: a, b, c, d
+    a = nt_prop; b = N_taper; c = N_zeros; d = nt_new
+    call taper_pad_trace(trace, a, b, c, d)
+  end subroutine
+
 end module ref_harness

@@ -42,6 +42,8 @@ struct psdm_rtm_ctx {
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pending;
     long long n_timed = 0;
     double sum_ms = 0.0;
+    // gather conditioning: filter taps of the last (nt_data, fe, typef, f_init, f_final)
+    std::vector<unsigned> cond_key;
 };
 
 namespace psdm {

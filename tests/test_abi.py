@@ -70,6 +70,9 @@ def test_python_structs_mirror_the_header():
 
     assert [f[0] for f in EngineCfg._fields_] == _struct_fields("psdm_rtm_engine_cfg")
     assert [f[0] for f in Job._fields_] == _struct_fields("psdm_rtm_job")
+    from psdm_amd.conditioning import Cond
+
+    assert [f[0] for f in Cond._fields_] == _struct_fields("psdm_rtm_cond")
 
 
 @pytest.mark.skipif(gpu_available(), reason="checks the no-device error path")
