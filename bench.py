@@ -350,6 +350,129 @@ def bench_uneven(args, mixed=False):
     ctx.close()
 
 
+def bench_condition(args):
+    """Gather conditioning on the device (psdm_rtm_condition): `--steps` MCS gathers of 480
+    traces x 2401 samples at 2 ms (4.8 s records), band-pass 5-40 Hz, Akima resampling to
+    the 0.8 ms propagation step (6000 samples), cut before the water-bottom reflection and
+    a 0.2 s half-cosine taper (calculate_PSDM_RTM.f90:229-358).  Inputs resident in HBM;
+    one JSON line.  The CPU baseline runs the reference's own routines (oracle/_ref)."""
+    import numpy as np
+
+    from psdm_amd.conditioning import cut_indices, taper_lengths
+
+    nt_data, dt_data, nrec = 2401, 0.002, 480
+    dt_prop, nt_prop2 = 0.0008, 6000
+    typef, f1, f2 = 3, 5.0, 40.0
+    N_taper, _ = taper_lengths(0.2, 0.0, dt_prop)
+    cpu = None
+    if not args.no_cpu:  # before anything touches the GPU (the pool spawns processes)
+        from oracle import cpu_baseline
+
+        per_rank = 96
+        log(f"[bench] condition cpu baseline: {args.cpu_procs} procs x {per_rank} traces")
+        r = cpu_baseline.run_conditioning(nt_data, dt_data, per_rank, nt_prop2, dt_prop, typef, f1, f2, N_taper,
+                                          args.cpu_procs)
+        cpu = {"value": round(r["traces_per_s"] / nrec, 4), "unit": "gathers/s", "cores": r["cores"],
+               "kind": r["kind"],
+               "sample": f"{r['cores']} processes, each conditioning {per_rank} traces of {nt_data} samples "
+                         f"(freq_filter band-pass {f1:g}-{f2:g} Hz with FFTPACK over the x24 zero-padded trace, "
+                         f"interpolation_akima {dt_data * 1e3:g} -> {dt_prop * 1e3:g} ms, taper_pad_trace); "
+                         f"gathers/s = traces/s / {nrec} (slowest process {r['rank_s']:.2f} s)"}
+        log(f"[bench] condition cpu baseline {cpu['value']} gathers/s")
+
+    import torch
+
+    from psdm_amd import conditioning as CD
+    from psdm_amd._native import Context
+    from psdm_amd.workloads import line_workload
+
+    dev = torch.device("cuda:0")
+    ctx = Context(0)
+    W = line_workload("mcs2001")
+    m, shots = W["model"], W["shots"]
+    bat = np.full(m.nx, m.water_depth, np.float32)
+    n_src = 16  # distinct device gathers cycled through the timed region
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(0)
+    raw = torch.randn((n_src, nrec, nt_data), generator=gen, device=dev)
+    outs = torch.empty((n_src, nrec, nt_prop2), device=dev)
+    sfs = [cut_indices(sh.nxSou, sh.nySou, sh.nxRec, sh.nyRec, bat, m.dx, dt_prop, "sf")
+           for sh in shots[:n_src]]
+    kw = dict(dt_data=dt_data, nt_prop2=nt_prop2, dt_prop=dt_prop, typef=typef, f_init=f1, f_final=f2,
+              taper_option=1, N_taper=N_taper)
+    for k in range(max(args.warmup, 1)):  # the first call designs the filter taps
+        CD.condition(ctx, raw[k % n_src], nt_sf=sfs[k % n_src], out=outs[k % n_src], **kw)
+    ctx.set_timing(1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        CD.condition(ctx, raw[k % n_src], nt_sf=sfs[k % n_src], out=outs[k % n_src], **kw)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    f_ms, r_ms, n_timed = CD.timing(ctx)
+    ctx.set_timing(0)
+    flops = 2.0 * nt_data * nt_data * nrec
+    ach = flops / (f_ms * 1e-3) / 1e12
+    rbytes = 4.0 * nrec * (nt_data + nt_prop2)
+    acc = condition_accuracy(ctx)
+    tpath = os.path.join(ROOT, "profiles", "traffic_cond_toeplitz_kernel_mcs.md")
+    traffic = None
+    if os.path.exists(tpath):
+        import re
+
+        mt = re.search(r"^traffic_bytes_per_launch = (\d+)", open(tpath).read(), re.M)
+        traffic = int(mt.group(1)) if mt else None
+    out = {
+        "metric": "conditioned gathers/s (filter + resample + cut/taper)",
+        "value": round(args.steps / elapsed, 3), "unit": "gathers/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": max(args.warmup, 1), "ms_per_step": round(1e3 * elapsed / args.steps, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded Gaussian traces; cut_sf from the mcs2001 line's geometry and water depth)",
+        "config": {"workload": f"MCS gathers: {nrec} traces x {nt_data} samples @ {dt_data * 1e3:g} ms -> "
+                               f"{nt_prop2} @ {dt_prop * 1e3:g} ms, band-pass {f1:g}-{f2:g} Hz, "
+                               f"cut_sf, taper {N_taper} samples"},
+        "roofline": {"bound": "mfma", "achieved": round(ach, 2), "peak": 157.3, "unit": "TFLOP/s",
+                     "frac": round(ach / 157.3, 4), "traffic": traffic, "kernel": "cond_toeplitz_kernel",
+                     "mean_kernel_ms": round(f_ms, 5), "timed_launches": n_timed,
+                     "algorithmic_flops_per_launch": flops,
+                     "note": "FP32 fused multiply-adds on the vector ALUs (2*nt^2*traces per gather); peak = "
+                             "MI355X FP32 157.3 TFLOP/s"},
+        "resample_kernel": {"mean_kernel_ms": round(r_ms, 5), "algorithmic_bytes": rbytes,
+                            "achieved_gbs": round(rbytes / (r_ms * 1e-3) / 1e9, 1) if r_ms else None},
+        "cpu_baseline": cpu,
+        "speedup_vs_cpu": round(args.steps / elapsed / cpu["value"], 1) if cpu else None,
+        "accuracy": acc,
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+def condition_accuracy(ctx):
+    """Device conditioning of the golden cases (written from the reference's own routines,
+    tests/golden/conditioning_ref.npz): rel-L2 where the filter runs, bit-equality else."""
+    import numpy as np
+    import torch
+
+    from psdm_amd.conditioning import condition
+
+    z = np.load(os.path.join(ROOT, "tests", "golden", "conditioning_ref.npz"))
+    res = {}
+    for name in [k[:-4] for k in z.files if k.endswith("_raw")]:
+        p = dict(zip(z["param_names"], z[name + "_params"]))
+        d = torch.from_numpy(np.ascontiguousarray(z[name + "_raw"].T)).to("cuda:0")
+        o = condition(ctx, d, dt_data=float(p["dt_data"]), typef=int(p["typef"]), f_init=float(p["f_init"]),
+                      f_final=float(p["f_final"]), nt_prop2=int(p["nt_prop2"]), dt_prop=float(p["dt_prop"]),
+                      nt_taper=int(p["nt_taper"]), N_taper=int(p["N_taper"]), N_zeros=int(p["N_zeros"]),
+                      taper_option=int(p["taper_option"]),
+                      nt_sf=z[name + "_nt_sf"] if name + "_nt_sf" in z.files else None,
+                      nt_cut=z[name + "_nt_cut"] if name + "_nt_cut" in z.files else None)
+        o = o.cpu().numpy().T
+        ref = z[name + "_out"]
+        res[name] = {"rel_l2_vs_reference": float(np.linalg.norm(o - ref) / np.linalg.norm(ref)),
+                     "bit_equal": bool(np.array_equal(o, ref))}
+    return res
+
+
 def bench_served(args):
     """Served stream: Poisson arrivals at 4 rates; 70% MCS gathers (480 ch @12.5 m, 8 km
     aperture, 601-deep model @10 m, nt 6000, FP32 snapshots every step) and 30% WAS
@@ -456,7 +579,7 @@ def main():
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="batched form with the default launch shape")
     ap.add_argument("--forms", default="plain,batched")
-    ap.add_argument("--workload", default="mcs2001", choices=["mcs2001", "uneven", "mixed", "served"],
+    ap.add_argument("--workload", default="mcs2001", choices=["mcs2001", "uneven", "mixed", "served", "condition"],
                     help="mcs2001: the 64-shot line (default line); uneven: 256 uneven shots, batched vs plain; "
                          "mixed: uneven + FP16/BF16 snapshots every 4th step; served: Poisson-arrival stream")
     ap.add_argument("--shots", type=int, default=256, help="uneven/mixed: shots of the 256-shot line used")
@@ -476,6 +599,8 @@ def main():
         return bench_uneven(args, mixed=args.workload == "mixed")
     if args.workload == "served":
         return bench_served(args)
+    if args.workload == "condition":
+        return bench_condition(args)
     forms = set(args.forms.split(","))
 
     from psdm_amd.workloads import line_workload, step_bytes, updates_per_shot

@@ -229,6 +229,10 @@ typedef struct {
 
 int psdm_rtm_condition(psdm_rtm_ctx *ctx, const psdm_rtm_cond *cond, int NumRec, const float *d_field,
                        float *d_target);
+/* With psdm_rtm_set_timing(ctx, n != 0): mean device time per call of the filter kernel
+ * and of the resample/cut/taper kernel (start/stop stamps of their dispatches), and the
+ * number of calls timed since timing was last set. */
+int psdm_rtm_condition_timing(psdm_rtm_ctx *ctx, double *filter_ms, double *resample_ms, long long *n_calls);
 
 #ifdef __cplusplus
 }

@@ -52,3 +52,41 @@ def run(ny: int, nx: int, nt_sample: int, dPML: int, dx: float, dt: float, nproc
     busy = max(per)
     return dict(kind=kind, cores=nprocs, updates_per_rank=updates, wall_s=wall, rank_s=busy,
                 gupd_per_s=nprocs * updates / busy / 1e9)
+
+
+def _one_rank_conditioning(args):
+    nt_data, dt_data, nrec, nt_prop2, dt_prop, typef, f1, f2, N_taper, seed = args
+    import sys
+
+    here = os.path.dirname(os.path.abspath(__file__))
+    sys.path.insert(0, os.path.dirname(here))
+    from oracle import conditioning as C
+
+    rng = np.random.default_rng(seed)
+    raw = np.asfortranarray(rng.standard_normal((nt_data, nrec)).astype(np.float32))
+    t0 = time.perf_counter()
+    f = C.ref_freq_filter(raw, dt_data, typef, f1, f2)
+    out = C.ref_akima(f, dt_data, nt_prop2, dt_prop)
+    for i in range(nrec):
+        out[:, i] = C.ref_taper_pad_trace(out[:, i], nt_prop2, N_taper, 0)
+    return time.perf_counter() - t0
+
+
+def run_conditioning(nt_data: int, dt_data: float, nrec: int, nt_prop2: int, dt_prop: float, typef: int, f1: float,
+                     f2: float, N_taper: int, nprocs: int):
+    """Gather conditioning by the reference's own routines (oracle/_ref: freq_filter with
+    FFTPACK, interpolation_akima, taper_pad_trace), one gather of `nrec` traces per
+    process; returns traces per second over the slowest process."""
+    from oracle import pyoracle as O
+
+    if not O.ref_available():
+        raise RuntimeError("oracle/_ref/libpsdm_ref.so is not built")
+    ctx = mp.get_context("spawn")
+    t0 = time.perf_counter()
+    with ctx.Pool(nprocs) as pool:
+        per = pool.map(_one_rank_conditioning, [(nt_data, dt_data, nrec, nt_prop2, dt_prop, typef, f1, f2, N_taper, s)
+                                                for s in range(nprocs)])
+    wall = time.perf_counter() - t0
+    busy = max(per)
+    return dict(kind="reference", cores=nprocs, traces_per_rank=nrec, wall_s=wall, rank_s=busy,
+                traces_per_s=nprocs * nrec / busy)

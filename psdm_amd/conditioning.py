@@ -90,3 +90,11 @@ def condition(ctx: Context, d_field, dt_data: float, nt_prop2: int, dt_prop: flo
     ctx.check(ctx._L.psdm_rtm_condition(ctx.h, C.byref(q), nrec, C.c_void_p(d_field.data_ptr()),
                                         C.c_void_p(out.data_ptr())), "psdm_rtm_condition")
     return out
+
+
+def timing(ctx: Context):
+    """(filter ms, resample/cut/taper ms, calls) per call since ctx.set_timing(n != 0)."""
+    f, r, n = C.c_double(), C.c_double(), C.c_longlong()
+    ctx.check(ctx._L.psdm_rtm_condition_timing(ctx.h, C.byref(f), C.byref(r), C.byref(n)),
+              "psdm_rtm_condition_timing")
+    return f.value, r.value, n.value

@@ -549,6 +549,8 @@ int psdm_rtm_destroy(psdm_rtm_ctx *c) {
         (void)hipEventDestroy(pr.second);
     }
     for (auto e : c->ev_free) (void)hipEventDestroy(e);
+    for (auto e : c->cond_ev)
+        if (e) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(c->stream);
     delete c;
     return 0;
@@ -674,6 +676,8 @@ int psdm_rtm_set_timing(psdm_rtm_ctx *c, int timing) {
     c->timing = timing < 0 ? 0 : timing;
     c->n_timed = 0;
     c->sum_ms = 0.0;
+    c->cond_ms[0] = c->cond_ms[1] = 0.0;
+    c->cond_n[0] = c->cond_n[1] = 0;
     return 0;
 }
 

@@ -17,6 +17,7 @@
 //   cut/taper  cut_sf zeroing (:325-329), taper_pad_data / taper_pad_trace with the
 //              cut_multiple zeroing (:331-346, filters.f90:1-48); the taper weights are
 //              computed on the host with libm's cosf, as the reference's cos.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -328,6 +329,10 @@ extern "C" int psdm_rtm_condition(psdm_rtm_ctx *c, const psdm_rtm_cond *q, int N
             if (q->nt_cut[i] < 1 || q->nt_cut[i] > q->nt_taper)
                 return fail(c, PSDM_RTM_EINVAL, "condition: nt_cut out of 1..nt_taper");
     if (int rc = set_device(c)) return rc;
+    const bool timed = c->timing != 0;
+    if (timed)
+        for (auto &e : c->cond_ev)
+            if (!e) HIP_TRY(c, hipEventCreate(&e));
     const int nt = q->nt_data;
     const float *src = d_field;
     if (filt) {
@@ -352,8 +357,9 @@ extern "C" int psdm_rtm_condition(psdm_rtm_ctx *c, const psdm_rtm_cond *q, int N
         }
         void *vf;
         if (int rc = get_buf(c, "cond_filtered", (size_t)nt * NumRec * sizeof(float), &vf)) return rc;
-        hipLaunchKernelGGL(cond_toeplitz_kernel, dim3((nt + kTN - 1) / kTN, (NumRec + kTR - 1) / kTR), dim3(256), 0,
-                           c->stream, d_field, nt, NumRec, (const float *)vt, (float *)vf);
+        hipExtLaunchKernelGGL(cond_toeplitz_kernel, dim3((nt + kTN - 1) / kTN, (NumRec + kTR - 1) / kTR), dim3(256),
+                              0, c->stream, timed ? c->cond_ev[0] : nullptr, timed ? c->cond_ev[1] : nullptr, 0,
+                              d_field, nt, NumRec, (const float *)vt, (float *)vf);
         HIP_TRY(c, hipGetLastError());
         src = (const float *)vf;
     }
@@ -372,10 +378,30 @@ extern "C" int psdm_rtm_condition(psdm_rtm_ctx *c, const psdm_rtm_cond *q, int N
     if (nw) HIP_TRY(c, hipMemcpyAsync(d_w, w.data(), nw * 4, hipMemcpyHostToDevice, c->stream));
     if (nsf) HIP_TRY(c, hipMemcpyAsync(d_sf, q->nt_sf, nsf * 4, hipMemcpyHostToDevice, c->stream));
     if (ncut) HIP_TRY(c, hipMemcpyAsync(d_cut, q->nt_cut, ncut * 4, hipMemcpyHostToDevice, c->stream));
-    hipLaunchKernelGGL(cond_resample_kernel, dim3((q->nt_taper + 255) / 256, NumRec), dim3(256), 0, c->stream, src,
-                       nt, q->dt_data, resample ? 1 : 0, q->nt_prop2, q->dt_prop, q->nt_taper, mode, q->N_taper, d_w,
-                       nsf ? d_sf : nullptr, ncut ? d_cut : nullptr, d_target);
+    hipExtLaunchKernelGGL(cond_resample_kernel, dim3((q->nt_taper + 255) / 256, NumRec), dim3(256), 0, c->stream,
+                          timed ? c->cond_ev[2] : nullptr, timed ? c->cond_ev[3] : nullptr, 0, src, nt, q->dt_data,
+                          resample ? 1 : 0, q->nt_prop2, q->dt_prop, q->nt_taper, mode, q->N_taper, d_w,
+                          nsf ? d_sf : nullptr, ncut ? d_cut : nullptr, d_target);
     HIP_TRY(c, hipGetLastError());
     HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (timed) {
+        float ms;
+        if (filt) {
+            HIP_TRY(c, hipEventElapsedTime(&ms, c->cond_ev[0], c->cond_ev[1]));
+            c->cond_ms[0] += ms;
+            c->cond_n[0]++;
+        }
+        HIP_TRY(c, hipEventElapsedTime(&ms, c->cond_ev[2], c->cond_ev[3]));
+        c->cond_ms[1] += ms;
+        c->cond_n[1]++;
+    }
+    return 0;
+}
+
+extern "C" int psdm_rtm_condition_timing(psdm_rtm_ctx *c, double *filter_ms, double *resample_ms, long long *n) {
+    if (!c) return PSDM_RTM_EINVAL;
+    if (filter_ms) *filter_ms = c->cond_n[0] ? c->cond_ms[0] / (double)c->cond_n[0] : 0.0;
+    if (resample_ms) *resample_ms = c->cond_n[1] ? c->cond_ms[1] / (double)c->cond_n[1] : 0.0;
+    if (n) *n = c->cond_n[1];
     return 0;
 }

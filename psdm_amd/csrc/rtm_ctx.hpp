@@ -42,8 +42,12 @@ struct psdm_rtm_ctx {
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pending;
     long long n_timed = 0;
     double sum_ms = 0.0;
-    // gather conditioning: filter taps of the last (nt_data, fe, typef, f_init, f_final)
+    // gather conditioning: filter taps of the last (nt_data, fe, typef, f_init, f_final);
+    // with timing on, device time of the filter and resample kernels per call
     std::vector<unsigned> cond_key;
+    hipEvent_t cond_ev[4] = {};
+    double cond_ms[2] = {0.0, 0.0};
+    long long cond_n[2] = {0, 0};
 };
 
 namespace psdm {
