@@ -435,8 +435,8 @@ def bench_condition(args):
                      "frac": round(ach / 157.3, 4), "traffic": traffic, "kernel": "cond_toeplitz_kernel",
                      "mean_kernel_ms": round(f_ms, 5), "timed_launches": n_timed,
                      "algorithmic_flops_per_launch": flops,
-                     "note": "FP32 fused multiply-adds on the vector ALUs (2*nt^2*traces per gather); peak = "
-                             "MI355X FP32 157.3 TFLOP/s"},
+                     "note": "Toeplitz product on the matrix cores (v_mfma_f32_32x32x2f32), 2*nt^2*traces "
+                             "FLOP per gather; peak = MI355X dense FP32 157.3 TFLOP/s"},
         "resample_kernel": {"mean_kernel_ms": round(r_ms, 5), "algorithmic_bytes": rbytes,
                             "achieved_gbs": round(rbytes / (r_ms * 1e-3) / 1e9, 1) if r_ms else None},
         "cpu_baseline": cpu,

@@ -10,7 +10,8 @@
 //              g = Re(IDFT_N(H)) at lags -(nt-1)..nt-1.  The taps are computed once per
 //              (nt, dt, filter) in FP64 (cond_taps_kernel) from H evaluated at the
 //              reference's own FP32-accumulated frequencies, and every gather is then one
-//              Toeplitz product over all its traces (cond_toeplitz_kernel, FP32 FMA).
+//              Toeplitz product over all its traces on the matrix cores
+//              (cond_toeplitz_kernel, FP32 MFMA).
 //   resample   interpolation_akima (interpolation_akima.f90:1-31) over INTRPL
 //              (akima433.f90:6-170), one thread per output sample, FP32 operations in the
 //              reference's order (bit for bit); fused with
@@ -64,56 +65,73 @@ __global__ __launch_bounds__(256) void cond_taps_kernel(const double *reH, const
     }
 }
 
-// y[r*nt + n] = sum_{m=0}^{nt-1} g[n - m + nt - 1] x[r*nt + m]: a 64-sample x 64-trace
-// output tile per workgroup, 4 x 4 per thread, m in chunks of 32 staged in LDS with the
-// 95 taps they need; m ascending, FP32 fused multiply-add.
-constexpr int kTN = 64, kTR = 64, kTM = 32;
+// y[r*nt + n] = sum_{m=0}^{nt-1} g[n - m + nt - 1] x[r*nt + m]: the gather times the
+// Toeplitz matrix T[n][m] = g[n - m] on the matrix cores.  A workgroup computes a
+// 64-sample x 64-trace tile, each wavefront 32 x 32 with v_mfma_f32_32x32x2f32
+// (A = T rows n, columns m..m+1: lane l holds T[i = l % 32][k = l / 32]; B = x rows
+// m..m+1: lane l holds x[k = l / 32][j = l % 32]; the 16 accumulators of lane l are
+// C[8*(v/4) + 4*(l/32) + v%4][l % 32]).  m runs in ascending chunks of 64 staged in LDS
+// with the 127 taps they need; the next chunk's global loads are issued into registers
+// before the current chunk's MFMAs, so their latency overlaps the matrix work.
+constexpr int kTN = 64, kTR = 64, kTM = 64, kTG = kTN + kTM - 1;
+typedef float v16f __attribute__((ext_vector_type(16)));
 __global__ __launch_bounds__(256) void cond_toeplitz_kernel(const float *__restrict__ x, int nt, int nrec,
                                                             const float *__restrict__ g, float *__restrict__ y) {
     __shared__ float xs[kTM][kTR + 1];
-    __shared__ float gs[kTN + kTM];
+    __shared__ float gs[kTG];
     const int n0 = blockIdx.x * kTN, r0 = blockIdx.y * kTR;
-    const int tid = threadIdx.x, tn = tid & 15, tr = tid >> 4;
-    float acc[4][4];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int wn = 32 * (w & 1), wr = 32 * (w >> 1);  // this wavefront's 32 x 32 sub-tile
+    const int li = lane & 31, lk = lane >> 5;
+    constexpr int kPer = kTM * kTR / 256;  // x values staged per thread and chunk
+    float xr[kPer], gr = 0.f;
+    auto fetch = [&](int m0) {  // element e = tid + 256 q: row m0 + e % kTM, trace r0 + e / kTM
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
+        for (int q = 0; q < kPer; ++q) {
+            const int e = tid + 256 * q, m = m0 + e % kTM, r = r0 + e / kTM;
+            xr[q] = (m < nt && r < nrec) ? x[(size_t)r * nt + m] : 0.f;
+        }
+        if (tid < kTG) {
+            const int d = n0 - m0 - (kTM - 1) + tid;  // lag n - m of this entry
+            gr = (d > -nt && d < nt) ? g[d + nt - 1] : 0.f;
+        }
+    };
+    auto stage = [&]() {
 #pragma unroll
-        for (int b = 0; b < 4; ++b) acc[a][b] = 0.f;
+        for (int q = 0; q < kPer; ++q) {
+            const int e = tid + 256 * q;
+            xs[e % kTM][e / kTM] = xr[q];
+        }
+        if (tid < kTG) gs[tid] = gr;
+    };
+    v16f acc;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) acc[v] = 0.f;
+    fetch(0);
+    stage();
+    __syncthreads();
     for (int m0 = 0; m0 < nt; m0 += kTM) {
-        for (int e = tid; e < kTM * kTR; e += 256) {
-            const int mm = e % kTM, rr = e / kTM;
-            const int m = m0 + mm, r = r0 + rr;
-            xs[mm][rr] = (m < nt && r < nrec) ? x[(size_t)r * nt + m] : 0.f;
-        }
-        for (int e = tid; e < kTN + kTM - 1; e += 256) {
-            const int d = n0 - m0 - (kTM - 1) + e;  // lag n - m of this entry
-            gs[e] = (d > -nt && d < nt) ? g[d + nt - 1] : 0.f;
-        }
-        __syncthreads();
-#pragma unroll 4
-        for (int mm = 0; mm < kTM; ++mm) {
-            float xv[4], gv[4];
-#pragma unroll
-            for (int b = 0; b < 4; ++b) xv[b] = xs[mm][tr + 16 * b];
-#pragma unroll
-            for (int a = 0; a < 4; ++a) gv[a] = gs[tn + 16 * a - mm + kTM - 1];
-#pragma unroll
-            for (int a = 0; a < 4; ++a)
-#pragma unroll
-                for (int b = 0; b < 4; ++b) acc[a][b] = fmaf(gv[a], xv[b], acc[a][b]);
+        const bool more = m0 + kTM < nt;
+        if (more) fetch(m0 + kTM);
+#pragma unroll 8
+        for (int mm = 0; mm < kTM; mm += 2) {
+            const float a = gs[wn + li - (mm + lk) + kTM - 1];  // T[n0 + wn + li][m0 + mm + lk]
+            const float b = xs[mm + lk][wr + li];               // x[m0 + mm + lk][r0 + wr + li]
+            acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc, 0, 0, 0);
         }
         __syncthreads();
-    }
-#pragma unroll
-    for (int b = 0; b < 4; ++b) {
-        const int r = r0 + tr + 16 * b;
-        if (r >= nrec) continue;
-#pragma unroll
-        for (int a = 0; a < 4; ++a) {
-            const int n = n0 + tn + 16 * a;
-            if (n < nt) y[(size_t)r * nt + n] = acc[a][b];
+        if (more) {
+            stage();
+            __syncthreads();
         }
     }
+    const int r = r0 + wr + li;
+    if (r < nrec)
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+            const int n = n0 + wn + 8 * (v >> 2) + 4 * lk + (v & 3);
+            if (n < nt) y[(size_t)r * nt + n] = acc[v];
+        }
 }
 
 // INTRPL (akima433.f90:6-170) at one abscissa uk for the samples y(1..L) at

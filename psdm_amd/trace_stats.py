@@ -21,7 +21,8 @@ from collections import defaultdict
 
 
 def _short(name: str) -> str:
-    return name.split("(")[0].replace("void ", "")
+    """Kernel name without its argument list ("(anonymous namespace)::" dropped first)."""
+    return name.replace("(anonymous namespace)::", "").split("(")[0].replace("void ", "")
 
 
 def union_ns(intervals) -> int:

@@ -1,7 +1,7 @@
 """The rocprofv3 CSV summariser used for the kernel-trace and counter figures."""
 import csv
 
-from psdm_amd.trace_stats import main, union_ns
+from psdm_amd.trace_stats import _short, main, union_ns
 
 
 def test_union_of_intervals():
@@ -23,3 +23,9 @@ def test_trace_busy_time_per_step_of_concurrent_dispatches(tmp_path, capsys):
     out = capsys.readouterr().out
     assert "4 dispatches = 2 steps" in out and "busy time per step 130 ns" in out
     assert "| `psdm::rtm_advance_kernel` | 1 | 10 |" in out
+
+
+def test_kernel_names_without_arguments():
+    assert _short("void psdm::rtm_batch_kernel<16, 0>(psdm::BatchArgs, x)") == "psdm::rtm_batch_kernel<16, 0>"
+    assert _short("psdm::(anonymous namespace)::cond_toeplitz_kernel(float const*, int)") == \
+        "psdm::cond_toeplitz_kernel"
