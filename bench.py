@@ -414,7 +414,7 @@ def bench_condition(args):
     flops = 2.0 * nt_data * nt_data * nrec
     ach = flops / (f_ms * 1e-3) / 1e12
     rbytes = 4.0 * nrec * (nt_data + nt_prop2)
-    acc = condition_accuracy(ctx)
+    acc = None if args.no_accuracy else condition_accuracy(ctx)
     tpath = os.path.join(ROOT, "profiles", "traffic_cond_toeplitz_kernel_mcs.md")
     traffic = None
     if os.path.exists(tpath):

@@ -76,10 +76,16 @@ __global__ __launch_bounds__(256) void cond_taps_kernel(const double *reH, const
 constexpr int kTN = 64, kTR = 64, kTM = 64, kTG = kTN + kTM - 1;
 typedef float v16f __attribute__((ext_vector_type(16)));
 __global__ __launch_bounds__(256) void cond_toeplitz_kernel(const float *__restrict__ x, int nt, int nrec,
-                                                            const float *__restrict__ g, float *__restrict__ y) {
+                                                            const float *__restrict__ g, float *__restrict__ y,
+                                                            int n_ntile, int n_tiles, int per_xcd) {
     __shared__ float xs[kTM][kTR + 1];
     __shared__ float gs[kTG];
-    const int n0 = blockIdx.x * kTN, r0 = blockIdx.y * kTR;
+    // XCD-aware order: the 8 XCDs take workgroups round-robin, so XCD b % 8 gets the
+    // contiguous tile range [(b % 8) * per_xcd, ...), tiles trace-block-major: all sample
+    // tiles of a trace block run on one XCD and read its traces through one L2
+    const int t = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
+    if (t >= n_tiles) return;
+    const int n0 = (t % n_ntile) * kTN, r0 = (t / n_ntile) * kTR;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int wn = 32 * (w & 1), wr = 32 * (w >> 1);  // this wavefront's 32 x 32 sub-tile
     const int li = lane & 31, lk = lane >> 5;
@@ -375,9 +381,11 @@ extern "C" int psdm_rtm_condition(psdm_rtm_ctx *c, const psdm_rtm_cond *q, int N
         }
         void *vf;
         if (int rc = get_buf(c, "cond_filtered", (size_t)nt * NumRec * sizeof(float), &vf)) return rc;
-        hipExtLaunchKernelGGL(cond_toeplitz_kernel, dim3((nt + kTN - 1) / kTN, (NumRec + kTR - 1) / kTR), dim3(256),
-                              0, c->stream, timed ? c->cond_ev[0] : nullptr, timed ? c->cond_ev[1] : nullptr, 0,
-                              d_field, nt, NumRec, (const float *)vt, (float *)vf);
+        const int n_ntile = (nt + kTN - 1) / kTN, n_tiles = n_ntile * ((NumRec + kTR - 1) / kTR);
+        const int per_xcd = (n_tiles + 7) / 8;
+        hipExtLaunchKernelGGL(cond_toeplitz_kernel, dim3(8 * per_xcd), dim3(256), 0, c->stream,
+                              timed ? c->cond_ev[0] : nullptr, timed ? c->cond_ev[1] : nullptr, 0, d_field, nt, NumRec,
+                              (const float *)vt, (float *)vf, n_ntile, n_tiles, per_xcd);
         HIP_TRY(c, hipGetLastError());
         src = (const float *)vf;
     }
