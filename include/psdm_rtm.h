@@ -229,6 +229,11 @@ typedef struct {
 
 int psdm_rtm_condition(psdm_rtm_ctx *ctx, const psdm_rtm_cond *cond, int NumRec, const float *d_field,
                        float *d_target);
+/* The same on host arrays (the driver's Field_Data / Data_target, calculate_PSDM_RTM.f90:
+ * 294-358), staged through the context's device buffers: for callers whose gathers live
+ * in host memory (the Fortran driver through psdm_condition_gather). */
+int psdm_rtm_condition_host(psdm_rtm_ctx *ctx, const psdm_rtm_cond *cond, int NumRec, const float *Field_Data,
+                            float *Data_target);
 /* With psdm_rtm_set_timing(ctx, n != 0): mean device time per call of the filter kernel
  * and of the resample/cut/taper kernel (start/stop stamps of their dispatches), and the
  * number of calls timed since timing was last set. */

@@ -60,22 +60,26 @@ FLANG = "/opt/rocm/bin/amdflang"
 FORTRAN_SHIM = os.path.join(CSRC, "psdm_rtm_fortran.f90")
 CALLER_DIR = os.path.join(ROOT, "tests", "fortran")
 CALLER = os.path.join(CALLER_DIR, "_build", "rtm_caller")
+COND_CALLER = os.path.join(CALLER_DIR, "_build", "cond_caller")
 
 
 def build_fortran_caller(verbose: bool = False):
     """Compiles the Fortran shim against the test fixture module mod_parfile and links
-    the test caller (tests/fortran/rtm_caller.f90) with libpsdm_rtm.so.  Returns the
-    executable path, or None where no Fortran compiler is installed."""
+    the test callers (tests/fortran/rtm_caller.f90: the solver calls of one shot;
+    tests/fortran/cond_caller.f90: the gather conditioning) with libpsdm_rtm.so.
+    Returns the solver caller's path, or None where no Fortran compiler is installed."""
     if not os.path.exists(FLANG):
         return None
     bdir = os.path.join(CALLER_DIR, "_build")
     os.makedirs(bdir, exist_ok=True)
-    srcs = [os.path.join(CALLER_DIR, "mod_parfile_test.f90"), FORTRAN_SHIM, os.path.join(CALLER_DIR, "rtm_caller.f90")]
     rpath = "$ORIGIN/" + os.path.relpath(PKG, bdir)
-    cmd = [FLANG, "-O2", "-J", bdir, *srcs, "-L", PKG, "-lpsdm_rtm", f"-Wl,-rpath,{rpath}", "-o", CALLER]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    subprocess.run(cmd, check=True, cwd=bdir)
+    common = [os.path.join(CALLER_DIR, "mod_parfile_test.f90"), FORTRAN_SHIM]
+    for main, exe in (("rtm_caller.f90", CALLER), ("cond_caller.f90", COND_CALLER)):
+        cmd = [FLANG, "-O2", "-J", bdir, *common, os.path.join(CALLER_DIR, main), "-L", PKG, "-lpsdm_rtm",
+               f"-Wl,-rpath,{rpath}", "-o", exe]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True, cwd=bdir)
     return CALLER
 
 

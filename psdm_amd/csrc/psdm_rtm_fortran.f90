@@ -16,6 +16,18 @@ module psdm_rtm_iface
   implicit none
   type(c_ptr), save :: psdm_ctx = c_null_ptr
 
+  ! psdm_rtm_cond of include/psdm_rtm.h (gather conditioning)
+  type, bind(C) :: psdm_rtm_cond
+    integer(c_int) :: nt_data
+    real(c_float) :: dt_data
+    integer(c_int) :: typef
+    real(c_float) :: f_init, f_final
+    integer(c_int) :: nt_prop2
+    real(c_float) :: dt_prop
+    integer(c_int) :: nt_taper, taper_option, N_taper, N_zeros
+    type(c_ptr) :: nt_sf, nt_cut
+  end type
+
   interface
     integer(c_int) function psdm_rtm_create(ctx, device) bind(C, name="psdm_rtm_create")
       import :: c_ptr, c_int
@@ -63,6 +75,14 @@ module psdm_rtm_iface
       integer(c_int) :: nxSou(*), nySou(*), nxRec(*), nyRec(*)
       real(c_float), value :: dmodel, dt
       real(c_float) :: Sadj(*), ctmp(*), kernel(*)
+    end function
+    integer(c_int) function psdm_rtm_condition_host(ctx, cond, NumRec, Field_Data, Data_target) &
+        bind(C, name="psdm_rtm_condition_host")
+      import :: c_ptr, c_int, c_float, psdm_rtm_cond
+      type(c_ptr), value :: ctx
+      type(psdm_rtm_cond) :: cond
+      integer(c_int), value :: NumRec
+      real(c_float) :: Field_Data(*), Data_target(*)
     end function
   end interface
 
@@ -169,4 +189,39 @@ store_snap,nt_snap,nt,kernel)
   character(len=500) :: folder_fields
   call psdm_check(psdm_rtm_backward_disk(psdm_context(), dPML, fs, NumSou, NumRec, nxSou, nySou, nxRec, nyRec, Sadj, &
                   dmodel, dt, ny, nx, ctmp, shotid, store_snap, nt_snap, nt, kernel), 'solver_ac_backward_disk')
+end subroutine
+
+
+! Gather conditioning of the driver on the device (calculate_PSDM_RTM.f90:299-358): the
+! frequency filter (when f_init > 0 and f_final > 0), the resampling to the propagation
+! step (or the copy of the first nt_prop2 samples), the cut_sf zeroing and the taper /
+! cut_multiple of the trace ends, with the driver's own variables.  A caller replaces that
+! block with one call.  Field_Data is left as it was (the reference filters it in place;
+! the driver does not read it again for this shot).
+subroutine psdm_condition_gather(Field_Data, nt_data, dt_data, NumRec, typef, f_init, f_final, &
+    Data_target, nt_prop2, dt_prop, nt_taper, taper_option, N_taper, N_zeros, cut_sf, nt_sf, cut_multiple, nt_cut)
+  use psdm_rtm_iface
+  implicit none
+  integer :: nt_data, NumRec, typef, nt_prop2, nt_taper, taper_option, N_taper, N_zeros, cut_sf, cut_multiple
+  real :: dt_data, f_init, f_final, dt_prop
+  real :: Field_Data(nt_data, NumRec), Data_target(nt_taper, NumRec)
+  integer(c_int), target :: nt_sf(NumRec), nt_cut(NumRec)
+  type(psdm_rtm_cond) :: q
+  q%nt_data = nt_data
+  q%dt_data = dt_data
+  q%typef = typef
+  q%f_init = f_init
+  q%f_final = f_final
+  q%nt_prop2 = nt_prop2
+  q%dt_prop = dt_prop
+  q%nt_taper = nt_taper
+  q%taper_option = taper_option
+  q%N_taper = N_taper
+  q%N_zeros = N_zeros
+  q%nt_sf = c_null_ptr
+  q%nt_cut = c_null_ptr
+  if (cut_sf > 0) q%nt_sf = c_loc(nt_sf)
+  if (cut_multiple > 0) q%nt_cut = c_loc(nt_cut)
+  call psdm_check(psdm_rtm_condition_host(psdm_context(), q, NumRec, Field_Data, Data_target), &
+                  'psdm_condition_gather')
 end subroutine

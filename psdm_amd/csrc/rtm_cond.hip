@@ -424,6 +424,23 @@ extern "C" int psdm_rtm_condition(psdm_rtm_ctx *c, const psdm_rtm_cond *q, int N
     return 0;
 }
 
+extern "C" int psdm_rtm_condition_host(psdm_rtm_ctx *c, const psdm_rtm_cond *q, int NumRec, const float *field,
+                                       float *target) {
+    if (!c) return PSDM_RTM_EINVAL;
+    if (!q || !field || !target) return fail(c, PSDM_RTM_EINVAL, "condition: null argument");
+    if (NumRec < 1 || q->nt_data < 1 || q->nt_taper < 1) return fail(c, PSDM_RTM_EINVAL, "condition: empty gather");
+    if (int rc = set_device(c)) return rc;
+    const size_t nin = (size_t)q->nt_data * NumRec, nout = (size_t)q->nt_taper * NumRec;
+    void *vi, *vo;
+    if (int rc = get_buf(c, "cond_in", nin * sizeof(float), &vi)) return rc;
+    if (int rc = get_buf(c, "cond_out", nout * sizeof(float), &vo)) return rc;
+    HIP_TRY(c, hipMemcpyAsync(vi, field, nin * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    if (int rc = psdm_rtm_condition(c, q, NumRec, (const float *)vi, (float *)vo)) return rc;
+    HIP_TRY(c, hipMemcpyAsync(target, vo, nout * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
 extern "C" int psdm_rtm_condition_timing(psdm_rtm_ctx *c, double *filter_ms, double *resample_ms, long long *n) {
     if (!c) return PSDM_RTM_EINVAL;
     if (filter_ms) *filter_ms = c->cond_n[0] ? c->cond_ms[0] / (double)c->cond_n[0] : 0.0;

@@ -84,3 +84,40 @@ def test_fortran_caller_through_shim_matches_oracle(name):
                     c["fs"], Fields=Fo, variant=c["variant"], table_mode=0)
     np.testing.assert_array_equal(Rec, Ro)
     np.testing.assert_array_equal(kernel, ko)
+
+
+@pytest.mark.gpu
+def test_fortran_conditioning_binding_matches_reference():
+    """The driver's conditioning block as one Fortran call (psdm_condition_gather) on the
+    golden cases written from the reference's own routines."""
+    from psdm_amd.build import COND_CALLER
+
+    assert os.path.exists(COND_CALLER), "tests/fortran/_build/cond_caller is built by __graft_entry__.build()"
+    z = np.load(os.path.join(ROOT, "tests", "golden", "conditioning_ref.npz"))
+    for name in [k[:-4] for k in z.files if k.endswith("_raw")]:
+        print(f"[gpu] fortran conditioning case={name}", flush=True)
+        p = dict(zip(z["param_names"], z[name + "_params"]))
+        raw = np.asfortranarray(z[name + "_raw"])
+        nt_data, n = raw.shape
+        nt_taper = int(p["nt_taper"])
+        sf = z[name + "_nt_sf"] if name + "_nt_sf" in z.files else np.zeros(n, np.int32)
+        cut = z[name + "_nt_cut"] if name + "_nt_cut" in z.files else np.zeros(n, np.int32)
+        tmp = tempfile.mkdtemp()
+        try:
+            with open(os.path.join(tmp, "in.bin"), "wb") as f:
+                np.array([nt_data, n, int(p["typef"]), int(p["nt_prop2"]), nt_taper, int(p["taper_option"]),
+                          int(p["N_taper"]), int(p["N_zeros"]), int(name + "_nt_sf" in z.files),
+                          int(name + "_nt_cut" in z.files)], np.int32).tofile(f)
+                np.array([p["dt_data"], p["f_init"], p["f_final"], p["dt_prop"]], np.float32).tofile(f)
+                sf.astype(np.int32).tofile(f)
+                cut.astype(np.int32).tofile(f)
+                raw.astype(np.float32).ravel(order="F").tofile(f)
+            subprocess.run([COND_CALLER, "in.bin", "out.bin"], cwd=tmp, check=True, timeout=120)
+            out = np.fromfile(os.path.join(tmp, "out.bin"), np.float32).reshape((nt_taper, n), order="F")
+        finally:
+            shutil.rmtree(tmp, ignore_errors=True)
+        ref = z[name + "_out"]
+        if float(p["f_init"]) > 0:
+            assert np.linalg.norm(out - ref) / np.linalg.norm(ref) < 1e-5, name
+        else:
+            np.testing.assert_array_equal(out, ref, err_msg=name)
