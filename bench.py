@@ -473,6 +473,87 @@ def condition_accuracy(ctx):
     return res
 
 
+def bench_poststack(args):
+    """The steps after the stack on the device: normalize (PSDM = PSDM_sum / Cov_sum,
+    auxiliar_PSDM.f90:1-20) and the depth-to-two-way-time conversion of the image and the
+    velocity (psdm2twt, :23-91) for the 2001 x 601 @ 10 m line (TWT step 4 ms).  `--steps`
+    conversions timed; one JSON line.  CPU baseline: the reference's linear_interpolation_1D
+    (oracle/_ref) over a share of the columns per process."""
+    import numpy as np
+
+    nz, nx, dd, dt = 601, 2001, 10.0, 0.004
+    cpu = None
+    if not args.no_cpu:
+        from oracle import cpu_baseline
+
+        per_rank = 125
+        r = cpu_baseline.run_twt(nz, per_rank, dd, dt, args.cpu_procs)
+        cpu = {"value": round(r["cols_per_s"] / nx, 3), "unit": "images/s", "cores": r["cores"], "kind": r["kind"],
+               "sample": f"{r['cores']} processes, each converting {per_rank} columns of {nz} depth samples "
+                         f"(two-way time accumulation, then linear_interpolation_1D of velocity and image onto "
+                         f"a {dt * 1e3:g} ms axis); images/s = columns/s / {nx} (slowest {r['rank_s']:.2f} s)"}
+        log(f"[bench] poststack cpu baseline {cpu['value']} images/s")
+
+    import torch
+
+    from psdm_amd import poststack as PS
+    from psdm_amd._native import Context
+    from psdm_amd.workloads import line_workload
+
+    dev = torch.device("cuda:0")
+    ctx = Context(0)
+    m = line_workload("mcs2001")["model"]
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(0)
+    d_vel = torch.from_numpy(np.ascontiguousarray(m.migration.T)).to(dev)  # (nx, nz)
+    d_sum = torch.randn((nx, nz), generator=gen, device=dev, dtype=torch.float64)
+    d_fold = torch.randint(0, 40, (nx, nz), generator=gen, device=dev, dtype=torch.int32)
+    d_img = torch.empty((nx, nz), device=dev)
+
+    def one():
+        import ctypes as C
+
+        ctx.check(ctx._L.psdm_rtm_normalize(ctx.h, C.c_void_p(d_sum.data_ptr()), C.c_void_p(d_fold.data_ptr()), nz,
+                                            nx, C.c_void_p(d_img.data_ptr())), "psdm_rtm_normalize")
+        return PS.psdm2twt(ctx, d_vel, d_img, dd, dt)
+
+    vt, pt = one()
+    nt = vt.shape[1]
+    ctx.set_timing(1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        one()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    a_ms, b_ms, n = PS.timing(ctx)
+    ctx.set_timing(0)
+    # dominant kernel: the time accumulation (reads the velocity, writes the times); the
+    # interpolation reads the time, velocity and image columns and writes both outputs
+    byts = 4.0 * (2 * nz * nx)
+    ach = byts / (a_ms * 1e-3) / 1e9
+    ibytes = 4.0 * (3 * nz * nx + 2 * nt * nx)
+    out = {
+        "metric": "post-stack images/s (normalize + depth-to-TWT of image and velocity)",
+        "value": round(args.steps / elapsed, 2), "unit": "images/s", "n_gpus": 1, "steps": args.steps, "warmup": 1,
+        "ms_per_step": round(1e3 * elapsed / args.steps, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (mcs2001 migration velocity; seeded FP64 stack and fold)",
+        "config": {"workload": f"{nx} x {nz} image @ {dd:g} m -> {nt} TWT samples @ {dt * 1e3:g} ms"},
+        "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "twt_times_kernel",
+                     "mean_kernel_ms": round(a_ms, 5), "timed_launches": n, "algorithmic_bytes_per_launch": byts,
+                     "note": "each column's times are an FP32 running sum in depth order (the reference's order, "
+                             "bit for bit): a serial chain of nz dependent additions per column, latency bound"},
+        "interp_kernel": {"kernel": "twt_interp_kernel", "mean_kernel_ms": round(b_ms, 5),
+                          "algorithmic_bytes": ibytes, "achieved_gbs": round(ibytes / (b_ms * 1e-3) / 1e9, 1)},
+        "cpu_baseline": cpu,
+        "speedup_vs_cpu": round(args.steps / elapsed / cpu["value"], 1) if cpu else None,
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
 def bench_served(args):
     """Served stream: Poisson arrivals at 4 rates; 70% MCS gathers (480 ch @12.5 m, 8 km
     aperture, 601-deep model @10 m, nt 6000, FP32 snapshots every step) and 30% WAS
@@ -579,7 +660,7 @@ def main():
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="batched form with the default launch shape")
     ap.add_argument("--forms", default="plain,batched")
-    ap.add_argument("--workload", default="mcs2001", choices=["mcs2001", "uneven", "mixed", "served", "condition"],
+    ap.add_argument("--workload", default="mcs2001", choices=["mcs2001", "uneven", "mixed", "served", "condition", "poststack"],
                     help="mcs2001: the 64-shot line (default line); uneven: 256 uneven shots, batched vs plain; "
                          "mixed: uneven + FP16/BF16 snapshots every 4th step; served: Poisson-arrival stream")
     ap.add_argument("--shots", type=int, default=256, help="uneven/mixed: shots of the 256-shot line used")
@@ -601,6 +682,8 @@ def main():
         return bench_served(args)
     if args.workload == "condition":
         return bench_condition(args)
+    if args.workload == "poststack":
+        return bench_poststack(args)
     forms = set(args.forms.split(","))
 
     from psdm_amd.workloads import line_workload, step_bytes, updates_per_shot

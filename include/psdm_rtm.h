@@ -239,6 +239,21 @@ int psdm_rtm_condition_host(psdm_rtm_ctx *ctx, const psdm_rtm_cond *cond, int Nu
  * number of calls timed since timing was last set. */
 int psdm_rtm_condition_timing(psdm_rtm_ctx *ctx, double *filter_ms, double *resample_ms, long long *n_calls);
 
+/* ------------------------------------------------------------------------------
+ * Depth-to-two-way-time conversion of the final image: psdm2twt (auxiliar_PSDM.f90:
+ * 23-91) without its file writers.  d_vel, d_psdm: device (nz, nx) at depth step dd;
+ * per column the two-way time t(k) = t(k-1) + 2*dd/v(k) (FP32, depth order), then
+ * nt = 1 + ceiling(max|t| / dt) and d_vel_twt, d_psdm_twt (nt, nx) device: the velocity
+ * and the image at t = (j-1)*dt by linear_interpolation_1D (linear_interpolation.f90:
+ * 72-103), bit for bit.  With d_vel_twt or d_psdm_twt NULL the call only returns nt;
+ * otherwise nt_cap (the outputs' rows) must be >= nt.  Synchronous.
+ * ------------------------------------------------------------------------------ */
+int psdm_rtm_psdm2twt(psdm_rtm_ctx *ctx, int nx, int nz, float dd, float dt, const float *d_vel,
+                      const float *d_psdm, int nt_cap, float *d_vel_twt, float *d_psdm_twt, int *nt);
+/* With psdm_rtm_set_timing(ctx, n != 0): mean device time per full conversion of the
+ * time-accumulation kernel and of the interpolation kernel, and the conversions timed. */
+int psdm_rtm_psdm2twt_timing(psdm_rtm_ctx *ctx, double *times_ms, double *interp_ms, long long *n_calls);
+
 #ifdef __cplusplus
 }
 #endif

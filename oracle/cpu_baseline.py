@@ -90,3 +90,40 @@ def run_conditioning(nt_data: int, dt_data: float, nrec: int, nt_prop2: int, dt_
     busy = max(per)
     return dict(kind="reference", cores=nprocs, traces_per_rank=nrec, wall_s=wall, rank_s=busy,
                 traces_per_s=nprocs * nrec / busy)
+
+
+def _one_rank_twt(args):
+    nz, ncols, dd, dt, seed = args
+    import sys
+
+    here = os.path.dirname(os.path.abspath(__file__))
+    sys.path.insert(0, os.path.dirname(here))
+    from oracle import poststack as P
+
+    rng = np.random.default_rng(seed)
+    z = np.arange(nz)[:, None] * dd
+    vel = np.where(z < 1500.0, 1500.0, 1800.0 + 0.3 * (z - 1500.0)).astype(np.float32) * np.ones((1, ncols), np.float32)
+    img = rng.standard_normal((nz, ncols)).astype(np.float32)
+    t0 = time.perf_counter()
+    t = P.twt_times(vel, dd)
+    nt = P.twt_length(t, dt)
+    time_axis = np.arange(nt).astype(np.float32) * np.float32(dt)
+    for i in range(ncols):
+        P.ref_linear_interpolation_1d(t[:, i], vel[:, i], time_axis)
+        P.ref_linear_interpolation_1d(t[:, i], img[:, i], time_axis)
+    return time.perf_counter() - t0
+
+
+def run_twt(nz: int, ncols_per_rank: int, dd: float, dt: float, nprocs: int):
+    """Depth-to-time conversion with the reference's linear_interpolation_1D (oracle/_ref),
+    `ncols_per_rank` image columns per process; returns columns per second."""
+    from oracle import pyoracle as O
+
+    if not O.ref_available():
+        raise RuntimeError("oracle/_ref/libpsdm_ref.so is not built")
+    ctx = mp.get_context("spawn")
+    with ctx.Pool(nprocs) as pool:
+        per = pool.map(_one_rank_twt, [(nz, ncols_per_rank, dd, dt, s) for s in range(nprocs)])
+    busy = max(per)
+    return dict(kind="reference", cores=nprocs, cols_per_rank=ncols_per_rank, rank_s=busy,
+                cols_per_s=nprocs * ncols_per_rank / busy)

@@ -174,4 +174,14 @@ contains
     call taper_pad_trace(trace, a, b, c, d)
   end subroutine
 
+  ! The linear interpolation of the depth-to-time conversion (psdm2twt,
+  ! auxiliar_PSDM.f90:23-91 -> linear_interpolation_1D, linear_interpolation.f90:72-103).
+  subroutine ref_linear_interpolation_1d(x1, y1, n1, x2, y2, n2) bind(C, name="ref_linear_interpolation_1d")
+    integer(c_int), value :: n1, n2
+    real(c_float) :: x1(n1), y1(n1), x2(n2), y2(n2)
+    integer :: a, b
+    a = n1; b = n2
+    call linear_interpolation_1D(x1, y1, a, x2, y2, b)
+  end subroutine
+
 end module ref_harness

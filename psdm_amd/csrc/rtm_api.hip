@@ -551,6 +551,8 @@ int psdm_rtm_destroy(psdm_rtm_ctx *c) {
     for (auto e : c->ev_free) (void)hipEventDestroy(e);
     for (auto e : c->cond_ev)
         if (e) (void)hipEventDestroy(e);
+    for (auto e : c->twt_ev)
+        if (e) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(c->stream);
     delete c;
     return 0;
@@ -678,6 +680,8 @@ int psdm_rtm_set_timing(psdm_rtm_ctx *c, int timing) {
     c->sum_ms = 0.0;
     c->cond_ms[0] = c->cond_ms[1] = 0.0;
     c->cond_n[0] = c->cond_n[1] = 0;
+    c->twt_ms[0] = c->twt_ms[1] = 0.0;
+    c->twt_n = 0;
     return 0;
 }
 

@@ -48,6 +48,10 @@ struct psdm_rtm_ctx {
     hipEvent_t cond_ev[4] = {};
     double cond_ms[2] = {0.0, 0.0};
     long long cond_n[2] = {0, 0};
+    // depth-to-time conversion: device time of its two kernels per call (timing on)
+    hipEvent_t twt_ev[4] = {};
+    double twt_ms[2] = {0.0, 0.0};
+    long long twt_n = 0;
 };
 
 namespace psdm {

@@ -1,0 +1,87 @@
+"""GPU parity of the depth-to-time conversion (psdm_rtm_psdm2twt) with the oracle
+(oracle/poststack.py, itself pinned to the reference's linear_interpolation_1D): bit for
+bit, NaN included, on a water-over-gradient model and on columns with zero and negative
+velocities."""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import poststack as P
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def device_twt(ctx, vel, psdm, dd, dt):
+    import torch
+
+    from psdm_amd.poststack import psdm2twt
+
+    dv = torch.from_numpy(np.ascontiguousarray(vel.T)).to("cuda:0")
+    dp = torch.from_numpy(np.ascontiguousarray(psdm.T)).to("cuda:0")
+    vt, pt = psdm2twt(ctx, dv, dp, dd, dt)
+    torch.cuda.synchronize()
+    return vt.cpu().numpy().T, pt.cpu().numpy().T
+
+
+def model(nz, nx, dd, seed):
+    rng = np.random.default_rng(seed)
+    z = np.arange(nz)[:, None] * dd
+    wd = 400.0 + 50.0 * np.sin(np.arange(nx) / 7.0)
+    vel = np.where(z < wd, 1500.0, 1800.0 + 1.2 * (z - wd)).astype(np.float32)
+    psdm = rng.standard_normal((nz, nx)).astype(np.float32)
+    psdm[z[:, 0] < 300.0, :] = 0.0  # no image in the shallow water
+    psdm[:, nx // 3] = 0.0          # a column of zero fold
+    return np.asfortranarray(vel), np.asfortranarray(psdm)
+
+
+@pytest.mark.parametrize("nz,nx,dd,dt", [(300, 48, 10.0, 0.004), (157, 31, 12.5, 0.002)])
+def test_psdm2twt_bit_exact_vs_oracle(ctx, nz, nx, dd, dt):
+    vel, psdm = model(nz, nx, dd, nz)
+    say(f"[gpu] psdm2twt {nz}x{nx} dd={dd} dt={dt}")
+    gv, gp = device_twt(ctx, vel, psdm, dd, dt)
+    ov, op = P.psdm2twt(vel, psdm, dd, dt)
+    assert gv.shape == ov.shape
+    np.testing.assert_array_equal(gv, ov)
+    np.testing.assert_array_equal(gp, op)
+
+
+def test_psdm2twt_zero_and_negative_velocities(ctx):
+    """Zero velocities (flat time runs; image values there give the reference's 0*inf =
+    NaN) and a negative velocity (a non-monotone time column, scanned as the reference)."""
+    nz, nx, dd, dt = 120, 6, 10.0, 0.004
+    vel, psdm = model(nz, nx, dd, 3)
+    vel[40:50, 1] = 0.0
+    vel[60, 2] = -1500.0
+    vel[70:75, 3] = 0.0
+    psdm[70:75, 3] = 0.0
+    say("[gpu] psdm2twt zero / negative velocities")
+    gv, gp = device_twt(ctx, vel, psdm, dd, dt)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        ov, op = P.psdm2twt(vel, psdm, dd, dt)
+    np.testing.assert_array_equal(gv, ov)
+    np.testing.assert_array_equal(gp, op)
+
+
+def test_psdm2twt_length_query_and_errors(ctx):
+    import ctypes as C
+
+    import torch
+
+    from psdm_amd._native import RtmError
+
+    vel, psdm = model(100, 8, 10.0, 1)
+    dv = torch.from_numpy(np.ascontiguousarray(vel.T)).to("cuda:0")
+    nt = C.c_int()
+    args = (ctx.h, 8, 100, 10.0, 0.004, C.c_void_p(dv.data_ptr()), C.c_void_p(dv.data_ptr()))
+    assert ctx._L.psdm_rtm_psdm2twt(*args, 0, None, None, C.byref(nt)) == 0
+    assert nt.value == P.twt_length(P.twt_times(vel, 10.0), 0.004)
+    out = torch.empty((8, 4), device="cuda:0")
+    with pytest.raises(RtmError, match="nt_cap"):
+        ctx.check(ctx._L.psdm_rtm_psdm2twt(*args, 4, C.c_void_p(out.data_ptr()), C.c_void_p(out.data_ptr()),
+                                           C.byref(nt)), "psdm2twt")
