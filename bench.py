@@ -486,7 +486,7 @@ def bench_poststack(args):
     if not args.no_cpu:
         from oracle import cpu_baseline
 
-        per_rank = 125
+        per_rank = 500
         r = cpu_baseline.run_twt(nz, per_rank, dd, dt, args.cpu_procs)
         cpu = {"value": round(r["cols_per_s"] / nx, 3), "unit": "images/s", "cores": r["cores"], "kind": r["kind"],
                "sample": f"{r['cores']} processes, each converting {per_rank} columns of {nz} depth samples "

@@ -246,7 +246,9 @@ int psdm_rtm_condition_timing(psdm_rtm_ctx *ctx, double *filter_ms, double *resa
  * nt = 1 + ceiling(max|t| / dt) and d_vel_twt, d_psdm_twt (nt, nx) device: the velocity
  * and the image at t = (j-1)*dt by linear_interpolation_1D (linear_interpolation.f90:
  * 72-103), bit for bit.  With d_vel_twt or d_psdm_twt NULL the call only returns nt;
- * otherwise nt_cap (the outputs' rows) must be >= nt.  Synchronous.
+ * otherwise nt_cap (the outputs' rows) must be >= nt.  A full call that directly follows
+ * a length query with the same arguments reuses the query's times (d_vel must not change
+ * in between).  Synchronous.
  * ------------------------------------------------------------------------------ */
 int psdm_rtm_psdm2twt(psdm_rtm_ctx *ctx, int nx, int nz, float dd, float dt, const float *d_vel,
                       const float *d_psdm, int nt_cap, float *d_vel_twt, float *d_psdm_twt, int *nt);

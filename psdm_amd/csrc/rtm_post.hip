@@ -17,7 +17,9 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdint>
 #include <cstring>
+#include <vector>
 
 #include "rtm_ctx.hpp"
 
@@ -131,25 +133,44 @@ extern "C" int psdm_rtm_psdm2twt(psdm_rtm_ctx *c, int nx, int nz, float dd, floa
     if (timed)
         for (auto &e : c->twt_ev)
             if (!e) HIP_TRY(c, hipEventCreate(&e));
+    unsigned bd, bt;
+    std::memcpy(&bd, &dd, 4);
+    std::memcpy(&bt, &dt, 4);
+    const std::vector<unsigned long long> key = {(unsigned long long)(uintptr_t)d_vel, (unsigned long long)nx,
+                                                 (unsigned long long)nz, bd, bt};
+    const bool query = !d_vel_twt || !d_psdm_twt;
     void *vt, *vm;
     if (int rc = get_buf(c, "twt_t", (size_t)nz * nx * sizeof(float), &vt)) return rc;
     if (int rc = get_buf(c, "twt_mono", ((size_t)nx + 1) * sizeof(int), &vm)) return rc;
     int *d_mono = (int *)vm;
     unsigned *d_max = (unsigned *)(d_mono + nx);
-    HIP_TRY(c, hipMemsetAsync(d_max, 0, sizeof(unsigned), c->stream));
-    hipExtLaunchKernelGGL(twt_times_kernel, dim3((nx + kTC - 1) / kTC), dim3(256), 0, c->stream,
-                          timed ? c->twt_ev[0] : nullptr, timed ? c->twt_ev[1] : nullptr, 0, d_vel, nz, nx, dd,
-                          (float *)vt, d_mono, d_max);
-    HIP_TRY(c, hipGetLastError());
-    unsigned bits = 0;
-    HIP_TRY(c, hipMemcpyAsync(&bits, d_max, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
-    float tmax;
-    std::memcpy(&tmax, &bits, sizeof(float));
-    const double nd = 1.0 + std::ceil((double)(tmax / dt));  // nt = 1 + ceiling(time_max/dt), quotient in FP32
-    if (!(nd < 2147483647.0)) return fail(c, PSDM_RTM_EINVAL, "psdm2twt: time axis too long (is dt or v wrong?)");
-    *nt = (int)nd;
-    if (!d_vel_twt || !d_psdm_twt) return 0;  // length query
+    float times_ms = 0.f;
+    if (!query && c->twt_key == key) {  // the times of the preceding length query
+        *nt = c->twt_nt;
+        times_ms = c->twt_times_ms;
+    } else {
+        HIP_TRY(c, hipMemsetAsync(d_max, 0, sizeof(unsigned), c->stream));
+        hipExtLaunchKernelGGL(twt_times_kernel, dim3((nx + kTC - 1) / kTC), dim3(256), 0, c->stream,
+                              timed ? c->twt_ev[0] : nullptr, timed ? c->twt_ev[1] : nullptr, 0, d_vel, nz, nx, dd,
+                              (float *)vt, d_mono, d_max);
+        HIP_TRY(c, hipGetLastError());
+        unsigned bits = 0;
+        HIP_TRY(c, hipMemcpyAsync(&bits, d_max, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        if (timed) HIP_TRY(c, hipEventElapsedTime(&times_ms, c->twt_ev[0], c->twt_ev[1]));
+        float tmax;
+        std::memcpy(&tmax, &bits, sizeof(float));
+        const double nd = 1.0 + std::ceil((double)(tmax / dt));  // nt = 1 + ceiling(time_max/dt), quotient in FP32
+        if (!(nd < 2147483647.0)) return fail(c, PSDM_RTM_EINVAL, "psdm2twt: time axis too long (is dt or v wrong?)");
+        *nt = (int)nd;
+    }
+    c->twt_key.clear();
+    if (query) {  // keep the times for the full call that follows
+        c->twt_key = key;
+        c->twt_nt = *nt;
+        c->twt_times_ms = times_ms;
+        return 0;
+    }
     if (*nt > nt_cap) return fail(c, PSDM_RTM_EINVAL, "psdm2twt: outputs need nt rows (nt_cap too small)");
     hipExtLaunchKernelGGL(twt_interp_kernel, dim3((*nt + 255) / 256, nx), dim3(256), 0, c->stream,
                           timed ? c->twt_ev[2] : nullptr, timed ? c->twt_ev[3] : nullptr, 0, (const float *)vt,
@@ -157,10 +178,9 @@ extern "C" int psdm_rtm_psdm2twt(psdm_rtm_ctx *c, int nx, int nz, float dd, floa
     HIP_TRY(c, hipGetLastError());
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     if (timed) {
-        float a, b;
-        HIP_TRY(c, hipEventElapsedTime(&a, c->twt_ev[0], c->twt_ev[1]));
+        float b;
         HIP_TRY(c, hipEventElapsedTime(&b, c->twt_ev[2], c->twt_ev[3]));
-        c->twt_ms[0] += a;
+        c->twt_ms[0] += times_ms;
         c->twt_ms[1] += b;
         c->twt_n++;
     }
