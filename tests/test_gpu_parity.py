@@ -130,6 +130,21 @@ def test_gpu_large_window_against_oracle(ctx):
     np.testing.assert_array_equal(kg, ko)
 
 
+def test_gpu_wide_window_against_oracle(ctx):
+    """An 801 x 5102 window (the width of the wide-angle OBS gathers of the served stream,
+    dPML 40, 510 receivers): traces and image bit for bit with the snapshots resident in
+    HBM between the forward and backward calls."""
+    c = case("ram_fs0", ny=801, nx=5102, nt=40, dPML=40, dx=10.0, dt=0.001, src=(2551, 3), rec_x=np.arange(11, 5102, 10),
+             rec_y=2, f0=40.0, vmin=1500.0, vmax=4500.0)
+    say("[gpu] wide window 801x5102 nt=40")
+    ctx.set_launch(tile_x=8)
+    Rg, _, kg = gpu_rtm(ctx, c, host_fields=False)
+    Ro, _, ko = oracle_rtm(c)
+    np.testing.assert_array_equal(Rg, Ro)
+    np.testing.assert_array_equal(kg, ko)
+    assert np.linalg.norm(ko) > 0
+
+
 # Interior layouts the small golden cases do not reach: two and three 248-row stream tiles
 # (the last one moved up over its neighbour's rows, ny not a multiple of 4), windows
 # narrower and wider than a stream item, sources and receivers in the rows two stream
