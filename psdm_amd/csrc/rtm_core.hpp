@@ -204,6 +204,13 @@ struct SlotDesc {
     float *pA, *pB, *phxA, *phxB, *phyA, *phyB, *hsA, *hsB;
     long long g0;  // global step of the slot's first forward step (k = 2); < 0: slot idle
     int nt, ss, nsnap;
+    // the slot's range [woff, woff + witems) in its launch group's compact work list (the
+    // work items of the group's occupied slots back to back; witems = 0 for an idle slot).
+    // Set in a group's first slot only: wtotal, the list's length; wuni, the items of every
+    // slot when all of the group's slots hold that many (> 0), else 0; wfd, division by
+    // wuni, or else by the mean items per slot (the first guess of an item's slot)
+    int woff, witems, wtotal, wuni;
+    FastDiv wfd;
     FastDiv fd_ss;  // division by ss
     const int *ev_ptr_f, *ev_cell_f, *ev_inj_f;
     const float *inj_f;
@@ -213,15 +220,19 @@ struct SlotDesc {
     int ninj_b;
 };
 
+// One launch group: slots[0 .. nslots) (their SlotDesc woff / witems give the group's
+// compact work list); the grid is sized on the host for every slot at capacity.
 struct BatchArgs {
     const SlotDesc *slots;
     int nslots;
-    int cap_tiles;           // work items (workgroups) per slot: nitems of the slot capacity
-    FastDiv fd_cap;          // division by cap_tiles
-    int nwork;               // nslots * cap_tiles
-    int per_xcd;             // work items per XCD
     const long long *gbase;  // global step of the chunk's first step
     int goff;                // this launch's offset inside the chunk
+    // compact = 0: slot s owns items [s*cap_tiles, (s+1)*cap_tiles) (the work items of the
+    // slot capacity), exact when every slot holds a full-capacity window and read from the
+    // launch arguments alone; compact = 1: the group's compact work list (SlotDesc woff)
+    int compact;
+    int cap_tiles;
+    FastDiv fd_cap;  // division by cap_tiles
 };
 
 void launch_step(const StepArgs &a, int tile_x, hipStream_t s, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);

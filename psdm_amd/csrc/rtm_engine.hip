@@ -132,8 +132,11 @@ struct psdm_rtm_engine {
     std::deque<Job> pending;
     Arena arena;
     long long G = 0, next_id = 1, chunks = 0;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
+    // one graph per work mapping (BatchArgs compact = 0, 1); compact: the mapping of the
+    // uploaded slots (0 while every slot holds a full-capacity window)
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    int compact = 1;
     static constexpr int kRing = 256;
     hipEvent_t ev_chunk[kRing] = {};   // recorded after each chunk
     hipEvent_t ev_cstart[kRing] = {};  // recorded before each chunk (device time of the chunks)
@@ -154,24 +157,24 @@ int efail(psdm_rtm_engine *e, int code, const std::string &m) { return fail(e->c
 
 // Launches one chunk of time steps (graph replay or eager launches).
 // Launch arguments of slot group q (slots [q*n0, min((q+1)*n0, nslots)), n0 = ceil(nslots/groups)).
-BatchArgs group_args(const psdm_rtm_engine *e, int q, int goff, int *grid) {
+BatchArgs group_args(const psdm_rtm_engine *e, int q, int goff, int compact, int *grid) {
     const int n0 = cdiv(e->cfg.nslots, e->groups);
     const int s0 = q * n0, n = std::min(n0, e->cfg.nslots - s0);
     const int nwork = n * e->cap_tiles, per_xcd = cdiv(nwork, 8);
     *grid = e->cfg.grid > 0 ? std::min(e->cfg.grid, per_xcd * 8) : per_xcd * 8;
-    return BatchArgs{e->d_slots + s0, n, e->cap_tiles, make_fastdiv(e->cap_tiles), nwork, per_xcd, e->d_gbase, goff};
+    return BatchArgs{e->d_slots + s0, n, e->d_gbase, goff, compact, e->cap_tiles, make_fastdiv(e->cap_tiles)};
 }
 
 int issue_chunk(psdm_rtm_engine *e) {
     psdm_rtm_ctx *c = e->c;
     const int C = e->cfg.chunk_steps;
-    if (e->exec) {
-        HIP_TRY(c, hipGraphLaunch(e->exec, c->stream));
+    if (e->exec[e->compact]) {
+        HIP_TRY(c, hipGraphLaunch(e->exec[e->compact], c->stream));
     } else {
         for (int g = 0; g < C; g++)
             for (int q = 0; q < e->groups; q++) {
                 int grid;
-                const BatchArgs b = group_args(e, q, g, &grid);
+                const BatchArgs b = group_args(e, q, g, e->compact, &grid);
                 launch_batch(b, e->cfg.tile_x, snap_variant(e->cfg.snap_format, e->cfg.kahan), grid, c->stream);
             }
         launch_advance(e->d_gbase, C, c->stream);
@@ -180,7 +183,7 @@ int issue_chunk(psdm_rtm_engine *e) {
     return 0;
 }
 
-int capture_graph(psdm_rtm_engine *e) {
+int capture_graph(psdm_rtm_engine *e, int compact) {
     psdm_rtm_ctx *c = e->c;
     HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
     if (e->groups > 1) {  // the other chains fork off the capture stream and join before the advance
@@ -190,7 +193,7 @@ int capture_graph(psdm_rtm_engine *e) {
     for (int g = 0; g < e->cfg.chunk_steps; g++)
         for (int q = 0; q < e->groups; q++) {
             int grid;
-            const BatchArgs b = group_args(e, q, g, &grid);
+            const BatchArgs b = group_args(e, q, g, compact, &grid);
             launch_batch(b, e->cfg.tile_x, snap_variant(e->cfg.snap_format, e->cfg.kahan), grid,
                          q == 0 ? c->stream : e->side[q - 1]);
         }
@@ -199,8 +202,8 @@ int capture_graph(psdm_rtm_engine *e) {
         HIP_TRY(c, hipStreamWaitEvent(c->stream, e->ev_join[q - 1], 0));
     }
     launch_advance(e->d_gbase, e->cfg.chunk_steps, c->stream);
-    HIP_TRY(c, hipStreamEndCapture(c->stream, &e->graph));
-    HIP_TRY(c, hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0));
+    HIP_TRY(c, hipStreamEndCapture(c->stream, &e->graph[compact]));
+    HIP_TRY(c, hipGraphInstantiate(&e->exec[compact], e->graph[compact], nullptr, nullptr, 0));
     return 0;
 }
 
@@ -381,6 +384,23 @@ int upload_slots(psdm_rtm_engine *e) {
     if (!e->slots_dirty) return 0;
     const int S = e->cfg.nslots;
     SlotDesc *src = e->h_slots + e->h_ring * S;
+    const int n0 = cdiv(S, e->groups);  // compact work list of each launch group (group_args)
+    e->compact = 0;
+    for (int s0 = 0; s0 < S; s0 += n0) {
+        int run = 0;
+        const int n = std::min(S, s0 + n0) - s0;
+        bool uni = true;
+        for (int s = s0; s < s0 + n; ++s) {
+            src[s].woff = run;
+            src[s].witems = src[s].g0 >= 0 ? src[s].a.nitems : 0;
+            run += src[s].witems;
+            uni = uni && src[s].witems == src[s0].witems;
+        }
+        src[s0].wtotal = run;
+        src[s0].wuni = uni ? src[s0].witems : 0;
+        src[s0].wfd = make_fastdiv(std::max(1, run / n));
+        if (src[s0].wuni != e->cap_tiles) e->compact = 1;
+    }
     HIP_TRY(c, hipMemcpyAsync(e->d_slots, src, S * sizeof(SlotDesc), hipMemcpyHostToDevice, c->stream));
     HIP_TRY(c, hipEventRecord(e->h_ring_ev[e->h_ring], c->stream));
     // continue editing in the other copy once its previous upload has been consumed
@@ -540,7 +560,8 @@ int psdm_rtm_engine_create(psdm_rtm_ctx *c, const psdm_rtm_engine_cfg *cfg, psdm
     if (int rc = upload_slots(e)) return bail(rc);
     ETRY(hipStreamSynchronize(c->stream));
     if (cfg->use_graph)
-        if (int rc = capture_graph(e)) return bail(rc);
+        for (int m = 0; m < 2; m++)
+            if (int rc = capture_graph(e, m)) return bail(rc);
     ETRY(hipEventRecord(e->ev_origin, c->stream));
     ETRY(hipEventSynchronize(e->ev_origin));
     e->t_origin = now_s();
@@ -554,8 +575,10 @@ int psdm_rtm_engine_destroy(psdm_rtm_engine *e) {
     psdm_rtm_ctx *c = e->c;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
-    if (e->exec) (void)hipGraphExecDestroy(e->exec);
-    if (e->graph) (void)hipGraphDestroy(e->graph);
+    for (int m = 0; m < 2; m++) {
+        if (e->exec[m]) (void)hipGraphExecDestroy(e->exec[m]);
+        if (e->graph[m]) (void)hipGraphDestroy(e->graph[m]);
+    }
     for (auto &sl : e->slots) {
         if (sl.d_ints) (void)hipFree(sl.d_ints);
         if (sl.d_inj) (void)hipFree(sl.d_inj);

@@ -650,20 +650,48 @@ __device__ __forceinline__ bool resolve_step(const SlotDesc &sd, long long G, St
     return true;
 }
 
-// One time step of every active slot: a grid of workgroups walks the slot x tile work
-// list; each XCD takes a contiguous range of items (neighbouring tiles of one shot).
+// One time step of every active slot of a launch group: a grid of workgroups walks the
+// group's compact work list (the occupied slots' work items back to back, so idle slots
+// and narrow windows launch nothing); each XCD takes a contiguous range of items
+// (neighbouring tiles of one shot).
 // The slot descriptors are read-only during a launch (the host rewrites them between
 // chunks), so their words are fetched with scalar loads.
-template <int TX, int SV>
+// CMP = BatchArgs compact (a separate instantiation, so that the full-slot mapping reads
+// nothing but its launch arguments before the first item's descriptor)
+template <int TX, int SV, bool CMP>
 __global__ __launch_bounds__(kBlock) void rtm_batch_kernel(BatchArgs b, const SlotDesc *__restrict__ slots,
                                                            const long long *__restrict__ gbase) {
     __shared__ __attribute__((aligned(16))) float lds[lds_floats(TX)];
     const long long G = *gbase + b.goff;
+    const int n = b.nslots;
     const int x = blockIdx.x & 7, nbx = gridDim.x >> 3;
-    for (int m = blockIdx.x >> 3; m < b.per_xcd; m += nbx) {
-        const int wk = x * b.per_xcd + m;
-        if (wk >= b.nwork) break;
-        const int s = fdiv(wk, b.fd_cap), t = wk - s * b.cap_tiles;
+    // the work list: with compact = 0 (every slot full) cap_tiles items per slot, read from
+    // the launch arguments alone; else the group's compact list, the items of its occupied
+    // slots back to back
+    int total = n * b.cap_tiles, uni = b.cap_tiles;
+    FastDiv wfd = b.fd_cap;
+    if (CMP) {
+        total = slots[0].wtotal;
+        uni = slots[0].wuni;
+        wfd = slots[0].wfd;
+    }
+    const int per_xcd = (total + 7) >> 3;
+    for (int m = blockIdx.x >> 3; m < per_xcd; m += nbx) {
+        const int wk = x * per_xcd + m;
+        if (wk >= total) break;
+        // the slot holding item wk: wk / uni when every slot holds uni items, else the last
+        // slot whose range starts at or before wk, found from the guess wk / (mean items per
+        // slot) by a walk
+        int s, t;
+        if (!CMP || uni > 0) {
+            s = fdiv(wk, wfd);
+            t = wk - s * uni;
+        } else {
+            s = min(fdiv(wk, wfd), n - 1);
+            while (s > 0 && slots[s].woff > wk) --s;
+            while (s + 1 < n && slots[s + 1].woff <= wk) ++s;
+            t = wk - slots[s].woff;
+        }
         const SlotDesc &sd = slots[s];
         StepArgs a = sd.a;
         if (!resolve_step(sd, G, a) || t >= a.nitems) continue;
@@ -749,11 +777,11 @@ void launch_step(const StepArgs &a, int tile_x, hipStream_t s, hipEvent_t e0, hi
 
 template <int TX, int SV>
 void launch_batch_tx(const BatchArgs &b, int grid, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+    auto k = b.compact ? rtm_batch_kernel<TX, SV, true> : rtm_batch_kernel<TX, SV, false>;
     if (!e0 && !e1)  // plain launch: also valid under stream capture
-        hipLaunchKernelGGL((rtm_batch_kernel<TX, SV>), dim3(grid), dim3(kBlock), 0, s, b, b.slots, b.gbase);
+        hipLaunchKernelGGL(k, dim3(grid), dim3(kBlock), 0, s, b, b.slots, b.gbase);
     else
-        hipExtLaunchKernelGGL((rtm_batch_kernel<TX, SV>), dim3(grid), dim3(kBlock), 0, s, e0, e1, 0, b, b.slots,
-                              b.gbase);
+        hipExtLaunchKernelGGL(k, dim3(grid), dim3(kBlock), 0, s, e0, e1, 0, b, b.slots, b.gbase);
 }
 
 template <int SV>
