@@ -256,6 +256,57 @@ int psdm_rtm_psdm2twt(psdm_rtm_ctx *ctx, int nx, int nz, float dd, float dt, con
  * time-accumulation kernel and of the interpolation kernel, and the conversions timed. */
 int psdm_rtm_psdm2twt_timing(psdm_rtm_ctx *ctx, double *times_ms, double *interp_ms, long long *n_calls);
 
+/* ------------------------------------------------------------------------------
+ * SU trace files and shot navigation without Seismic Unix.  An SU trace is a 240-byte
+ * header (60 words, size_su_header modules.f90:19) followed by ns 4-byte IEEE samples;
+ * big_endian selects the byte order (endianness_data modules.f90:186, default big).
+ * ------------------------------------------------------------------------------ */
+
+/* Replaces get_shot_data (get_mcs_data.f90:150-214) and get_su_data (auxiliar.f90:9-44):
+ * d_su holds ntr traces of ns samples back to back as read from the file (nbytes >=
+ * ntr*(240 + 4*ns), 4-byte aligned, device); d_out (ntr, nt) device FP32, trace-major
+ * (trace j's nt samples contiguous, the layout psdm_rtm_condition takes), sample i of
+ * trace j = sample i - shift of the file's trace (0 outside 0..nt-1); shift 0 is the raw
+ * gather, shift = int(delay_time_data/dt_mcs) the delayed one of get_mcs_data.f90:184-204
+ * (whose samples the reference leaves unwritten are 0 here).  1 <= nt <= ns.  Bit for bit:
+ * a pure byte reordering.  Synchronous; d_su must be complete when called. */
+int psdm_rtm_su_decode(psdm_rtm_ctx *ctx, const void *d_su, long long nbytes, int ntr, int ns, int nt,
+                       int big_endian, int shift, float *d_out);
+/* Header words of every trace: d_out[j] = the INTEGER*4 at 1-based byte `pos` of trace j
+ * (byte_fldr = 9 the shot number of get_pos_channels get_mcs_data.f90:457, byte_offset =
+ * 37 the offset of :467; modules.f90:19-27).  Synchronous. */
+int psdm_rtm_su_header_i32(psdm_rtm_ctx *ctx, const void *d_su, long long nbytes, int ntr, int ns, int big_endian,
+                           int pos, int *d_out);
+/* Replaces the SU branch of write_psdm / write_vel (auxiliar.f90:275-299, 179-210), which
+ * shells out to a2b | suaddhead | sushw | suchw: d_img (ni, nj) device FP32 trace-major
+ * (model(nj, ni) in the reference's column order) -> d_su, ni*(240 + 4*nj) bytes (device,
+ * 4-byte aligned): tracl = tracr = trace number (from 1), ns = nj, d1 = dj, f1 = 0, d2 = di,
+ * f2 = 0, cdp = int(di*(tracl-1)) as suchw computes it, trid; all other header bytes 0.
+ * nj <= 65535.  Synchronous. */
+int psdm_rtm_su_encode(psdm_rtm_ctx *ctx, const float *d_img, int ni, int nj, float di, float dj, int trid,
+                       int big_endian, void *d_su);
+/* With psdm_rtm_set_timing(ctx, n != 0): mean device time per decode / encode call. */
+int psdm_rtm_su_timing(psdm_rtm_ctx *ctx, double *decode_ms, double *encode_ms, long long *n_decode,
+                       long long *n_encode);
+
+/* Host only.  Receiver and source grid indices of one MCS shot, get_pos_channels
+ * (get_mcs_data.f90:418-510) and get_geometry_mcs (:128-148) in FP32: with offset_header
+ * = 1 the receiver offsets come from the header words `offsets` (|offset|*offset_unit, or
+ * |offset|/|offset_unit| when offset_unit < 0), with 0 from near_offset + r*drec; pos_rec
+ * = pos_shot - dir_boat*offset; nxRec = 1 + floor(pos_rec/dd), nyRec = 1 +
+ * floor(streamer_depth/dd), nxSou = 1 + floor(pos_shot/dd), nySou = 1 + floor(shot_depth/dd). */
+int psdm_rtm_mcs_geometry(const int *offsets, int nrec, float pos_shot, int offset_header, float offset_unit,
+                          int dir_boat, float near_offset, float drec, float dd, float shot_depth,
+                          float streamer_depth, float *pos_rec, int *nxRec, int *nyRec, int *nxSou, int *nySou);
+/* Host only.  read_nav_shot_mcs (get_mcs_data.f90:30-126): one shot per line, "shotID x"
+ * (cols 11), "shotID x y" (111) or "shotID x y depth" (1111); pos = d_shot_model +
+ * sqrt((x-x1)**2 + (y-y1)**2) in FP32 from the first line's (dir_boat 1) or the last
+ * line's (-1) coordinates; bat = |depth|.  *shot_i / *shot_f = 0 select the first / last
+ * shot (and are set to its ID); *line_i / *line_f = the 1-based lines holding them (0:
+ * absent).  *n = number of lines; with cap = 0 only *n is returned. */
+int psdm_rtm_nav_read(const char *path, int cols, int dir_boat, float d_shot_model, int *shot_i, int *shot_f,
+                      int cap, int *ids, float *pos, float *bat, int *n, int *line_i, int *line_f);
+
 #ifdef __cplusplus
 }
 #endif

@@ -184,4 +184,121 @@ contains
     call linear_interpolation_1D(x1, y1, a, x2, y2, b)
   end subroutine
 
+  ! The SU trace reader (open_shot_mcs get_mcs_data.f90:286-314, get_shot_data :150-214):
+  ! nrec traces of ntmax samples behind 60-word headers, the first nt samples of each.
+  subroutine ref_su_read_shot(cpath, big, nrec, ntmax, nt, raw) bind(C, name="ref_su_read_shot")
+    use mod_parfile, only: delay_time_data, dt_mcs
+    character(kind=c_char) :: cpath(*)
+    integer(c_int), value :: big, nrec, ntmax, nt
+    real(c_float) :: raw(nt, nrec)
+    real, allocatable :: delay(:, :)
+    call su_setup(cpath, big, nrec, ntmax, nt)
+    ! get_shot_data tests an uninitialised local before computing the delay shift; with
+    ! delay_time_data = 0 that shift is 0 whatever the test decides
+    delay_time_data = 0.; dt_mcs = 1.
+    allocate(delay(nt, nrec))
+    delay = 0.
+    call open_shot_mcs(0, 1)
+    call get_shot_data(1, raw, delay)
+    call close_mcs_files(1)
+    deallocate(delay)
+  end subroutine
+
+  subroutine su_setup(cpath, big, nrec, ntmax, nt)
+    use mod_parfile, only: folder_input_mcs, su_file_mcs, endianness_data, endianness_machine, NumRec_MCS, &
+                           nt_mcs, nt_mcs_max
+    character(kind=c_char) :: cpath(*)
+    integer(c_int), value :: big, nrec, ntmax, nt
+    character(len=500) :: p
+    call c_to_folder(cpath, p)
+    folder_input_mcs = ' '
+    if (allocated(su_file_mcs)) deallocate(su_file_mcs)
+    allocate(su_file_mcs(1))
+    su_file_mcs(1) = trim(p)
+    endianness_machine = 0; endianness_data = big
+    NumRec_MCS = nrec; nt_mcs = nt; nt_mcs_max = ntmax
+  end subroutine
+
+  ! Header words and receiver geometry (get_pos_channels get_mcs_data.f90:418-510,
+  ! get_geometry_mcs :128-148) of one shot file.
+  subroutine ref_su_geometry(cpath, big, nrec, ntmax, byte_shot, offset_hdr, offset_unit_in, dir, near_off, &
+                             drec_in, pos_shot, dd, shot_depth, rec_depth, shot_id, offs, posrec, nxs, nys, &
+                             nxr, nyr) bind(C, name="ref_su_geometry")
+    use mod_parfile, only: byte_shotnumber_mcs, offset_header, offset_unit, dir_boat, near_offset, drec, &
+                           shot_depth_mcs, streamer_depth
+    use mod_data_arrays, only: shotID_su_mcs, offset_su, pos_rec, pos_shot_mcs, nxSou_MCS, nySou_MCS, &
+                               nxRec_MCS, nyRec_MCS
+    include 'mpif.h'
+    character(kind=c_char) :: cpath(*)
+    integer(c_int), value :: big, nrec, ntmax, byte_shot, offset_hdr, dir
+    real(c_float), value :: offset_unit_in, near_off, drec_in, pos_shot, dd, shot_depth, rec_depth
+    integer(c_int) :: shot_id, nxs, nys, nxr(nrec), nyr(nrec)
+    real(c_float) :: offs(nrec), posrec(nrec)
+    integer :: ierr
+    logical :: up
+    real :: d
+    call MPI_Initialized(up, ierr)
+    if (.not. up) call MPI_Init(ierr)
+    call su_setup(cpath, big, nrec, ntmax, ntmax)
+    byte_shotnumber_mcs = byte_shot; offset_header = offset_hdr; offset_unit = offset_unit_in
+    dir_boat = dir; near_offset = near_off; drec = drec_in
+    shot_depth_mcs = shot_depth; streamer_depth = rec_depth
+    if (allocated(shotID_su_mcs)) deallocate(shotID_su_mcs, offset_su, pos_rec, pos_shot_mcs, nxSou_MCS, &
+                                             nySou_MCS, nxRec_MCS, nyRec_MCS)
+    allocate(shotID_su_mcs(1), offset_su(nrec, 1), pos_rec(nrec, 1), pos_shot_mcs(1), nxSou_MCS(1), &
+             nySou_MCS(1), nxRec_MCS(nrec, 1), nyRec_MCS(nrec, 1))
+    offset_su = 0.; pos_shot_mcs(1) = pos_shot
+    call open_shot_mcs(0, 1)
+    call get_pos_channels(0, 1)
+    d = dd
+    call get_geometry_mcs(0, 1, d)
+    call close_mcs_files(1)
+    shot_id = shotID_su_mcs(1); nxs = nxSou_MCS(1); nys = nySou_MCS(1)
+    offs = offset_su(:, 1); posrec = pos_rec(:, 1); nxr = nxRec_MCS(:, 1); nyr = nyRec_MCS(:, 1)
+  end subroutine
+
+  ! Shot positions along the line from the navigation file (read_nav_shot_mcs
+  ! get_mcs_data.f90:30-126), called as a rank other than 0 (no report, no stop).
+  subroutine ref_read_nav(cpath, cols, dir, add, n, ids, pos, bat, li, lf) bind(C, name="ref_read_nav")
+    use mod_parfile, only: folder_input_mcs, nav_shot_mcs, nav_mcs_cols, dir_boat, shot_i, shot_f, &
+                           d_shot_model, line_i, line_f
+    use mod_data_arrays, only: shotID_nav_mcs, pos_shot_mcs, pos_bat_mcs
+    character(kind=c_char) :: cpath(*)
+    integer(c_int), value :: cols, dir, n
+    real(c_float), value :: add
+    integer(c_int) :: ids(n), li, lf
+    real(c_float) :: pos(n), bat(n)
+    character(len=500) :: p
+    call c_to_folder(cpath, p)
+    folder_input_mcs = ' '; nav_shot_mcs = trim(p); nav_mcs_cols = cols; dir_boat = dir
+    d_shot_model = add; shot_i = 0; shot_f = 0; line_i = 0; line_f = 0
+    if (allocated(shotID_nav_mcs)) deallocate(shotID_nav_mcs)
+    if (allocated(pos_shot_mcs)) deallocate(pos_shot_mcs)
+    if (allocated(pos_bat_mcs)) deallocate(pos_bat_mcs)
+    allocate(shotID_nav_mcs(n), pos_shot_mcs(n), pos_bat_mcs(n))
+    call read_nav_shot_mcs(1)
+    ids = shotID_nav_mcs; pos = pos_shot_mcs; bat = pos_bat_mcs; li = line_i; lf = line_f
+  end subroutine
+
+  ! get_su_data (auxiliar.f90:9-44) on a file opened as get_vel opens an SU model
+  ! (:92, CONVERT='BIG_ENDIAN'), or little-endian: ni traces of ns samples, the first
+  ! ns_max of each.
+  subroutine ref_get_su_data(cpath, big, ni, ns, ns_max, out) bind(C, name="ref_get_su_data")
+    character(kind=c_char) :: cpath(*)
+    integer(c_int), value :: big, ni, ns, ns_max
+    real(c_float) :: out(ns_max, ni)
+    character(len=500) :: p
+    integer :: a, b, c, u
+    call c_to_folder(cpath, p)
+    u = 77
+    if (big /= 0) then
+      open(u, FILE=trim(p), ACCESS='stream', FORM='unformatted', CONVERT='BIG_ENDIAN', STATUS='old')
+    else
+      open(u, FILE=trim(p), ACCESS='stream', FORM='unformatted', CONVERT='LITTLE_ENDIAN', STATUS='old')
+    endif
+    a = ni; b = ns; c = ns_max
+    call get_su_data(a, b, c, u, out)
+    close(u)
+  end subroutine
+
 end module ref_harness

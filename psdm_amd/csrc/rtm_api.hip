@@ -553,6 +553,8 @@ int psdm_rtm_destroy(psdm_rtm_ctx *c) {
         if (e) (void)hipEventDestroy(e);
     for (auto e : c->twt_ev)
         if (e) (void)hipEventDestroy(e);
+    for (auto e : c->su_ev)
+        if (e) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(c->stream);
     delete c;
     return 0;
@@ -682,6 +684,8 @@ int psdm_rtm_set_timing(psdm_rtm_ctx *c, int timing) {
     c->cond_n[0] = c->cond_n[1] = 0;
     c->twt_ms[0] = c->twt_ms[1] = 0.0;
     c->twt_n = 0;
+    c->su_ms[0] = c->su_ms[1] = 0.0;
+    c->su_n[0] = c->su_n[1] = 0;
     return 0;
 }
 
