@@ -554,6 +554,122 @@ def bench_poststack(args):
     ctx.close()
 
 
+def bench_su(args):
+    """The SU formats on either side of the migration: the raw records of the 64-shot line
+    (480 traces x 2401 samples @ 2 ms per shot, big-endian SU, 302 MB) decoded from
+    device-resident file bytes into the trace-major FP32 gathers in one call per step
+    (get_shot_data, get_mcs_data.f90:150-214), and the final 2001 x 601 image encoded as an
+    SU file (write_psdm, auxiliar.f90:275-299).  `--steps` line decodes timed; one JSON
+    line.  CPU baseline: the reference's open_shot_mcs + get_shot_data (oracle/_ref) on a
+    file in the page cache."""
+    import numpy as np
+
+    nshot, nrec, ns = 64, 480, 2401
+    cpu = None
+    if not args.no_cpu:
+        from oracle import cpu_baseline
+
+        reps = 150
+        r = cpu_baseline.run_su(nrec, ns, reps, args.cpu_procs)
+        cpu = {"value": round(r["gathers_per_s"], 2), "unit": "gathers/s", "cores": r["cores"], "kind": r["kind"],
+               "sample": f"{r['cores']} processes, each reading one {nrec}-trace x {ns}-sample big-endian SU gather "
+                         f"{reps} times through open_shot_mcs + get_shot_data (page cache; slowest "
+                         f"{r['rank_s']:.2f} s)"}
+        log(f"[bench] su cpu baseline {cpu['value']} gathers/s")
+
+    import torch
+
+    from psdm_amd import su
+    from psdm_amd._native import Context
+
+    dev = torch.device("cuda:0")
+    ctx = Context(0)
+    rng = np.random.default_rng(0)
+    shot = b"".join(rng.integers(0, 1000, 60).astype(">i4").tobytes() + rng.standard_normal(ns).astype(">f4").tobytes()
+                    for _ in range(nrec))
+    host = np.frombuffer(shot * nshot, np.uint8)
+    pinned = torch.empty(host.size, dtype=torch.uint8, pin_memory=True)
+    pinned.numpy()[:] = host
+    d_su = pinned.to(dev)
+    ntr = nshot * nrec
+    out = torch.empty((ntr, ns), device=dev)
+    su.decode(ctx, d_su, ntr, ns, out=out)
+    ok = bool(np.array_equal(out[:nrec].cpu().numpy(), np.frombuffer(shot, ">f4").reshape(nrec, 60 + ns)[:, 60:]))
+    ctx.set_timing(1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        su.decode(ctx, d_su, ntr, ns, out=out)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    dec_ms, _, ndec, _ = su.timing(ctx)
+    # with the host-to-device copy of the file bytes from pinned memory (PCIe)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(3):
+        d_su.copy_(pinned)
+        su.decode(ctx, d_su, ntr, ns, out=out)
+    torch.cuda.synchronize()
+    h2d_s = (time.perf_counter() - t1) / 3
+    # the same bytes moved by a plain device-to-device copy (torch's copy kernel): the
+    # practical streaming rate the decode is compared with
+    src = out.clone()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    out.copy_(src)
+    e0.record()
+    for _ in range(20):
+        out.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    copy_ms = e0.elapsed_time(e1) / 20
+    del src
+    # the final image as an SU file
+    nz, nx = 601, 2001
+    img = torch.randn((nx, nz), device=dev)
+    enc = torch.empty(nx * (240 + 4 * nz), dtype=torch.uint8, device=dev)
+    su.encode(ctx, img, 10.0, 10.0, out=enc)
+    ctx.set_timing(0)
+    ctx.set_timing(1)
+    for _ in range(20):
+        su.encode(ctx, img, 10.0, 10.0, out=enc)
+    _, enc_ms, _, nenc = su.timing(ctx)
+    ctx.set_timing(0)
+    byts = 8.0 * ntr * ns  # read and write each sample once (headers are not read)
+    ach = byts / (dec_ms * 1e-3) / 1e9
+    tpath = os.path.join(ROOT, "profiles", "traffic_su_decode_kernel_line.md")
+    traffic = None
+    if os.path.exists(tpath):
+        import re
+
+        mt = re.search(r"^traffic_bytes_per_launch = (\d+)", open(tpath).read(), re.M)
+        traffic = int(mt.group(1)) if mt else None
+    ebytes = 4.0 * nx * nz + nx * (240 + 4 * nz)
+    out_j = {
+        "metric": "SU gathers decoded per second (device-resident file bytes -> FP32 gathers)",
+        "value": round(nshot * args.steps / elapsed, 1), "unit": "gathers/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": 1, "ms_per_step": round(1e3 * elapsed / args.steps, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u32",
+        "data": "synthetic (seeded big-endian SU records)",
+        "config": {"workload": f"{nshot} shots x {nrec} traces x {ns} samples (2 ms, 4.8 s), big-endian SU, "
+                               f"{host.size / 1e6:.1f} MB per line, one decode call per line"},
+        "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": "su_decode_kernel",
+                     "mean_kernel_ms": round(dec_ms, 5), "timed_launches": ndec, "algorithmic_bytes_per_launch": byts},
+        "decode_matches_bytes": ok,
+        "plain_copy": {"bytes": byts, "ms": round(copy_ms, 5), "achieved_gbs": round(byts / (copy_ms * 1e-3) / 1e9, 1),
+                       "note": "torch device-to-device copy of the decoded gathers (same bytes read and written)"},
+        "with_h2d": {"gathers_per_s": round(nshot / h2d_s, 1), "ms_per_line": round(1e3 * h2d_s, 3),
+                     "note": "host-to-device copy of the line's bytes from pinned memory, then the decode"},
+        "encode": {"kernel": "su_encode_kernel", "workload": f"{nx} x {nz} image -> SU", "mean_kernel_ms":
+                   round(enc_ms, 5), "calls": nenc, "algorithmic_bytes": ebytes,
+                   "achieved_gbs": round(ebytes / (enc_ms * 1e-3) / 1e9, 1)},
+        "cpu_baseline": cpu,
+        "speedup_vs_cpu": round(nshot * args.steps / elapsed / cpu["value"], 1) if cpu else None,
+    }
+    print(json.dumps(out_j), flush=True)
+    ctx.close()
+
+
 def bench_served(args):
     """Served stream: Poisson arrivals at 4 rates; 70% MCS gathers (480 ch @12.5 m, 8 km
     aperture, 601-deep model @10 m, nt 6000, FP32 snapshots every step) and 30% WAS
@@ -660,9 +776,11 @@ def main():
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="batched form with the default launch shape")
     ap.add_argument("--forms", default="plain,batched")
-    ap.add_argument("--workload", default="mcs2001", choices=["mcs2001", "uneven", "mixed", "served", "condition", "poststack"],
+    ap.add_argument("--workload", default="mcs2001",
+                    choices=["mcs2001", "uneven", "mixed", "served", "condition", "poststack", "su"],
                     help="mcs2001: the 64-shot line (default line); uneven: 256 uneven shots, batched vs plain; "
-                         "mixed: uneven + FP16/BF16 snapshots every 4th step; served: Poisson-arrival stream")
+                         "mixed: uneven + FP16/BF16 snapshots every 4th step; served: Poisson-arrival stream; "
+                         "condition / poststack / su: the steps before and after the migration")
     ap.add_argument("--shots", type=int, default=256, help="uneven/mixed: shots of the 256-shot line used")
     ap.add_argument("--slots", type=int, default=16, help="uneven/mixed/served: engine slots")
     ap.add_argument("--hbm-gb", type=float, default=200.0, help="uneven/mixed/served: snapshot budget (GiB)")
@@ -684,6 +802,8 @@ def main():
         return bench_condition(args)
     if args.workload == "poststack":
         return bench_poststack(args)
+    if args.workload == "su":
+        return bench_su(args)
     forms = set(args.forms.split(","))
 
     from psdm_amd.workloads import line_workload, step_bytes, updates_per_shot

@@ -272,6 +272,12 @@ int psdm_rtm_psdm2twt_timing(psdm_rtm_ctx *ctx, double *times_ms, double *interp
  * a pure byte reordering.  Synchronous; d_su must be complete when called. */
 int psdm_rtm_su_decode(psdm_rtm_ctx *ctx, const void *d_su, long long nbytes, int ntr, int ns, int nt,
                        int big_endian, int shift, float *d_out);
+/* Host-array form of psdm_rtm_su_decode for the Fortran seam (get_shot_data's own
+ * arguments): reads ntr traces from byte `offset` of the file at `path` (pinned staging,
+ * one host-to-device copy), decodes them on the device and copies the gather to h_out
+ * (nt, ntr) column-major = (ntr, nt) trace-major.  Synchronous. */
+int psdm_rtm_su_read_host(psdm_rtm_ctx *ctx, const char *path, long long offset, int ntr, int ns, int nt,
+                          int big_endian, int shift, float *h_out);
 /* Header words of every trace: d_out[j] = the INTEGER*4 at 1-based byte `pos` of trace j
  * (byte_fldr = 9 the shot number of get_pos_channels get_mcs_data.f90:457, byte_offset =
  * 37 the offset of :467; modules.f90:19-27).  Synchronous. */

@@ -127,3 +127,39 @@ def run_twt(nz: int, ncols_per_rank: int, dd: float, dt: float, nprocs: int):
     busy = max(per)
     return dict(kind="reference", cores=nprocs, cols_per_rank=ncols_per_rank, rank_s=busy,
                 cols_per_s=nprocs * ncols_per_rank / busy)
+
+
+def _one_rank_su(args):
+    nrec, ns, reps, seed = args
+    import tempfile
+
+    from oracle import su_codec as S
+
+    rng = np.random.default_rng(seed)
+    buf = b"".join(rng.integers(0, 1000, 60).astype(">i4").tobytes() + rng.standard_normal(ns).astype(">f4").tobytes()
+                   for _ in range(nrec))
+    fd, path = tempfile.mkstemp(suffix=".su")
+    try:
+        with os.fdopen(fd, "wb") as f:
+            f.write(buf)
+        S.ref_read_shot(path, True, nrec, ns, ns)  # page cache warm
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            S.ref_read_shot(path, True, nrec, ns, ns)
+        return time.perf_counter() - t0
+    finally:
+        os.unlink(path)
+
+
+def run_su(nrec: int, ns: int, reps: int, nprocs: int):
+    """SU gather reads with the reference's open_shot_mcs + get_shot_data (oracle/_ref,
+    big-endian file in the page cache), `reps` reads per process; returns gathers per second."""
+    from oracle import pyoracle as O
+
+    if not O.ref_available():
+        raise RuntimeError("oracle/_ref/libpsdm_ref.so is not built")
+    ctx = mp.get_context("spawn")
+    with ctx.Pool(nprocs) as pool:
+        per = pool.map(_one_rank_su, [(nrec, ns, reps, s) for s in range(nprocs)])
+    busy = max(per)
+    return dict(kind="reference", cores=nprocs, reps=reps, rank_s=busy, gathers_per_s=nprocs * reps / busy)

@@ -61,12 +61,14 @@ FORTRAN_SHIM = os.path.join(CSRC, "psdm_rtm_fortran.f90")
 CALLER_DIR = os.path.join(ROOT, "tests", "fortran")
 CALLER = os.path.join(CALLER_DIR, "_build", "rtm_caller")
 COND_CALLER = os.path.join(CALLER_DIR, "_build", "cond_caller")
+SU_CALLER = os.path.join(CALLER_DIR, "_build", "su_caller")
 
 
 def build_fortran_caller(verbose: bool = False):
     """Compiles the Fortran shim against the test fixture module mod_parfile and links
     the test callers (tests/fortran/rtm_caller.f90: the solver calls of one shot;
-    tests/fortran/cond_caller.f90: the gather conditioning) with libpsdm_rtm.so.
+    tests/fortran/cond_caller.f90: the gather conditioning; tests/fortran/su_caller.f90: the
+    SU shot reader) with libpsdm_rtm.so.
     Returns the solver caller's path, or None where no Fortran compiler is installed."""
     if not os.path.exists(FLANG):
         return None
@@ -74,7 +76,7 @@ def build_fortran_caller(verbose: bool = False):
     os.makedirs(bdir, exist_ok=True)
     rpath = "$ORIGIN/" + os.path.relpath(PKG, bdir)
     common = [os.path.join(CALLER_DIR, "mod_parfile_test.f90"), FORTRAN_SHIM]
-    for main, exe in (("rtm_caller.f90", CALLER), ("cond_caller.f90", COND_CALLER)):
+    for main, exe in (("rtm_caller.f90", CALLER), ("cond_caller.f90", COND_CALLER), ("su_caller.f90", SU_CALLER)):
         cmd = [FLANG, "-O2", "-J", bdir, *common, os.path.join(CALLER_DIR, main), "-L", PKG, "-lpsdm_rtm",
                f"-Wl,-rpath,{rpath}", "-o", exe]
         if verbose:

@@ -76,6 +76,15 @@ module psdm_rtm_iface
       real(c_float), value :: dmodel, dt
       real(c_float) :: Sadj(*), ctmp(*), kernel(*)
     end function
+    integer(c_int) function psdm_rtm_su_read_host(ctx, path, offset, ntr, ns, nt, big_endian, shift, h_out) &
+        bind(C, name="psdm_rtm_su_read_host")
+      import :: c_ptr, c_int, c_float, c_char, c_long_long
+      type(c_ptr), value :: ctx
+      character(kind=c_char) :: path(*)
+      integer(c_long_long), value :: offset
+      integer(c_int), value :: ntr, ns, nt, big_endian, shift
+      real(c_float) :: h_out(*)
+    end function
     integer(c_int) function psdm_rtm_condition_host(ctx, cond, NumRec, Field_Data, Data_target) &
         bind(C, name="psdm_rtm_condition_host")
       import :: c_ptr, c_int, c_float, psdm_rtm_cond
@@ -224,4 +233,30 @@ subroutine psdm_condition_gather(Field_Data, nt_data, dt_data, NumRec, typef, f_
   if (cut_multiple > 0) q%nt_cut = c_loc(nt_cut)
   call psdm_check(psdm_rtm_condition_host(psdm_context(), q, NumRec, Field_Data, Data_target), &
                   'psdm_condition_gather')
+end subroutine
+
+! Replaces get_shot_data (get_mcs_data.f90:150-214) with its own arguments, after
+! open_shot_mcs (:286-314) would have opened the shot's file: the file is
+! folder_input_mcs // su_file_mcs(icount), NumRec_MCS traces of nt_mcs_max samples, byte
+! order endianness_data (1 big-endian, 0 little).  raw_data gets the first nt_mcs samples
+! of each trace; with delay_time_data /= 0, delay_data gets them shifted by
+! int(delay_time_data/dt_mcs) samples (0 where nothing is shifted in).
+subroutine psdm_get_shot_data(icount, raw_data, delay_data)
+  use psdm_rtm_iface
+  use mod_parfile, only: folder_input_mcs, su_file_mcs, endianness_data, NumRec_MCS, nt_mcs, nt_mcs_max, &
+                         delay_time_data, dt_mcs
+  implicit none
+  integer :: icount
+  real :: raw_data(nt_mcs, NumRec_MCS), delay_data(nt_mcs, NumRec_MCS)
+  character(len=1000) :: file_name
+  integer :: id
+  file_name = trim(adjustl(folder_input_mcs)) // trim(adjustl(su_file_mcs(icount)))
+  call psdm_check(psdm_rtm_su_read_host(psdm_context(), trim(file_name) // c_null_char, 0_c_long_long, NumRec_MCS, &
+                                        nt_mcs_max, nt_mcs, endianness_data, 0_c_int, raw_data), 'psdm_get_shot_data')
+  if (delay_time_data /= 0.) then
+    id = int(delay_time_data / dt_mcs)
+    call psdm_check(psdm_rtm_su_read_host(psdm_context(), trim(file_name) // c_null_char, 0_c_long_long, &
+                                          NumRec_MCS, nt_mcs_max, nt_mcs, endianness_data, id, delay_data), &
+                    'psdm_get_shot_data')
+  endif
 end subroutine

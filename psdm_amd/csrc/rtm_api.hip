@@ -555,6 +555,7 @@ int psdm_rtm_destroy(psdm_rtm_ctx *c) {
         if (e) (void)hipEventDestroy(e);
     for (auto e : c->su_ev)
         if (e) (void)hipEventDestroy(e);
+    if (c->su_pinned) (void)hipHostFree(c->su_pinned);
     (void)hipStreamDestroy(c->stream);
     delete c;
     return 0;

@@ -59,6 +59,8 @@ struct psdm_rtm_ctx {
     hipEvent_t su_ev[4] = {};
     double su_ms[2] = {0.0, 0.0};
     long long su_n[2] = {0, 0};
+    void *su_pinned = nullptr;  // pinned staging of psdm_rtm_su_read_host
+    size_t su_pinned_n = 0;
 };
 
 namespace psdm {

@@ -121,3 +121,38 @@ def test_fortran_conditioning_binding_matches_reference():
             assert np.linalg.norm(out - ref) / np.linalg.norm(ref) < 1e-5, name
         else:
             np.testing.assert_array_equal(out, ref, err_msg=name)
+
+
+@pytest.mark.gpu
+def test_fortran_su_reader_binding_matches_reference_reader():
+    """The driver's shot read (open_shot_mcs + get_shot_data) as one Fortran call
+    (psdm_get_shot_data) on big- and little-endian files, with and without the data delay;
+    against the oracle reader (pinned to the reference's get_shot_data)."""
+    from oracle import su_codec as S
+    from psdm_amd.build import SU_CALLER
+
+    assert os.path.exists(SU_CALLER), "tests/fortran/_build/su_caller is built by __graft_entry__.build()"
+    rng = np.random.default_rng(4)
+    for big, nrec, ntmax, nt, delay, dt in ((1, 96, 1200, 1100, 0.0, 0.002), (0, 33, 700, 700, 0.05, 0.002),
+                                            (1, 5, 64, 60, -0.012, 0.004)):
+        print(f"[gpu] fortran su reader big={big} nrec={nrec} nt={nt} delay={delay}", flush=True)
+        e = ">" if big else "<"
+        buf = b"".join(rng.integers(0, 99, 60).astype(e + "i4").tobytes()
+                       + rng.standard_normal(ntmax).astype(e + "f4").tobytes() for _ in range(nrec))
+        tmp = tempfile.mkdtemp()
+        try:
+            with open(os.path.join(tmp, "shot.su"), "wb") as f:
+                f.write(buf)
+            subprocess.run([SU_CALLER, tmp + "/", "shot.su", str(nrec), str(ntmax), str(nt), str(big), repr(delay),
+                            repr(dt), "out.bin"], cwd=tmp, check=True, timeout=120)
+            o = np.fromfile(os.path.join(tmp, "out.bin"), np.float32)
+        finally:
+            shutil.rmtree(tmp, ignore_errors=True)
+        raw = o[:nrec * nt].reshape(nrec, nt)
+        dl = o[nrec * nt:].reshape(nrec, nt)
+        np.testing.assert_array_equal(raw, S.decode(buf, nrec, ntmax, nt, bool(big)))
+        if delay != 0.0:
+            shift = int(np.float32(delay) / np.float32(dt))
+            np.testing.assert_array_equal(dl, S.decode(buf, nrec, ntmax, nt, bool(big), shift))
+        else:
+            assert (dl == -1.0).all()  # untouched, as the reference's caller ignores it then
