@@ -313,6 +313,34 @@ int psdm_rtm_mcs_geometry(const int *offsets, int nrec, float pos_shot, int offs
 int psdm_rtm_nav_read(const char *path, int cols, int dir_boat, float d_shot_model, int *shot_i, int *shot_f,
                       int cap, int *ids, float *pos, float *bat, int *n, int *line_i, int *line_f);
 
+/* ------------------------------------------------------------------------------
+ * Kirchhoff depth migration of one shot (calculate_PSDM_kirchoff.f90:352-447).
+ * ------------------------------------------------------------------------------ */
+
+/* Replaces solve_eikonal_FSM (solve_eikonal_FSM.f90:1-83) and its callers
+ * dynamic_window_tt_forward / _backward (dynamic_window.f90:230-325): d_vel (ny, nx)
+ * device, column-major (the window Vp_w); for each of nsrc source points (src_x, src_z:
+ * host, 1-based column and row in the window) the travel-time field, d_tt (ny, nx, nsrc)
+ * device, column-major = TT_r(ny, nx_w, NumRec) (or TT_s for one source).  Four fast
+ * sweeps from T = 1e9, T(source) = 0, bit for bit.  nx <= 8192.  Synchronous. */
+int psdm_rtm_eikonal(psdm_rtm_ctx *ctx, const float *d_vel, int ny, int nx, float dx, float dy, int nsrc,
+                     const int *src_x, const int *src_z, float *d_tt);
+/* The imaging loop of calculate_PSDM_kirchoff.f90:394-447: d_img (ny, nx) device =
+ * sum over receivers k (in order) of weight * (d(it+1) - d(it-1)) / (2*dt) with
+ * it = 1 + floor((TT_s + TT_r(k))/dt) (skipped unless 1 < it < nt_prop), Veff = vp0,
+ * weight = w_theta * min(cos_theta / sqrt(rs*rr + 1e-6), wmax), the aperture taper
+ * w_theta = exp(-((theta - theta_max)/theta_taper)**2) beyond theta_max.  d_tts (ny, nx),
+ * d_ttr (ny, nx, nrec) from psdm_rtm_eikonal; d_data (nrec, ld) device, trace-major
+ * (Data_target(nt_taper, NumRec) in the reference's order), ld >= nt_prop.  The travel-time
+ * index is exact; acos and exp are the device's (within a few ulp of the host's libm).
+ * Synchronous. */
+int psdm_rtm_kirchhoff_image(psdm_rtm_ctx *ctx, const float *d_tts, const float *d_ttr, const float *d_data, int ld,
+                             int nrec, int nt_prop, int ny, int nx, float dd, float dt, float vp0, float theta_max,
+                             float theta_taper, float wmax, float *d_img);
+/* With psdm_rtm_set_timing(ctx, n != 0): mean device time per eikonal / imaging call. */
+int psdm_rtm_kirchhoff_timing(psdm_rtm_ctx *ctx, double *eikonal_ms, double *image_ms, long long *n_eikonal,
+                              long long *n_image);
+
 #ifdef __cplusplus
 }
 #endif

@@ -301,4 +301,15 @@ contains
     close(u)
   end subroutine
 
+  ! solve_eikonal_FSM (solve_eikonal_FSM.f90:1-83) for one source point (1-based sx, sz).
+  subroutine ref_eikonal(nx, ny, dx, dy, v, sx, sz, T) bind(C, name="ref_eikonal")
+    integer(c_int), value :: nx, ny, sx, sz
+    real(c_float), value :: dx, dy
+    real(c_float) :: v(ny, nx), T(ny, nx)
+    integer :: a, b, px(1), pz(1)
+    real :: ddx, ddy
+    a = nx; b = ny; px(1) = sx; pz(1) = sz; ddx = dx; ddy = dy
+    call solve_eikonal_FSM(a, b, ddx, ddy, v, px, pz, T)
+  end subroutine
+
 end module ref_harness

@@ -53,6 +53,10 @@ SIGNATURES = {
     "psdm_rtm_su_read_host": ([_P, C.c_char_p, _LL, _I, _I, _I, _I, _I, _P], _I),
     "psdm_rtm_su_encode": ([_P, _P, _I, _I, _F, _F, _I, _I, _P], _I),
     "psdm_rtm_su_timing": ([_P, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(_LL), C.POINTER(_LL)], _I),
+    "psdm_rtm_eikonal": ([_P, _P, _I, _I, _F, _F, _I, _P, _P, _P], _I),
+    "psdm_rtm_kirchhoff_image": ([_P, _P, _P, _P, _I, _I, _I, _I, _I, _F, _F, _F, _F, _F, _F, _P], _I),
+    "psdm_rtm_kirchhoff_timing": ([_P, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(_LL),
+                                   C.POINTER(_LL)], _I),
     "psdm_rtm_mcs_geometry": ([_P, _I, _F, _I, _F, _I, _F, _F, _F, _F, _F, _P, _P, _P, C.POINTER(_I), C.POINTER(_I)],
                               _I),
     "psdm_rtm_nav_read": ([C.c_char_p, _I, _I, _F, C.POINTER(_I), C.POINTER(_I), _I, _P, _P, _P, C.POINTER(_I),

@@ -556,6 +556,8 @@ int psdm_rtm_destroy(psdm_rtm_ctx *c) {
     for (auto e : c->su_ev)
         if (e) (void)hipEventDestroy(e);
     if (c->su_pinned) (void)hipHostFree(c->su_pinned);
+    for (auto e : c->kir_ev)
+        if (e) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(c->stream);
     delete c;
     return 0;
@@ -687,6 +689,8 @@ int psdm_rtm_set_timing(psdm_rtm_ctx *c, int timing) {
     c->twt_n = 0;
     c->su_ms[0] = c->su_ms[1] = 0.0;
     c->su_n[0] = c->su_n[1] = 0;
+    c->kir_ms[0] = c->kir_ms[1] = 0.0;
+    c->kir_n[0] = c->kir_n[1] = 0;
     return 0;
 }
 

@@ -61,6 +61,10 @@ struct psdm_rtm_ctx {
     long long su_n[2] = {0, 0};
     void *su_pinned = nullptr;  // pinned staging of psdm_rtm_su_read_host
     size_t su_pinned_n = 0;
+    // Kirchhoff: device time of the eikonal [0] and imaging [1] kernels per call (timing on)
+    hipEvent_t kir_ev[4] = {};
+    double kir_ms[2] = {0.0, 0.0};
+    long long kir_n[2] = {0, 0};
 };
 
 namespace psdm {

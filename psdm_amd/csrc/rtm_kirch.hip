@@ -1,0 +1,268 @@
+// Kirchhoff depth migration of one shot on the device: eikonal travel times by fast
+// sweeping and the diffraction stack with the aperture-angle weight.
+//
+//   eikonal_fsm_kernel     solve_eikonal_FSM (solve_eikonal_FSM.f90:1-83) for one source
+//                          point per workgroup, bit for bit: four Gauss-Seidel sweeps, each
+//                          run as a wavefront over the sweep's diagonals.  In a sweep the
+//                          cell at sweep-order (row r, column c) reads the updated values of
+//                          (r-1, c) and (r, c-1) and the previous sweep's values of (r+1, c)
+//                          and (r, c+1), so the cells of diagonal d = r + c depend only on
+//                          diagonal d-1 (new) and d+1 (old): processing the diagonals in
+//                          order with a barrier between them gives the sequential sweep's
+//                          results exactly.  Each thread owns model columns and walks down
+//                          them one row per step (loads stream down the column); the two
+//                          values a column's neighbours need each step -- its new value and
+//                          its not-yet-updated next row -- pass through LDS (double
+//                          buffered, one barrier per step).
+//   kirchhoff_image_kernel the imaging loop of calculate_PSDM_kirchoff.f90:394-447: per
+//                          pixel the sum over receivers (in receiver order) of the weight
+//                          times the central time derivative of the trace at the total
+//                          travel time.
+#include <hip/hip_ext.h>
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "rtm_ctx.hpp"
+
+using psdm::host::fail;
+using psdm::host::get_buf;
+using psdm::host::set_device;
+
+namespace psdm {
+namespace {
+
+constexpr float kBig = 1.0e9f;  // initial travel time (BIG, solve_eikonal_FSM.f90:11)
+constexpr int kMaxCols = 8;     // model columns per thread (nx <= 8 * 1024)
+
+// local_solver (solve_eikonal_FSM.f90:63-79), the reference's FP32 operations in order
+__device__ __forceinline__ float local_solver(float a, float b, float dx, float dy, float v_inv) {
+    const float aa = 1.0f / (dx * dx) + 1.0f / (dy * dy);
+    const float bb = -2.0f * (a / (dx * dx) + b / (dy * dy));
+    const float cc = (a * a) / (dx * dx) + (b * b) / (dy * dy) - v_inv * v_inv;
+    const float disc = bb * bb - 4.0f * aa * cc;
+    float t;
+    if (disc >= 0.0f) {
+        t = (-bb + sqrtf(disc)) / (2.0f * aa);
+        if (t < fmaxf(a, b)) t = fmaxf(a, b) + v_inv * fminf(dx, dy);
+    } else {
+        t = fminf(a, b) + v_inv * fminf(dx, dy);
+    }
+    return t;
+}
+
+template <int K>
+__global__ __launch_bounds__(1024) void eikonal_fsm_kernel(const float *__restrict__ vel, int ny, int nx, float dx,
+                                                           float dy, const int *__restrict__ src_x,
+                                                           const int *__restrict__ src_z, float *__restrict__ tt) {
+    extern __shared__ float lds[];
+    float *nw = lds, *od = lds + 2 * nx;  // [2][nx] each: new value at the current row, old value of the next row
+    const int B = blockDim.x, tid = threadIdx.x;
+    float *T = tt + (size_t)blockIdx.x * nx * ny;
+    for (size_t e = tid; e < (size_t)nx * ny; e += B) T[e] = kBig;
+    __syncthreads();
+    if (tid == 0) T[(size_t)(src_x[blockIdx.x] - 1) * ny + (src_z[blockIdx.x] - 1)] = 0.0f;
+    __syncthreads();
+    for (int sweep = 1; sweep <= 4; ++sweep) {
+        // sweep_pass (:24-58): 1 i up j up, 2 i down j up, 3 i down j down, 4 i up j down.
+        // Column c (sweep order) is at row r = d - c in step d.  Per column the thread keeps
+        // cur = old T(r), nxt = old T(r+1) (the value the previous column needs from it
+        // this step), their velocities, and prv = new T(r-1).
+        const bool iup = sweep == 1 || sweep == 4, jup = sweep <= 2;
+        float cur[K], nxt[K], vcur[K], vnxt[K], prv[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const int c = tid + k * B;
+            cur[k] = nxt[k] = vcur[k] = vnxt[k] = prv[k] = 0.f;
+            if (c < nx) {
+                const size_t col = (size_t)(iup ? c : nx - 1 - c) * ny;
+                const int r0 = -c;  // row at step 0
+                if (r0 >= 0 && r0 < ny) {
+                    cur[k] = T[col + (jup ? r0 : ny - 1 - r0)];
+                    vcur[k] = vel[col + (jup ? r0 : ny - 1 - r0)];
+                }
+                if (r0 + 1 >= 0 && r0 + 1 < ny) {
+                    nxt[k] = T[col + (jup ? r0 + 1 : ny - 2 - r0)];
+                    vnxt[k] = vel[col + (jup ? r0 + 1 : ny - 2 - r0)];
+                }
+            }
+        }
+        const int nsteps = nx + ny - 1;
+        for (int d = 0; d < nsteps; ++d) {
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                const int c = tid + k * B;
+                if (c < nx) od[(d & 1) * nx + c] = nxt[k];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                const int c = tid + k * B, r = d - c;
+                if (c >= nx) continue;
+                const size_t col = (size_t)(iup ? c : nx - 1 - c) * ny;
+                if (r >= 0 && r < ny) {
+                    const float self = cur[k];
+                    // neighbours in sweep order: previous column (new, LDS), next column (old,
+                    // its lookahead in LDS), previous row (new, register), next row (old)
+                    const float cp = c > 0 ? nw[((d - 1) & 1) * nx + c - 1] : self;
+                    const float cn = c + 1 < nx ? od[(d & 1) * nx + c + 1] : self;
+                    const float rp = r > 0 ? prv[k] : self;
+                    const float rn = r + 1 < ny ? nxt[k] : self;
+                    float tnew = self;
+                    if (self != 0.0f) {  // the source point keeps 0
+                        // a = min(T(j,i-1), T(j,i+1)), b = min(T(j-1,i), T(j+1,i)): min is
+                        // symmetric, so the sweep-order neighbours give the same values
+                        const float a = fminf(cp, cn), b = fminf(rp, rn);
+                        const float v_inv = 1.0f / vcur[k];
+                        const float t = local_solver(a, b, dx, dy, v_inv);
+                        if (t < self) tnew = t;
+                    }
+                    T[col + (jup ? r : ny - 1 - r)] = tnew;
+                    prv[k] = tnew;
+                    nw[(d & 1) * nx + c] = tnew;
+                }
+                // advance to row r + 1 for the next step
+                cur[k] = nxt[k];
+                vcur[k] = vnxt[k];
+                const int r2 = r + 2;
+                if (r2 >= 0 && r2 < ny) {
+                    nxt[k] = T[col + (jup ? r2 : ny - 1 - r2)];
+                    vnxt[k] = vel[col + (jup ? r2 : ny - 1 - r2)];
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void kirchhoff_image_kernel(const float *__restrict__ tts, const float *__restrict__ ttr,
+                                                              const float *__restrict__ data, int ld, int nrec,
+                                                              int nt_prop, int ny, int nx, float dd, float dt,
+                                                              float vp0, float theta_max, float theta_taper,
+                                                              float wmax, float *__restrict__ img) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;  // pixel (j, i), j fastest
+    if (p >= nx * ny) return;
+    const int j = p % ny;
+    const float tmin = 0.02f, eps = 1.0e-6f, pi = 3.14159265f;  // calculate_PSDM_kirchoff.f90:8-10
+    const float z_img = (float)j * dd;
+    const float ts = tts[p];
+    const float rs = vp0 * fmaxf(ts, tmin);
+    const size_t n = (size_t)nx * ny;
+    float acc = 0.0f;
+    for (int k = 0; k < nrec; ++k) {
+        const float tr = ttr[(size_t)k * n + p];
+        const float tt_tot = ts + tr;
+        const float q = tt_tot / dt;
+        const int it = 1 + (int)floorf(q);
+        if (it <= 1 || it >= nt_prop) continue;
+        const float rr = vp0 * fmaxf(tr, tmin);
+        float cos_theta = z_img / (rs + rr + 1.e-6f);
+        cos_theta = fmaxf(fminf(cos_theta, 1.0f), 0.0f);
+        const float theta = acosf(cos_theta) * 180.0f / pi;
+        float w_theta = 1.0f;
+        if (!(theta <= theta_max)) {
+            const float u = (theta - theta_max) / theta_taper;
+            w_theta = expf(-(u * u));
+        }
+        const float weight = w_theta * fminf(cos_theta / sqrtf(rs * rr + eps), wmax);
+        const float *trace = data + (size_t)k * ld;
+        const float amp = (trace[it] - trace[it - 2]) / (2.0f * dt);  // Data_target(it+1) - Data_target(it-1)
+        acc = acc + weight * amp;
+    }
+    img[p] = acc;
+}
+
+}  // namespace
+}  // namespace psdm
+
+using namespace psdm;
+
+extern "C" int psdm_rtm_eikonal(psdm_rtm_ctx *c, const float *d_vel, int ny, int nx, float dx, float dy, int nsrc,
+                                const int *src_x, const int *src_z, float *d_tt) {
+    if (!c) return PSDM_RTM_EINVAL;
+    if (!d_vel || !d_tt || !src_x || !src_z) return fail(c, PSDM_RTM_EINVAL, "eikonal: null argument");
+    if (ny < 1 || nx < 1 || nx > kMaxCols * 1024 || nsrc < 1)
+        return fail(c, PSDM_RTM_EINVAL, "eikonal: need ny >= 1, 1 <= nx <= 8192, nsrc >= 1");
+    if (!(dx > 0.f) || !(dy > 0.f)) return fail(c, PSDM_RTM_EINVAL, "eikonal: dx, dy must be > 0");
+    for (int s = 0; s < nsrc; ++s)
+        if (src_x[s] < 1 || src_x[s] > nx || src_z[s] < 1 || src_z[s] > ny)
+            return fail(c, PSDM_RTM_EINVAL, "eikonal: source point outside the model (1-based indices)");
+    if (int rc = set_device(c)) return rc;
+    void *d_src;
+    if (int rc = get_buf(c, "eik_src", 2 * (size_t)nsrc * sizeof(int), &d_src)) return rc;
+    std::vector<int> h(2 * (size_t)nsrc);
+    std::memcpy(h.data(), src_x, nsrc * sizeof(int));
+    std::memcpy(h.data() + nsrc, src_z, nsrc * sizeof(int));
+    HIP_TRY(c, hipMemcpyAsync(d_src, h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    const int B = nx >= 1024 ? 1024 : (nx + 63) / 64 * 64;
+    const int K = (nx + B - 1) / B;
+    const unsigned lds = 4u * (unsigned)nx * (unsigned)sizeof(float);
+    const bool timed = c->timing != 0;
+    if (timed)
+        for (int q = 0; q < 2; ++q)
+            if (!c->kir_ev[q]) HIP_TRY(c, hipEventCreate(&c->kir_ev[q]));
+    const int *sx = (const int *)d_src, *sz = sx + nsrc;
+    hipEvent_t e0 = timed ? c->kir_ev[0] : nullptr, e1 = timed ? c->kir_ev[1] : nullptr;
+    if (K == 1)
+        hipExtLaunchKernelGGL(eikonal_fsm_kernel<1>, dim3(nsrc), dim3(B), lds, c->stream, e0, e1, 0, d_vel, ny, nx, dx, dy,
+                              sx, sz, d_tt);
+    else if (K == 2)
+        hipExtLaunchKernelGGL(eikonal_fsm_kernel<2>, dim3(nsrc), dim3(B), lds, c->stream, e0, e1, 0, d_vel, ny, nx, dx, dy,
+                              sx, sz, d_tt);
+    else if (K <= 4)
+        hipExtLaunchKernelGGL(eikonal_fsm_kernel<4>, dim3(nsrc), dim3(B), lds, c->stream, e0, e1, 0, d_vel, ny, nx, dx, dy,
+                              sx, sz, d_tt);
+    else
+        hipExtLaunchKernelGGL(eikonal_fsm_kernel<8>, dim3(nsrc), dim3(B), lds, c->stream, e0, e1, 0, d_vel, ny, nx, dx, dy,
+                              sx, sz, d_tt);
+    HIP_TRY(c, hipGetLastError());
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (timed) {
+        float ms;
+        HIP_TRY(c, hipEventElapsedTime(&ms, e0, e1));
+        c->kir_ms[0] += ms;
+        c->kir_n[0]++;
+    }
+    return 0;
+}
+
+extern "C" int psdm_rtm_kirchhoff_image(psdm_rtm_ctx *c, const float *d_tts, const float *d_ttr, const float *d_data,
+                                        int ld, int nrec, int nt_prop, int ny, int nx, float dd, float dt, float vp0,
+                                        float theta_max, float theta_taper, float wmax, float *d_img) {
+    if (!c) return PSDM_RTM_EINVAL;
+    if (!d_tts || !d_ttr || !d_data || !d_img) return fail(c, PSDM_RTM_EINVAL, "kirchhoff_image: null argument");
+    if (ny < 1 || nx < 1 || nrec < 1 || nt_prop < 1 || ld < nt_prop)
+        return fail(c, PSDM_RTM_EINVAL, "kirchhoff_image: need ny, nx, nrec, nt_prop >= 1 and ld >= nt_prop");
+    if ((long long)nx * ny > 2147483647LL - 256) return fail(c, PSDM_RTM_EINVAL, "kirchhoff_image: image too large");
+    if (!(dt > 0.f)) return fail(c, PSDM_RTM_EINVAL, "kirchhoff_image: dt must be > 0");
+    if (int rc = set_device(c)) return rc;
+    const bool timed = c->timing != 0;
+    if (timed)
+        for (int q = 2; q < 4; ++q)
+            if (!c->kir_ev[q]) HIP_TRY(c, hipEventCreate(&c->kir_ev[q]));
+    const int np = nx * ny;
+    hipExtLaunchKernelGGL(kirchhoff_image_kernel, dim3((np + 255) / 256), dim3(256), 0, c->stream,
+                          timed ? c->kir_ev[2] : nullptr, timed ? c->kir_ev[3] : nullptr, 0, d_tts, d_ttr, d_data, ld,
+                          nrec, nt_prop, ny, nx, dd, dt, vp0, theta_max, theta_taper, wmax, d_img);
+    HIP_TRY(c, hipGetLastError());
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (timed) {
+        float ms;
+        HIP_TRY(c, hipEventElapsedTime(&ms, c->kir_ev[2], c->kir_ev[3]));
+        c->kir_ms[1] += ms;
+        c->kir_n[1]++;
+    }
+    return 0;
+}
+
+extern "C" int psdm_rtm_kirchhoff_timing(psdm_rtm_ctx *c, double *eikonal_ms, double *image_ms, long long *n_eikonal,
+                                         long long *n_image) {
+    if (!c) return PSDM_RTM_EINVAL;
+    if (eikonal_ms) *eikonal_ms = c->kir_n[0] ? c->kir_ms[0] / (double)c->kir_n[0] : 0.0;
+    if (image_ms) *image_ms = c->kir_n[1] ? c->kir_ms[1] / (double)c->kir_n[1] : 0.0;
+    if (n_eikonal) *n_eikonal = c->kir_n[0];
+    if (n_image) *n_image = c->kir_n[1];
+    return 0;
+}

@@ -1,0 +1,107 @@
+"""GPU parity of the Kirchhoff migration (psdm_rtm_eikonal, psdm_rtm_kirchhoff_image) with
+the oracle (oracle/kirchhoff.c; its eikonal is pinned bit for bit to the reference's
+solve_eikonal_FSM in tests/test_kirchhoff_oracle.py).
+
+Bars: travel times bit for bit; the image within 1e-5 rel-L2 (the device's acosf / expf
+differ from the host libm's by a few ulp; the travel-time index is exact)."""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import kirchhoff as K
+
+pytestmark = pytest.mark.gpu
+IMAGE_BAR = 1e-5
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def model(nx, ny, seed):
+    rng = np.random.default_rng(seed)
+    z = np.arange(ny)[None, :]
+    v = np.where(z < max(1, ny // 5), 1500.0, 1700.0 + 2.0 * z * 10) * np.ones((nx, 1))
+    v = v + 200.0 * rng.random((nx, ny))
+    return v.astype(np.float32)
+
+
+@pytest.mark.parametrize("nx,ny,srcs", [
+    (40, 30, [(5, 1)]), (1, 5, [(1, 3)]), (7, 1, [(4, 1)]), (1, 1, [(1, 1)]), (64, 64, [(64, 64), (1, 1), (33, 2)]),
+    (715, 601, [(300, 1)]),            # one MCS window, 768 threads x 1 column
+    (1500, 40, [(700, 2), (1, 1)]),    # 2 columns per thread
+    (3000, 12, [(2999, 12)]),          # 3 columns per thread (the 4-column kernel)
+    (5000, 6, [(2500, 1)])])           # 5 columns per thread (the 8-column kernel)
+def test_eikonal_bit_exact(ctx, nx, ny, srcs):
+    import torch
+
+    from psdm_amd.kirchhoff import travel_times
+
+    say(f"[gpu] eikonal nx={nx} ny={ny} nsrc={len(srcs)}")
+    v = model(nx, ny, nx + ny)
+    T = travel_times(ctx, torch.from_numpy(v).cuda(), 10.0, [s[0] for s in srcs], [s[1] for s in srcs]).cpu().numpy()
+    for q, (sx, sz) in enumerate(srcs):
+        np.testing.assert_array_equal(T[q].view(np.uint32), K.eikonal(v, 10.0, 10.0, sx, sz).view(np.uint32))
+
+
+def test_eikonal_shot_receivers_bit_exact(ctx):
+    """A shot's receivers in one call (48 of them along the surface of a 715 x 601 window)."""
+    import torch
+
+    from psdm_amd.kirchhoff import travel_times
+
+    nx, ny = 715, 601
+    v = model(nx, ny, 3)
+    xs = np.arange(40, 40 + 48 * 12, 12)
+    zs = np.full(48, 2)
+    say("[gpu] eikonal 48 receivers 715 x 601")
+    T = travel_times(ctx, torch.from_numpy(v).cuda(), 10.0, xs, zs).cpu().numpy()
+    for q in (0, 17, 47):
+        np.testing.assert_array_equal(T[q].view(np.uint32), K.eikonal(v, 10.0, 10.0, int(xs[q]), 2).view(np.uint32))
+
+
+@pytest.mark.parametrize("nx,ny,nrec,nt,dt", [(120, 90, 24, 1500, 0.002), (301, 201, 60, 4000, 0.001)])
+def test_image_vs_oracle(ctx, nx, ny, nrec, nt, dt):
+    import torch
+
+    from psdm_amd.kirchhoff import image, travel_times
+
+    say(f"[gpu] kirchhoff image {nx}x{ny} nrec={nrec}")
+    v = model(nx, ny, 7)
+    rng = np.random.default_rng(nx)
+    xr = np.linspace(5, nx - 5, nrec).astype(np.int32)
+    dv = torch.from_numpy(v).cuda()
+    tts = travel_times(ctx, dv, 10.0, [nx // 2], [1])[0]
+    ttr = travel_times(ctx, dv, 10.0, xr, np.full(nrec, 2, np.int32))
+    data = rng.standard_normal((nrec, nt)).astype(np.float32)
+    got = image(ctx, tts, ttr, torch.from_numpy(data).cuda(), nt, 10.0, dt).cpu().numpy()
+    want = K.image(tts.cpu().numpy(), ttr.cpu().numpy(), data, nt, 10.0, dt)
+    assert np.linalg.norm(got - want) / np.linalg.norm(want) < IMAGE_BAR
+    # the aperture and index logic agree exactly: the same pixels are zero
+    np.testing.assert_array_equal(got == 0, want == 0)
+
+
+def test_migrate_shot_and_errors(ctx):
+    import torch
+
+    from psdm_amd._native import RtmError
+    from psdm_amd.kirchhoff import migrate_shot, travel_times
+
+    say("[gpu] kirchhoff migrate_shot")
+    nx, ny, nrec, nt, dt = 90, 70, 16, 1200, 0.002
+    v = model(nx, ny, 11)
+    data = np.random.default_rng(0).standard_normal((nrec, nt)).astype(np.float32)
+    xr = np.arange(10, 10 + 4 * nrec, 4).astype(np.int32)
+    zr = np.full(nrec, 1, np.int32)
+    img = migrate_shot(ctx, torch.from_numpy(v).cuda(), 10.0, 45, 1, xr, zr, torch.from_numpy(data).cuda(), nt,
+                       dt).cpu().numpy()
+    tts = K.eikonal(v, 10.0, 10.0, 45, 1)
+    ttr = np.stack([K.eikonal(v, 10.0, 10.0, int(x), 1) for x in xr])
+    want = K.image(tts, ttr, data, nt, 10.0, dt)
+    assert np.linalg.norm(img - want) / np.linalg.norm(want) < IMAGE_BAR
+    with pytest.raises(RtmError, match="outside"):
+        travel_times(ctx, torch.from_numpy(v).cuda(), 10.0, [nx + 1], [1])
+    with pytest.raises(RtmError, match="8192"):
+        travel_times(ctx, torch.zeros((8193, 2), device="cuda"), 10.0, [1], [1])
