@@ -670,6 +670,88 @@ def bench_su(args):
     ctx.close()
 
 
+def bench_kirchhoff(args):
+    """Kirchhoff migration of the 64-shot line's shots (calculate_PSDM_kirchoff.f90:352-447):
+    per shot the travel times in the 601 x 715 window from the source and the 480 receivers
+    (481 fast-sweeping eikonal solves, one call) and the diffraction stack over 480 traces of
+    6000 samples at 0.8 ms.  `--steps` shots timed; one JSON line.  CPU baseline: the
+    oracle's C restatement (its eikonal is the reference's, bit for bit) on a sample."""
+    import numpy as np
+
+    from psdm_amd.workloads import line_workload
+
+    W = line_workload("mcs2001")
+    m, sh, dt, nt = W["model"], W["shots"][0], W["dt"], W["nt"]
+    ny, nx, dd, nrec = m.nz, sh.nx_w, m.dx, sh.nxRec.size
+    cpu = None
+    if not args.no_cpu:
+        from oracle import cpu_baseline
+
+        r = cpu_baseline.run_kirchhoff(ny, nx, dd, nrec, nt, dt, 12, 12, args.cpu_procs)
+        cpu = {"value": round(r["shots_per_s"], 5), "unit": "shots/s", "cores": r["cores"], "kind": r["kind"],
+               "sample": f"{r['cores']} processes, each timing 12 eikonal solves and the imaging of 12 receivers in "
+                         f"the {ny} x {nx} window; a shot = {nrec + 1} solves + {nrec} receivers "
+                         f"({r['shot_s']:.1f} s per shot per process)"}
+        log(f"[bench] kirchhoff cpu baseline {cpu['value']} shots/s")
+
+    import torch
+
+    from psdm_amd import kirchhoff as KI
+    from psdm_amd._native import Context
+
+    dev = torch.device("cuda:0")
+    ctx = Context(0)
+    d_vel = torch.from_numpy(np.ascontiguousarray(m.migration.T[sh.x_i - 1:sh.x_i - 1 + nx])).to(dev)  # (nx, ny)
+    xs = np.concatenate([[sh.nxSou - sh.x_i + 1], sh.nxRec - sh.x_i + 1]).astype(np.int32)
+    zs = np.concatenate([[sh.nySou], sh.nyRec]).astype(np.int32)
+    tt = torch.empty((nrec + 1, nx, ny), device=dev)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(0)
+    data = torch.randn((nrec, nt), generator=gen, device=dev)
+    img = torch.empty((nx, ny), device=dev)
+
+    def one():
+        KI.travel_times(ctx, d_vel, dd, xs, zs, out=tt)
+        KI.image(ctx, tt[0], tt[1:], data, nt, dd, dt, out=img)
+
+    one()
+    ctx.set_timing(1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        one()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    eik_ms, img_ms, n_eik, n_img = KI.timing(ctx)
+    ctx.set_timing(0)
+    cells = (nrec + 1) * nx * ny
+    # eikonal: per cell and sweep the old value and the velocity read, the new value written
+    ebytes = 12.0 * 4 * cells
+    ach = ebytes / (eik_ms * 1e-3) / 1e9
+    ibytes = 4.0 * (nrec + 1) * nx * ny + 4.0 * nx * ny  # travel times read once, image written
+    out = {
+        "metric": "Kirchhoff-migrated shots/s (eikonal travel times + diffraction stack)",
+        "value": round(args.steps / elapsed, 3), "unit": "shots/s", "n_gpus": 1, "steps": args.steps, "warmup": 1,
+        "ms_per_step": round(1e3 * elapsed / args.steps, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded mcs2001 migration velocity; seeded traces)",
+        "config": {"workload": f"mcs2001 shot: {ny} x {nx} window @ {dd:g} m, {nrec + 1} eikonal solves, "
+                               f"{nrec} traces x {nt} samples @ {dt * 1e3:g} ms"},
+        "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "eikonal_fsm_kernel",
+                     "mean_kernel_ms": round(eik_ms, 4), "timed_launches": n_eik, "algorithmic_bytes_per_launch": ebytes,
+                     "note": "four sweeps of nx + ny - 1 dependent wavefront steps per solve, one barrier each: "
+                             "bound by the step chain's latency, not by bandwidth"},
+        "image_kernel": {"kernel": "kirchhoff_image_kernel", "mean_kernel_ms": round(img_ms, 4),
+                         "pixel_receiver_pairs": nrec * nx * ny, "algorithmic_bytes": ibytes,
+                         "achieved_gbs": round(ibytes / (img_ms * 1e-3) / 1e9, 1)},
+        "cpu_baseline": cpu,
+        "speedup_vs_cpu": round(args.steps / elapsed / cpu["value"], 1) if cpu else None,
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
 def bench_served(args):
     """Served stream: Poisson arrivals at 4 rates; 70% MCS gathers (480 ch @12.5 m, 8 km
     aperture, 601-deep model @10 m, nt 6000, FP32 snapshots every step) and 30% WAS
@@ -777,10 +859,11 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="batched form with the default launch shape")
     ap.add_argument("--forms", default="plain,batched")
     ap.add_argument("--workload", default="mcs2001",
-                    choices=["mcs2001", "uneven", "mixed", "served", "condition", "poststack", "su"],
+                    choices=["mcs2001", "uneven", "mixed", "served", "condition", "poststack", "su", "kirchhoff"],
                     help="mcs2001: the 64-shot line (default line); uneven: 256 uneven shots, batched vs plain; "
                          "mixed: uneven + FP16/BF16 snapshots every 4th step; served: Poisson-arrival stream; "
-                         "condition / poststack / su: the steps before and after the migration")
+                         "condition / poststack / su: the steps before and after the migration; "
+                         "kirchhoff: the Kirchhoff migration of the line's shots")
     ap.add_argument("--shots", type=int, default=256, help="uneven/mixed: shots of the 256-shot line used")
     ap.add_argument("--slots", type=int, default=16, help="uneven/mixed/served: engine slots")
     ap.add_argument("--hbm-gb", type=float, default=200.0, help="uneven/mixed/served: snapshot budget (GiB)")
@@ -804,6 +887,8 @@ def main():
         return bench_poststack(args)
     if args.workload == "su":
         return bench_su(args)
+    if args.workload == "kirchhoff":
+        return bench_kirchhoff(args)
     forms = set(args.forms.split(","))
 
     from psdm_amd.workloads import line_workload, step_bytes, updates_per_shot

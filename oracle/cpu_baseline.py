@@ -163,3 +163,33 @@ def run_su(nrec: int, ns: int, reps: int, nprocs: int):
         per = pool.map(_one_rank_su, [(nrec, ns, reps, s) for s in range(nprocs)])
     busy = max(per)
     return dict(kind="reference", cores=nprocs, reps=reps, rank_s=busy, gathers_per_s=nprocs * reps / busy)
+
+
+def _one_rank_kirchhoff(args):
+    ny, nx, dd, n_eik, n_img, nt, dt, seed = args
+    from oracle import kirchhoff as K
+
+    rng = np.random.default_rng(seed)
+    z = np.arange(ny)[None, :]
+    v = (np.where(z < 100, 1500.0, 1700.0 + 2.0 * z) + 100 * rng.random((nx, ny))).astype(np.float32)
+    t0 = time.perf_counter()
+    tts = K.eikonal(v, dd, dd, nx // 2, 1)
+    ttr = np.stack([K.eikonal(v, dd, dd, 1 + (nx - 1) * q // max(1, n_eik - 1), 2) for q in range(n_eik - 1)])
+    t_eik = (time.perf_counter() - t0) / n_eik
+    data = rng.standard_normal((n_img, nt)).astype(np.float32)
+    t1 = time.perf_counter()
+    K.image(tts, ttr[np.arange(n_img) % ttr.shape[0]], data, nt, dd, dt)
+    t_img = (time.perf_counter() - t1) / n_img
+    return t_eik, t_img
+
+
+def run_kirchhoff(ny: int, nx: int, dd: float, nrec: int, nt: int, dt: float, n_eik: int, n_img: int, nprocs: int):
+    """Kirchhoff migration of one shot with the oracle's C restatement (the eikonal is the
+    reference's solve_eikonal_FSM bit for bit): each process times n_eik travel-time solves
+    and the imaging of n_img receivers; a shot is 1 + nrec solves and nrec receivers."""
+    ctx = mp.get_context("spawn")
+    with ctx.Pool(nprocs) as pool:
+        per = pool.map(_one_rank_kirchhoff, [(ny, nx, dd, n_eik, n_img, nt, dt, s) for s in range(nprocs)])
+    shot_s = max((1 + nrec) * te + nrec * ti for te, ti in per)
+    return dict(kind="port", cores=nprocs, eikonal_s=max(te for te, _ in per), image_s_per_rec=max(ti for _, ti in per),
+                shot_s=shot_s, shots_per_s=nprocs / shot_s)

@@ -53,8 +53,22 @@ __device__ __forceinline__ float local_solver(float a, float b, float dx, float 
     return t;
 }
 
-template <int K>
-__global__ __launch_bounds__(1024) void eikonal_fsm_kernel(const float *__restrict__ vel, int ny, int nx, float dx,
+// Workgroup barrier for the wavefront steps: LDS writes complete, then s_barrier.  Global
+// memory is not fenced (that would wait for the prefetch loads every step): within a sweep
+// each thread reads and writes only its own columns; the sweeps are separated by full
+// __syncthreads().
+__device__ __forceinline__ void lds_barrier() { __asm__ volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+__global__ void reciprocal_kernel(const float *__restrict__ v, float *__restrict__ vinv, size_t n) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n) vinv[e] = 1.0f / v[e];  // v_inv = 1.0/v(j,i) (solve_eikonal_FSM.f90:52), once per cell
+}
+
+// K model columns per thread; P: prefetch depth, in rows, of each column's old travel times
+// and velocities (a row's loads are issued P steps before the step that uses them, so a
+// memory latency is spread over P barrier steps)
+template <int K, int P>
+__global__ __launch_bounds__(1024) void eikonal_fsm_kernel(const float *__restrict__ vinv, int ny, int nx, float dx,
                                                            float dy, const int *__restrict__ src_x,
                                                            const int *__restrict__ src_z, float *__restrict__ tt) {
     extern __shared__ float lds[];
@@ -65,71 +79,74 @@ __global__ __launch_bounds__(1024) void eikonal_fsm_kernel(const float *__restri
     __syncthreads();
     if (tid == 0) T[(size_t)(src_x[blockIdx.x] - 1) * ny + (src_z[blockIdx.x] - 1)] = 0.0f;
     __syncthreads();
+    constexpr int R = P + 2;  // ring: rows r .. r + P + 1 of the column
     for (int sweep = 1; sweep <= 4; ++sweep) {
         // sweep_pass (:24-58): 1 i up j up, 2 i down j up, 3 i down j down, 4 i up j down.
         // Column c (sweep order) is at row r = d - c in step d.  Per column the thread keeps
-        // cur = old T(r), nxt = old T(r+1) (the value the previous column needs from it
-        // this step), their velocities, and prv = new T(r-1).
+        // the old T and 1/v of rows r .. r+P+1 in a ring (tq, vq; the old T(r+1) is what the
+        // previous column needs from it this step) and prv = new T(r-1).
         const bool iup = sweep == 1 || sweep == 4, jup = sweep <= 2;
-        float cur[K], nxt[K], vcur[K], vnxt[K], prv[K];
+        float tq[K][R], vq[K][R], prv[K];
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             const int c = tid + k * B;
-            cur[k] = nxt[k] = vcur[k] = vnxt[k] = prv[k] = 0.f;
-            if (c < nx) {
-                const size_t col = (size_t)(iup ? c : nx - 1 - c) * ny;
-                const int r0 = -c;  // row at step 0
-                if (r0 >= 0 && r0 < ny) {
-                    cur[k] = T[col + (jup ? r0 : ny - 1 - r0)];
-                    vcur[k] = vel[col + (jup ? r0 : ny - 1 - r0)];
-                }
-                if (r0 + 1 >= 0 && r0 + 1 < ny) {
-                    nxt[k] = T[col + (jup ? r0 + 1 : ny - 2 - r0)];
-                    vnxt[k] = vel[col + (jup ? r0 + 1 : ny - 2 - r0)];
+            prv[k] = 0.f;
+            const size_t col = (size_t)(iup ? (c < nx ? c : 0) : nx - 1 - (c < nx ? c : 0)) * ny;
+#pragma unroll
+            for (int m = 0; m < R; ++m) {
+                const int row = m - c;  // row of ring slot m at step 0
+                tq[k][m] = vq[k][m] = 0.f;
+                if (c < nx && row >= 0 && row < ny) {
+                    tq[k][m] = T[col + (jup ? row : ny - 1 - row)];
+                    vq[k][m] = vinv[col + (jup ? row : ny - 1 - row)];
                 }
             }
         }
         const int nsteps = nx + ny - 1;
-        for (int d = 0; d < nsteps; ++d) {
+        // the step loop unrolled by the ring size R: in sub-step u the slots are fixed
+        // (row r + m in slot (u + m) % R), so no register of an in-flight load is moved
+        for (int d0 = 0; d0 < nsteps; d0 += R) {
 #pragma unroll
-            for (int k = 0; k < K; ++k) {
-                const int c = tid + k * B;
-                if (c < nx) od[(d & 1) * nx + c] = nxt[k];
-            }
-            __syncthreads();
+            for (int u = 0; u < R; ++u) {
+                const int d = d0 + u;
+                if (d >= nsteps) break;  // uniform over the workgroup
 #pragma unroll
-            for (int k = 0; k < K; ++k) {
-                const int c = tid + k * B, r = d - c;
-                if (c >= nx) continue;
-                const size_t col = (size_t)(iup ? c : nx - 1 - c) * ny;
-                if (r >= 0 && r < ny) {
-                    const float self = cur[k];
-                    // neighbours in sweep order: previous column (new, LDS), next column (old,
-                    // its lookahead in LDS), previous row (new, register), next row (old)
-                    const float cp = c > 0 ? nw[((d - 1) & 1) * nx + c - 1] : self;
-                    const float cn = c + 1 < nx ? od[(d & 1) * nx + c + 1] : self;
-                    const float rp = r > 0 ? prv[k] : self;
-                    const float rn = r + 1 < ny ? nxt[k] : self;
-                    float tnew = self;
-                    if (self != 0.0f) {  // the source point keeps 0
-                        // a = min(T(j,i-1), T(j,i+1)), b = min(T(j-1,i), T(j+1,i)): min is
-                        // symmetric, so the sweep-order neighbours give the same values
-                        const float a = fminf(cp, cn), b = fminf(rp, rn);
-                        const float v_inv = 1.0f / vcur[k];
-                        const float t = local_solver(a, b, dx, dy, v_inv);
-                        if (t < self) tnew = t;
-                    }
-                    T[col + (jup ? r : ny - 1 - r)] = tnew;
-                    prv[k] = tnew;
-                    nw[(d & 1) * nx + c] = tnew;
+                for (int k = 0; k < K; ++k) {
+                    const int c = tid + k * B;
+                    if (c < nx) od[(d & 1) * nx + c] = tq[k][(u + 1) % R];
                 }
-                // advance to row r + 1 for the next step
-                cur[k] = nxt[k];
-                vcur[k] = vnxt[k];
-                const int r2 = r + 2;
-                if (r2 >= 0 && r2 < ny) {
-                    nxt[k] = T[col + (jup ? r2 : ny - 1 - r2)];
-                    vnxt[k] = vel[col + (jup ? r2 : ny - 1 - r2)];
+                lds_barrier();
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    const int c = tid + k * B, r = d - c;
+                    if (c >= nx) continue;
+                    const size_t col = (size_t)(iup ? c : nx - 1 - c) * ny;
+                    if (r >= 0 && r < ny) {
+                        const float self = tq[k][u];
+                        // neighbours in sweep order: previous column (new, LDS), next column
+                        // (old, its lookahead in LDS), previous row (new, register), next row (old)
+                        const float cp = c > 0 ? nw[((d - 1) & 1) * nx + c - 1] : self;
+                        const float cn = c + 1 < nx ? od[(d & 1) * nx + c + 1] : self;
+                        const float rp = r > 0 ? prv[k] : self;
+                        const float rn = r + 1 < ny ? tq[k][(u + 1) % R] : self;
+                        float tnew = self;
+                        if (self != 0.0f) {  // the source point keeps 0
+                            // a = min(T(j,i-1), T(j,i+1)), b = min(T(j-1,i), T(j+1,i)): min is
+                            // symmetric, so the sweep-order neighbours give the same values
+                            const float a = fminf(cp, cn), b = fminf(rp, rn);
+                            const float t = local_solver(a, b, dx, dy, vq[k][u]);
+                            if (t < self) tnew = t;
+                        }
+                        T[col + (jup ? r : ny - 1 - r)] = tnew;
+                        prv[k] = tnew;
+                        nw[(d & 1) * nx + c] = tnew;
+                    }
+                    // slot u now takes row r + R
+                    const int rl = r + R;
+                    if (rl >= 0 && rl < ny) {
+                        tq[k][u] = T[col + (jup ? rl : ny - 1 - rl)];
+                        vq[k][u] = vinv[col + (jup ? rl : ny - 1 - rl)];
+                    }
                 }
             }
         }
@@ -190,8 +207,9 @@ extern "C" int psdm_rtm_eikonal(psdm_rtm_ctx *c, const float *d_vel, int ny, int
         if (src_x[s] < 1 || src_x[s] > nx || src_z[s] < 1 || src_z[s] > ny)
             return fail(c, PSDM_RTM_EINVAL, "eikonal: source point outside the model (1-based indices)");
     if (int rc = set_device(c)) return rc;
-    void *d_src;
+    void *d_src, *d_vinv;
     if (int rc = get_buf(c, "eik_src", 2 * (size_t)nsrc * sizeof(int), &d_src)) return rc;
+    if (int rc = get_buf(c, "eik_vinv", (size_t)nx * ny * sizeof(float), &d_vinv)) return rc;
     std::vector<int> h(2 * (size_t)nsrc);
     std::memcpy(h.data(), src_x, nsrc * sizeof(int));
     std::memcpy(h.data() + nsrc, src_z, nsrc * sizeof(int));
@@ -204,18 +222,22 @@ extern "C" int psdm_rtm_eikonal(psdm_rtm_ctx *c, const float *d_vel, int ny, int
         for (int q = 0; q < 2; ++q)
             if (!c->kir_ev[q]) HIP_TRY(c, hipEventCreate(&c->kir_ev[q]));
     const int *sx = (const int *)d_src, *sz = sx + nsrc;
+    const size_t ncell = (size_t)nx * ny;
+    hipLaunchKernelGGL(reciprocal_kernel, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, c->stream, d_vel,
+                       (float *)d_vinv, ncell);
+    const float *vi = (const float *)d_vinv;
     hipEvent_t e0 = timed ? c->kir_ev[0] : nullptr, e1 = timed ? c->kir_ev[1] : nullptr;
     if (K == 1)
-        hipExtLaunchKernelGGL(eikonal_fsm_kernel<1>, dim3(nsrc), dim3(B), lds, c->stream, e0, e1, 0, d_vel, ny, nx, dx, dy,
+        hipExtLaunchKernelGGL(eikonal_fsm_kernel<1, 8>, dim3(nsrc), dim3(B), lds, c->stream, e0, e1, 0, vi, ny, nx, dx, dy,
                               sx, sz, d_tt);
     else if (K == 2)
-        hipExtLaunchKernelGGL(eikonal_fsm_kernel<2>, dim3(nsrc), dim3(B), lds, c->stream, e0, e1, 0, d_vel, ny, nx, dx, dy,
+        hipExtLaunchKernelGGL(eikonal_fsm_kernel<2, 4>, dim3(nsrc), dim3(B), lds, c->stream, e0, e1, 0, vi, ny, nx, dx, dy,
                               sx, sz, d_tt);
     else if (K <= 4)
-        hipExtLaunchKernelGGL(eikonal_fsm_kernel<4>, dim3(nsrc), dim3(B), lds, c->stream, e0, e1, 0, d_vel, ny, nx, dx, dy,
+        hipExtLaunchKernelGGL(eikonal_fsm_kernel<4, 2>, dim3(nsrc), dim3(B), lds, c->stream, e0, e1, 0, vi, ny, nx, dx, dy,
                               sx, sz, d_tt);
     else
-        hipExtLaunchKernelGGL(eikonal_fsm_kernel<8>, dim3(nsrc), dim3(B), lds, c->stream, e0, e1, 0, d_vel, ny, nx, dx, dy,
+        hipExtLaunchKernelGGL(eikonal_fsm_kernel<8, 1>, dim3(nsrc), dim3(B), lds, c->stream, e0, e1, 0, vi, ny, nx, dx, dy,
                               sx, sz, d_tt);
     HIP_TRY(c, hipGetLastError());
     HIP_TRY(c, hipStreamSynchronize(c->stream));
