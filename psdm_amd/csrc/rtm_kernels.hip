@@ -670,22 +670,32 @@ __global__ __launch_bounds__(kBlock) void rtm_batch_kernel(BatchArgs b, const Sl
     // slots back to back
     int total = n * b.cap_tiles, uni = b.cap_tiles;
     FastDiv wfd = b.fd_cap;
+    // groups of <= 64 slots: each slot's first item in the compact list, in LDS (read per item
+    // by the lane of the slot's index; kept out of registers for the step's register budget)
+    __shared__ int woffs[64];
     if (CMP) {
         total = slots[0].wtotal;
         uni = slots[0].wuni;
         wfd = slots[0].wfd;
+        if (n <= 64 && (int)threadIdx.x < 64) woffs[threadIdx.x] = (int)threadIdx.x < n ? slots[threadIdx.x].woff : 0x7fffffff;
+        __syncthreads();
     }
     const int per_xcd = (total + 7) >> 3;
     for (int m = blockIdx.x >> 3; m < per_xcd; m += nbx) {
         const int wk = x * per_xcd + m;
         if (wk >= total) break;
         // the slot holding item wk: wk / uni when every slot holds uni items, else the last
-        // slot whose range starts at or before wk, found from the guess wk / (mean items per
-        // slot) by a walk
+        // slot whose range starts at or before wk -- with <= 64 slots one wave-wide compare
+        // of the lanes' first items (the ranges are in slot order), else a walk from the
+        // guess wk / (mean items per slot)
         int s, t;
         if (!CMP || uni > 0) {
             s = fdiv(wk, wfd);
             t = wk - s * uni;
+        } else if (n <= 64) {
+            const int mine = woffs[threadIdx.x & 63];
+            s = __popcll(__ballot(mine <= wk)) - 1;
+            t = wk - __builtin_amdgcn_readlane(mine, s);
         } else {
             s = min(fdiv(wk, wfd), n - 1);
             while (s > 0 && slots[s].woff > wk) --s;
