@@ -160,6 +160,21 @@ def test_engine_launch_chains_bit_identical(ctx):
         np.testing.assert_array_equal(fold, runs[0][1][1])
 
 
+def test_engine_compact_lookup_paths_bit_identical(ctx):
+    """70 resident shots of uneven windows: one group of 70 slots (the slot of a work item
+    found by a walk over the slot descriptors) and two groups of 35 (a wave-wide compare of
+    the group's offsets) give every shot the image it gets in a 5-slot engine."""
+    vel = model(6)
+    idx = [q % 5 for q in range(70)]
+    say("[gpu] engine 70 slots: 1 group of 70 / 2 groups of 35 / reference 5 slots")
+    ref, (_, fold_ref), _ = run_engine(ctx, vel, SHOTS, range(5), nslots=5, chunk_steps=32)
+    for groups in (1, 2):
+        imgs, (_, fold), _ = run_engine(ctx, vel, SHOTS, idx, nslots=70, groups=groups, chunk_steps=32)
+        for q in range(5):
+            np.testing.assert_array_equal(imgs[q], ref[q])
+        np.testing.assert_array_equal(fold, 14 * fold_ref)
+
+
 def test_engine_hbm_budget_admission(ctx):
     """A budget that holds one FP32 snapshot store at a time runs the shots one after
     the other in a 3-slot engine, with the same results."""
