@@ -59,25 +59,91 @@ __device__ __forceinline__ float local_solver(float a, float b, float dx, float 
 // __syncthreads().
 __device__ __forceinline__ void lds_barrier() { __asm__ volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 
-__global__ void reciprocal_kernel(const float *__restrict__ v, float *__restrict__ vinv, size_t n) {
+// 1/v once per cell (v_inv = 1.0/v(j,i), solve_eikonal_FSM.f90:52, the same IEEE division),
+// stored in the two diagonal-major layouts the sweeps read (see eikonal_fsm_kernel).
+__global__ void reciprocal_kernel(const float *__restrict__ v, int ny, int nx, float *__restrict__ va,
+                                  float *__restrict__ vm) {
     const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e < n) vinv[e] = 1.0f / v[e];  // v_inv = 1.0/v(j,i) (solve_eikonal_FSM.f90:52), once per cell
+    if (e >= (size_t)nx * ny) return;
+    const int i = (int)(e / ny), j = (int)(e - (size_t)i * ny);
+    const float r = 1.0f / v[e];
+    va[(size_t)(i + j) * nx + i] = r;
+    vm[(size_t)(nx - 1 - i + j) * nx + (nx - 1 - i)] = r;
+}
+
+// Travel-time layouts of one solve: A[(i + j) * nx + i] (anti-diagonals contiguous), M[(nx-1-i
+// + j) * nx + (nx-1-i)] (diagonals contiguous) and the model's column-major i * ny + j.
+enum { kLayA = 0, kLayM = 1, kLayModel = 2 };
+constexpr int kTile = 64, kTileLd = kTile + 1;
+__device__ __forceinline__ size_t lay_addr(int kind, int i, int j, int nx, int ny) {
+    return kind == kLayA ? (size_t)(i + j) * nx + i
+                         : kind == kLayM ? (size_t)(nx - 1 - i + j) * nx + (nx - 1 - i) : (size_t)i * ny + j;
+}
+// Slot s of a 64 x 64 tile, numbered so that consecutive slots are consecutive addresses of
+// the layout: (il, jl), false for slots outside the tile.
+__device__ __forceinline__ bool tile_slot(int kind, int s, int &il, int &jl) {
+    if (kind == kLayModel) {
+        il = s >> 6;
+        jl = s & 63;
+        return true;
+    }
+    const int e = s >> 6, f = s & 63;  // e: diagonal within the tile, f: position along it
+    il = kind == kLayA ? f : kTile - 1 - f;
+    jl = e - f;
+    return jl >= 0 && jl < kTile;
+}
+// Copies a solve's travel times between layouts through 64 x 64 LDS tiles (both sides
+// coalesced).  The caller separates it from the accesses around it with __syncthreads().
+__device__ void relayout(const float *__restrict__ src, int sk, float *__restrict__ dst, int dk, int nx, int ny,
+                         float *tile) {
+    const int ns = sk == kLayModel ? kTile * kTile : (2 * kTile - 1) * kTile;
+    const int nd = dk == kLayModel ? kTile * kTile : (2 * kTile - 1) * kTile;
+    for (int i0 = 0; i0 < nx; i0 += kTile)
+        for (int j0 = 0; j0 < ny; j0 += kTile) {
+            for (int q = threadIdx.x; q < ns; q += blockDim.x) {
+                int il, jl;
+                if (!tile_slot(sk, q, il, jl) || i0 + il >= nx || j0 + jl >= ny) continue;
+                tile[il * kTileLd + jl] = src[lay_addr(sk, i0 + il, j0 + jl, nx, ny)];
+            }
+            lds_barrier();
+            for (int q = threadIdx.x; q < nd; q += blockDim.x) {
+                int il, jl;
+                if (!tile_slot(dk, q, il, jl) || i0 + il >= nx || j0 + jl >= ny) continue;
+                dst[lay_addr(dk, i0 + il, j0 + jl, nx, ny)] = tile[il * kTileLd + jl];
+            }
+            lds_barrier();
+        }
 }
 
 // K model columns per thread; P: prefetch depth, in rows, of each column's old travel times
-// and velocities (a row's loads are issued P steps before the step that uses them, so a
-// memory latency is spread over P barrier steps)
+// and 1/v.
+//
+// A sweep processes the cells of one diagonal per step, so the travel times are kept in the
+// layout whose diagonals are contiguous for that sweep -- A for sweeps 1 and 3
+// (anti-diagonals), M for sweeps 2 and 4 -- and every load and store of a step coalesces.
+// A sweep reads and writes its layout in place (a step never reads a cell that an earlier
+// step of the same sweep wrote: its new neighbours come through LDS and registers); between
+// sweeps the travel times are re-laid out through LDS tiles (A -> M -> A -> M), and after
+// the last sweep M -> the model's column-major output.
 template <int K, int P>
-__global__ __launch_bounds__(1024) void eikonal_fsm_kernel(const float *__restrict__ vinv, int ny, int nx, float dx,
-                                                           float dy, const int *__restrict__ src_x,
-                                                           const int *__restrict__ src_z, float *__restrict__ tt) {
+__global__ __launch_bounds__(1024) void eikonal_fsm_kernel(const float *__restrict__ va, const float *__restrict__ vm,
+                                                           int ny, int nx, float dx, float dy,
+                                                           const int *__restrict__ src_x,
+                                                           const int *__restrict__ src_z, float *__restrict__ work,
+                                                           float *__restrict__ tt) {
     extern __shared__ float lds[];
     float *nw = lds, *od = lds + 2 * nx;  // [2][nx] each: new value at the current row, old value of the next row
+    float *tile = lds + 4 * nx;           // relayout tile, 64 x 65
     const int B = blockDim.x, tid = threadIdx.x;
+    const size_t nd = (size_t)(nx + ny - 1) * nx;  // diagonal-major array (padding cells never read)
+    float *TA = work + (size_t)blockIdx.x * 2 * nd, *TM = TA + nd;
     float *T = tt + (size_t)blockIdx.x * nx * ny;
-    for (size_t e = tid; e < (size_t)nx * ny; e += B) T[e] = kBig;
+    for (size_t e = tid; e < nd; e += B) TA[e] = kBig;
     __syncthreads();
-    if (tid == 0) T[(size_t)(src_x[blockIdx.x] - 1) * ny + (src_z[blockIdx.x] - 1)] = 0.0f;
+    if (tid == 0) {
+        const int i = src_x[blockIdx.x] - 1, j = src_z[blockIdx.x] - 1;
+        TA[(size_t)(i + j) * nx + i] = 0.0f;
+    }
     __syncthreads();
     constexpr int R = P + 2;  // ring: rows r .. r + P + 1 of the column
     for (int sweep = 1; sweep <= 4; ++sweep) {
@@ -86,19 +152,25 @@ __global__ __launch_bounds__(1024) void eikonal_fsm_kernel(const float *__restri
         // the old T and 1/v of rows r .. r+P+1 in a ring (tq, vq; the old T(r+1) is what the
         // previous column needs from it this step) and prv = new T(r-1).
         const bool iup = sweep == 1 || sweep == 4, jup = sweep <= 2;
+        // sweep-order cell (r, c) of step d lies at D * nx + X of the sweep's layout, with
+        // D = d, X = c (sweeps 1, 2) or D = nx+ny-2-d, X = nx-1-c (sweeps 3, 4)
+        const bool fwd = sweep <= 2;
+        float *Ts = (sweep == 1 || sweep == 3) ? TA : TM;
+        const float *Vs = (sweep == 1 || sweep == 3) ? va : vm;
+        const int dmax = nx + ny - 2;
         float tq[K][R], vq[K][R], prv[K];
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             const int c = tid + k * B;
             prv[k] = 0.f;
-            const size_t col = (size_t)(iup ? (c < nx ? c : 0) : nx - 1 - (c < nx ? c : 0)) * ny;
 #pragma unroll
             for (int m = 0; m < R; ++m) {
-                const int row = m - c;  // row of ring slot m at step 0
+                const int row = m - c;  // row of ring slot m at step 0 (diagonal m)
                 tq[k][m] = vq[k][m] = 0.f;
                 if (c < nx && row >= 0 && row < ny) {
-                    tq[k][m] = T[col + (jup ? row : ny - 1 - row)];
-                    vq[k][m] = vinv[col + (jup ? row : ny - 1 - row)];
+                    const unsigned q = fwd ? (unsigned)(m * nx + c) : (unsigned)((dmax - m) * nx + (nx - 1 - c));
+                    tq[k][m] = Ts[q];
+                    vq[k][m] = Vs[q];
                 }
             }
         }
@@ -120,7 +192,6 @@ __global__ __launch_bounds__(1024) void eikonal_fsm_kernel(const float *__restri
                 for (int k = 0; k < K; ++k) {
                     const int c = tid + k * B, r = d - c;
                     if (c >= nx) continue;
-                    const size_t col = (size_t)(iup ? c : nx - 1 - c) * ny;
                     if (r >= 0 && r < ny) {
                         const float self = tq[k][u];
                         // neighbours in sweep order: previous column (new, LDS), next column
@@ -137,19 +208,28 @@ __global__ __launch_bounds__(1024) void eikonal_fsm_kernel(const float *__restri
                             const float t = local_solver(a, b, dx, dy, vq[k][u]);
                             if (t < self) tnew = t;
                         }
-                        T[col + (jup ? r : ny - 1 - r)] = tnew;
+                        Ts[fwd ? (unsigned)(d * nx + c) : (unsigned)((dmax - d) * nx + (nx - 1 - c))] = tnew;
                         prv[k] = tnew;
                         nw[(d & 1) * nx + c] = tnew;
                     }
-                    // slot u now takes row r + R
+                    // slot u now takes row r + R (diagonal d + R)
                     const int rl = r + R;
                     if (rl >= 0 && rl < ny) {
-                        tq[k][u] = T[col + (jup ? rl : ny - 1 - rl)];
-                        vq[k][u] = vinv[col + (jup ? rl : ny - 1 - rl)];
+                        const unsigned q =
+                            fwd ? (unsigned)((d + R) * nx + c) : (unsigned)((dmax - d - R) * nx + (nx - 1 - c));
+                        tq[k][u] = Ts[q];
+                        vq[k][u] = Vs[q];
                     }
                 }
             }
         }
+        __syncthreads();
+        if (sweep == 1 || sweep == 3)
+            relayout(TA, kLayA, TM, kLayM, nx, ny, tile);
+        else if (sweep == 2)
+            relayout(TM, kLayM, TA, kLayA, nx, ny, tile);
+        else
+            relayout(TM, kLayM, T, kLayModel, nx, ny, tile);
         __syncthreads();
     }
 }
@@ -203,42 +283,46 @@ extern "C" int psdm_rtm_eikonal(psdm_rtm_ctx *c, const float *d_vel, int ny, int
     if (ny < 1 || nx < 1 || nx > kMaxCols * 1024 || nsrc < 1)
         return fail(c, PSDM_RTM_EINVAL, "eikonal: need ny >= 1, 1 <= nx <= 8192, nsrc >= 1");
     if (!(dx > 0.f) || !(dy > 0.f)) return fail(c, PSDM_RTM_EINVAL, "eikonal: dx, dy must be > 0");
+    if ((long long)(nx + ny - 1) * nx > 2147483647LL - 1024)
+        return fail(c, PSDM_RTM_EINVAL, "eikonal: (nx + ny - 1) * nx must stay below 2^31");
     for (int s = 0; s < nsrc; ++s)
         if (src_x[s] < 1 || src_x[s] > nx || src_z[s] < 1 || src_z[s] > ny)
             return fail(c, PSDM_RTM_EINVAL, "eikonal: source point outside the model (1-based indices)");
     if (int rc = set_device(c)) return rc;
-    void *d_src, *d_vinv;
+    void *d_src, *d_v, *d_work;
+    const size_t nd = (size_t)(nx + ny - 1) * nx;
     if (int rc = get_buf(c, "eik_src", 2 * (size_t)nsrc * sizeof(int), &d_src)) return rc;
-    if (int rc = get_buf(c, "eik_vinv", (size_t)nx * ny * sizeof(float), &d_vinv)) return rc;
+    if (int rc = get_buf(c, "eik_vinv", 2 * nd * sizeof(float), &d_v)) return rc;
+    if (int rc = get_buf(c, "eik_work", 2 * nd * (size_t)nsrc * sizeof(float), &d_work)) return rc;
     std::vector<int> h(2 * (size_t)nsrc);
     std::memcpy(h.data(), src_x, nsrc * sizeof(int));
     std::memcpy(h.data() + nsrc, src_z, nsrc * sizeof(int));
     HIP_TRY(c, hipMemcpyAsync(d_src, h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
     const int B = nx >= 1024 ? 1024 : (nx + 63) / 64 * 64;
     const int K = (nx + B - 1) / B;
-    const unsigned lds = 4u * (unsigned)nx * (unsigned)sizeof(float);
+    const unsigned lds = (4u * (unsigned)nx + (unsigned)(kTile * kTileLd)) * (unsigned)sizeof(float);
     const bool timed = c->timing != 0;
     if (timed)
         for (int q = 0; q < 2; ++q)
             if (!c->kir_ev[q]) HIP_TRY(c, hipEventCreate(&c->kir_ev[q]));
     const int *sx = (const int *)d_src, *sz = sx + nsrc;
     const size_t ncell = (size_t)nx * ny;
-    hipLaunchKernelGGL(reciprocal_kernel, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, c->stream, d_vel,
-                       (float *)d_vinv, ncell);
-    const float *vi = (const float *)d_vinv;
+    float *va = (float *)d_v, *vm = va + nd, *work = (float *)d_work;
+    hipLaunchKernelGGL(reciprocal_kernel, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, c->stream, d_vel, ny, nx,
+                       va, vm);
     hipEvent_t e0 = timed ? c->kir_ev[0] : nullptr, e1 = timed ? c->kir_ev[1] : nullptr;
     if (K == 1)
-        hipExtLaunchKernelGGL(eikonal_fsm_kernel<1, 8>, dim3(nsrc), dim3(B), lds, c->stream, e0, e1, 0, vi, ny, nx, dx, dy,
-                              sx, sz, d_tt);
+        hipExtLaunchKernelGGL(eikonal_fsm_kernel<1, 4>, dim3(nsrc), dim3(B), lds, c->stream, e0, e1, 0, va, vm, ny, nx,
+                              dx, dy, sx, sz, work, d_tt);
     else if (K == 2)
-        hipExtLaunchKernelGGL(eikonal_fsm_kernel<2, 4>, dim3(nsrc), dim3(B), lds, c->stream, e0, e1, 0, vi, ny, nx, dx, dy,
-                              sx, sz, d_tt);
+        hipExtLaunchKernelGGL(eikonal_fsm_kernel<2, 4>, dim3(nsrc), dim3(B), lds, c->stream, e0, e1, 0, va, vm, ny, nx,
+                              dx, dy, sx, sz, work, d_tt);
     else if (K <= 4)
-        hipExtLaunchKernelGGL(eikonal_fsm_kernel<4, 2>, dim3(nsrc), dim3(B), lds, c->stream, e0, e1, 0, vi, ny, nx, dx, dy,
-                              sx, sz, d_tt);
+        hipExtLaunchKernelGGL(eikonal_fsm_kernel<4, 2>, dim3(nsrc), dim3(B), lds, c->stream, e0, e1, 0, va, vm, ny, nx,
+                              dx, dy, sx, sz, work, d_tt);
     else
-        hipExtLaunchKernelGGL(eikonal_fsm_kernel<8, 1>, dim3(nsrc), dim3(B), lds, c->stream, e0, e1, 0, vi, ny, nx, dx, dy,
-                              sx, sz, d_tt);
+        hipExtLaunchKernelGGL(eikonal_fsm_kernel<8, 1>, dim3(nsrc), dim3(B), lds, c->stream, e0, e1, 0, va, vm, ny, nx,
+                              dx, dy, sx, sz, work, d_tt);
     HIP_TRY(c, hipGetLastError());
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     if (timed) {
