@@ -75,17 +75,24 @@ __global__ __launch_bounds__(256) void cond_taps_kernel(const double *reH, const
 // before the current chunk's MFMAs, so their latency overlaps the matrix work.
 constexpr int kTN = 64, kTR = 64, kTM = 64, kTG = kTN + kTM - 1;
 typedef float v16f __attribute__((ext_vector_type(16)));
+// Split over m: workgroup part kp of a tile sums the chunks [kp*cpp, (kp+1)*cpp) of 64 and
+// writes its partial product to part[kp] (ksplit > 1; cond_reduce_kernel adds the parts in
+// order), so a gather of few tiles still spreads over all CUs.
 __global__ __launch_bounds__(256) void cond_toeplitz_kernel(const float *__restrict__ x, int nt, int nrec,
                                                             const float *__restrict__ g, float *__restrict__ y,
-                                                            int n_ntile, int n_tiles, int per_xcd) {
+                                                            int n_ntile, int n_tiles, int per_xcd, int ksplit,
+                                                            int cpp, float *__restrict__ part) {
     __shared__ float xs[kTM][kTR + 1];
     __shared__ float gs[kTG];
     // XCD-aware order: the 8 XCDs take workgroups round-robin, so XCD b % 8 gets the
     // contiguous tile range [(b % 8) * per_xcd, ...), tiles trace-block-major: all sample
     // tiles of a trace block run on one XCD and read its traces through one L2
-    const int t = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
-    if (t >= n_tiles) return;
+    const int tt = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
+    if (tt >= n_tiles * ksplit) return;
+    const int t = tt / ksplit, kp = tt - t * ksplit;  // the parts of a tile are neighbours
     const int n0 = (t % n_ntile) * kTN, r0 = (t / n_ntile) * kTR;
+    const int mb = kp * cpp * kTM, me = min(nt, mb + cpp * kTM);
+    float *out = ksplit > 1 ? part + (size_t)kp * nrec * nt : y;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int wn = 32 * (w & 1), wr = 32 * (w >> 1);  // this wavefront's 32 x 32 sub-tile
     const int li = lane & 31, lk = lane >> 5;
@@ -110,21 +117,29 @@ __global__ __launch_bounds__(256) void cond_toeplitz_kernel(const float *__restr
         }
         if (tid < kTG) gs[tid] = gr;
     };
-    v16f acc;
+    // four independent accumulators (m steps 8q, 8q+2, 8q+4, 8q+6): a 32x32x2 MFMA waits
+    // for the previous one on the same accumulator, and with ~1 wavefront per SIMD (304
+    // workgroups for a 480-trace gather) one chain would leave the matrix core idle
+    v16f acc[4];
 #pragma unroll
-    for (int v = 0; v < 16; ++v) acc[v] = 0.f;
-    fetch(0);
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int v = 0; v < 16; ++v) acc[q][v] = 0.f;
+    fetch(mb);
     stage();
     __syncthreads();
-    for (int m0 = 0; m0 < nt; m0 += kTM) {
-        const bool more = m0 + kTM < nt;
+    for (int m0 = mb; m0 < me; m0 += kTM) {
+        const bool more = m0 + kTM < me;
         if (more) fetch(m0 + kTM);
-#pragma unroll 8
-        for (int mm = 0; mm < kTM; mm += 2) {
-            const float a = gs[wn + li - (mm + lk) + kTM - 1];  // T[n0 + wn + li][m0 + mm + lk]
-            const float b = xs[mm + lk][wr + li];               // x[m0 + mm + lk][r0 + wr + li]
-            acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc, 0, 0, 0);
-        }
+#pragma unroll 2
+        for (int m8 = 0; m8 < kTM; m8 += 8)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int mm = m8 + 2 * q;
+                const float a = gs[wn + li - (mm + lk) + kTM - 1];  // T[n0 + wn + li][m0 + mm + lk]
+                const float b = xs[mm + lk][wr + li];               // x[m0 + mm + lk][r0 + wr + li]
+                acc[q] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc[q], 0, 0, 0);
+            }
         __syncthreads();
         if (more) {
             stage();
@@ -136,8 +151,16 @@ __global__ __launch_bounds__(256) void cond_toeplitz_kernel(const float *__restr
 #pragma unroll
         for (int v = 0; v < 16; ++v) {
             const int n = n0 + wn + 8 * (v >> 2) + 4 * lk + (v & 3);
-            if (n < nt) y[(size_t)r * nt + n] = acc[v];
+            if (n < nt) out[(size_t)r * nt + n] = (acc[0][v] + acc[1][v]) + (acc[2][v] + acc[3][v]);
         }
+}
+
+__global__ void cond_reduce_kernel(const float *__restrict__ part, int ksplit, size_t n, float *__restrict__ y) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    float acc = part[e];
+    for (int k = 1; k < ksplit; ++k) acc = acc + part[(size_t)k * n + e];
+    y[e] = acc;
 }
 
 // INTRPL (akima433.f90:6-170) at one abscissa uk for the samples y(1..L) at
@@ -382,11 +405,26 @@ extern "C" int psdm_rtm_condition(psdm_rtm_ctx *c, const psdm_rtm_cond *q, int N
         void *vf;
         if (int rc = get_buf(c, "cond_filtered", (size_t)nt * NumRec * sizeof(float), &vf)) return rc;
         const int n_ntile = (nt + kTN - 1) / kTN, n_tiles = n_ntile * ((NumRec + kTR - 1) / kTR);
-        const int per_xcd = (n_tiles + 7) / 8;
+        // split the m sums so that ~1024 workgroups run (4 per CU), each over >= 4 chunks
+        const int nchunk = (nt + kTM - 1) / kTM;
+        int ksplit = std::max(1, std::min({8, (1024 + n_tiles - 1) / n_tiles, nchunk / 4}));
+        const int cpp = (nchunk + ksplit - 1) / ksplit;
+        ksplit = (nchunk + cpp - 1) / cpp;
+        const int per_xcd = (n_tiles * ksplit + 7) / 8;
+        void *vp = nullptr;
+        const size_t n = (size_t)nt * NumRec;
+        if (ksplit > 1)
+            if (int rc = get_buf(c, "cond_parts", (size_t)ksplit * n * sizeof(float), &vp)) return rc;
         hipExtLaunchKernelGGL(cond_toeplitz_kernel, dim3(8 * per_xcd), dim3(256), 0, c->stream,
-                              timed ? c->cond_ev[0] : nullptr, timed ? c->cond_ev[1] : nullptr, 0, d_field, nt, NumRec,
-                              (const float *)vt, (float *)vf, n_ntile, n_tiles, per_xcd);
+                              timed ? c->cond_ev[0] : nullptr, (timed && ksplit == 1) ? c->cond_ev[1] : nullptr, 0,
+                              d_field, nt, NumRec, (const float *)vt, (float *)vf, n_ntile, n_tiles, per_xcd, ksplit, cpp,
+                              (float *)vp);
         HIP_TRY(c, hipGetLastError());
+        if (ksplit > 1) {
+            hipExtLaunchKernelGGL(cond_reduce_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, nullptr,
+                                  timed ? c->cond_ev[1] : nullptr, 0, (const float *)vp, ksplit, n, (float *)vf);
+            HIP_TRY(c, hipGetLastError());
+        }
         src = (const float *)vf;
     }
     // taper weights w(n) = 0.5*(1 + cos(pi*n/N_taper)), pi = 4 atan(1) (filters.f90:32-40)
