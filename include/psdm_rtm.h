@@ -325,6 +325,11 @@ int psdm_rtm_nav_read(const char *path, int cols, int dir_boat, float d_shot_mod
  * sweeps from T = 1e9, T(source) = 0, bit for bit.  nx <= 8192.  Synchronous. */
 int psdm_rtm_eikonal(psdm_rtm_ctx *ctx, const float *d_vel, int ny, int nx, float dx, float dy, int nsrc,
                      const int *src_x, const int *src_z, float *d_tt);
+/* Host-array form of psdm_rtm_eikonal for the Fortran seam (dynamic_window_tt_forward /
+ * _backward's arrays): h_vel (ny, nx) and h_tt (ny, nx, nsrc) column-major host arrays.
+ * Synchronous. */
+int psdm_rtm_eikonal_host(psdm_rtm_ctx *ctx, const float *h_vel, int ny, int nx, float dx, float dy, int nsrc,
+                          const int *src_x, const int *src_z, float *h_tt);
 /* The imaging loop of calculate_PSDM_kirchoff.f90:394-447: d_img (ny, nx) device =
  * sum over receivers k (in order) of weight * (d(it+1) - d(it-1)) / (2*dt) with
  * it = 1 + floor((TT_s + TT_r(k))/dt) (skipped unless 1 < it < nt_prop), Veff = vp0,

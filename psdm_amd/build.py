@@ -62,13 +62,15 @@ CALLER_DIR = os.path.join(ROOT, "tests", "fortran")
 CALLER = os.path.join(CALLER_DIR, "_build", "rtm_caller")
 COND_CALLER = os.path.join(CALLER_DIR, "_build", "cond_caller")
 SU_CALLER = os.path.join(CALLER_DIR, "_build", "su_caller")
+TT_CALLER = os.path.join(CALLER_DIR, "_build", "tt_caller")
 
 
 def build_fortran_caller(verbose: bool = False):
     """Compiles the Fortran shim against the test fixture module mod_parfile and links
     the test callers (tests/fortran/rtm_caller.f90: the solver calls of one shot;
     tests/fortran/cond_caller.f90: the gather conditioning; tests/fortran/su_caller.f90: the
-    SU shot reader) with libpsdm_rtm.so.
+    SU shot reader; tests/fortran/tt_caller.f90: the Kirchhoff travel times, with the window
+    fixture dynamic_window_test.f90) with libpsdm_rtm.so.
     Returns the solver caller's path, or None where no Fortran compiler is installed."""
     if not os.path.exists(FLANG):
         return None
@@ -76,8 +78,10 @@ def build_fortran_caller(verbose: bool = False):
     os.makedirs(bdir, exist_ok=True)
     rpath = "$ORIGIN/" + os.path.relpath(PKG, bdir)
     common = [os.path.join(CALLER_DIR, "mod_parfile_test.f90"), FORTRAN_SHIM]
-    for main, exe in (("rtm_caller.f90", CALLER), ("cond_caller.f90", COND_CALLER), ("su_caller.f90", SU_CALLER)):
-        cmd = [FLANG, "-O2", "-J", bdir, *common, os.path.join(CALLER_DIR, main), "-L", PKG, "-lpsdm_rtm",
+    for main, exe in (("rtm_caller.f90", CALLER), ("cond_caller.f90", COND_CALLER), ("su_caller.f90", SU_CALLER),
+                      ("tt_caller.f90", TT_CALLER)):
+        extra = [os.path.join(CALLER_DIR, "dynamic_window_test.f90")]  # the shim's travel-time entries call it
+        cmd = [FLANG, "-O2", "-J", bdir, *common, *extra, os.path.join(CALLER_DIR, main), "-L", PKG, "-lpsdm_rtm",
                f"-Wl,-rpath,{rpath}", "-o", exe]
         if verbose:
             print(" ".join(cmd), flush=True)

@@ -85,6 +85,15 @@ module psdm_rtm_iface
       integer(c_int), value :: ntr, ns, nt, big_endian, shift
       real(c_float) :: h_out(*)
     end function
+    integer(c_int) function psdm_rtm_eikonal_host(ctx, h_vel, ny, nx, dx, dy, nsrc, src_x, src_z, h_tt) &
+        bind(C, name="psdm_rtm_eikonal_host")
+      import :: c_ptr, c_int, c_float
+      type(c_ptr), value :: ctx
+      real(c_float) :: h_vel(*), h_tt(*)
+      integer(c_int), value :: ny, nx, nsrc
+      real(c_float), value :: dx, dy
+      integer(c_int) :: src_x(*), src_z(*)
+    end function
     integer(c_int) function psdm_rtm_condition_host(ctx, cond, NumRec, Field_Data, Data_target) &
         bind(C, name="psdm_rtm_condition_host")
       import :: c_ptr, c_int, c_float, psdm_rtm_cond
@@ -259,4 +268,48 @@ subroutine psdm_get_shot_data(icount, raw_data, delay_data)
                                           NumRec_MCS, nt_mcs_max, nt_mcs, endianness_data, id, delay_data), &
                     'psdm_get_shot_data')
   endif
+end subroutine
+
+! Replace dynamic_window_tt_forward / dynamic_window_tt_backward (dynamic_window.f90:230-325)
+! with their own argument lists: the window of the shot from the reference's dynamic_window,
+! then the travel times of the source (TT_s) or of all receivers at once (TT_r) on the
+! device, bit for bit those of solve_eikonal_FSM.
+subroutine psdm_window_tt_forward(rank,NumRec,nxSou,nySou,nxRec,nyRec,dd,ny,nx,nx_w,Vp_w,dd_space,TT_s)
+  use psdm_rtm_iface
+  implicit none
+  integer :: rank, NumRec, ny, nx, nx_w, x_i, x_f
+  integer :: nxSou(1), nySou(1), nxRec(NumRec), nyRec(NumRec)
+  real :: dd, dd_space
+  real :: Vp_w(ny, nx_w), TT_s(ny, nx_w)
+  integer(c_int) :: nxs(1), nys(1)
+  call dynamic_window(rank, 1, NumRec, nxSou, nxRec, dd, nx, dd_space, nx_w, x_i, x_f)
+  nxs = nxSou - x_i + 1
+  nys = nySou
+  call psdm_check(psdm_rtm_eikonal_host(psdm_context(), Vp_w, ny, nx_w, dd, dd, 1_c_int, nxs, nys, TT_s), &
+                  'psdm_window_tt_forward')
+end subroutine
+
+subroutine psdm_window_tt_backward(rank,NumRec,nxSou,nySou,nxRec,nyRec,dd,ny,nx,nx_w,Vp_w,dd_space,TT_r)
+  use psdm_rtm_iface
+  implicit none
+  integer :: rank, NumRec, ny, nx, nx_w, x_i, x_f, j
+  integer :: nxSou(1), nySou(1), nxRec(NumRec), nyRec(NumRec)
+  real :: dd, dd_space
+  real :: Vp_w(ny, nx_w), TT_r(ny, nx_w, NumRec)
+  integer(c_int), allocatable :: nxr(:), nyr(:)
+  call dynamic_window(rank, 1, NumRec, nxSou, nxRec, dd, nx, dd_space, nx_w, x_i, x_f)
+  allocate(nxr(NumRec), nyr(NumRec))
+  nxr = nxRec - x_i + 1
+  nyr = nyRec
+  do j = 1, NumRec  ! the reference stops on receivers outside the window (:307-315)
+    if (nxr(j) < 1 .or. nxr(j) > nx_w) then
+      write(*,*) "nxr out of bounds"
+      write(*,*) j, nxr(j), nx_w
+      stop
+    endif
+    if (nyr(j) < 1 .or. nyr(j) > ny) stop "nyr out of bounds"
+  enddo
+  call psdm_check(psdm_rtm_eikonal_host(psdm_context(), Vp_w, ny, nx_w, dd, dd, NumRec, nxr, nyr, TT_r), &
+                  'psdm_window_tt_backward')
+  deallocate(nxr, nyr)
 end subroutine

@@ -334,6 +334,22 @@ extern "C" int psdm_rtm_eikonal(psdm_rtm_ctx *c, const float *d_vel, int ny, int
     return 0;
 }
 
+extern "C" int psdm_rtm_eikonal_host(psdm_rtm_ctx *c, const float *h_vel, int ny, int nx, float dx, float dy,
+                                     int nsrc, const int *src_x, const int *src_z, float *h_tt) {
+    if (!c) return PSDM_RTM_EINVAL;
+    if (!h_vel || !h_tt) return fail(c, PSDM_RTM_EINVAL, "eikonal_host: null argument");
+    if (ny < 1 || nx < 1 || nsrc < 1) return fail(c, PSDM_RTM_EINVAL, "eikonal_host: need ny, nx, nsrc >= 1");
+    if (int rc = set_device(c)) return rc;
+    void *dv, *dt;
+    const size_t n = (size_t)nx * ny;
+    if (int rc = get_buf(c, "eikh_vel", n * sizeof(float), &dv)) return rc;
+    if (int rc = get_buf(c, "eikh_tt", n * (size_t)nsrc * sizeof(float), &dt)) return rc;
+    HIP_TRY(c, hipMemcpyAsync(dv, h_vel, n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    if (int rc = psdm_rtm_eikonal(c, (const float *)dv, ny, nx, dx, dy, nsrc, src_x, src_z, (float *)dt)) return rc;
+    HIP_TRY(c, hipMemcpy(h_tt, dt, n * (size_t)nsrc * sizeof(float), hipMemcpyDeviceToHost));
+    return 0;
+}
+
 extern "C" int psdm_rtm_kirchhoff_image(psdm_rtm_ctx *c, const float *d_tts, const float *d_ttr, const float *d_data,
                                         int ld, int nrec, int nt_prop, int ny, int nx, float dd, float dt, float vp0,
                                         float theta_max, float theta_taper, float wmax, float *d_img) {
