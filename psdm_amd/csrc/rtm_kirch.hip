@@ -36,6 +36,7 @@ namespace {
 
 constexpr float kBig = 1.0e9f;  // initial travel time (BIG, solve_eikonal_FSM.f90:11)
 constexpr int kMaxCols = 8;     // model columns per thread (nx <= 8 * 1024)
+constexpr int kTrash = 1024;    // per-solve sink for the stores of lanes without a cell (>= blockDim.x)
 
 // local_solver (solve_eikonal_FSM.f90:63-79), the reference's FP32 operations in order
 __device__ __forceinline__ float local_solver(float a, float b, float dx, float dy, float v_inv) {
@@ -115,6 +116,11 @@ __device__ void relayout(const float *__restrict__ src, int sk, float *__restric
         }
 }
 
+// Offset in a sweep's layout of the cell of step (diagonal) D, sweep-order column c.
+__device__ __forceinline__ unsigned ring_addr(bool fwd, int D, int c, int nx, int dmax) {
+    return fwd ? (unsigned)(D * nx + c) : (unsigned)((dmax - D) * nx + (nx - 1 - c));
+}
+
 // K model columns per thread; P: prefetch depth, in rows, of each column's old travel times
 // and 1/v.
 //
@@ -136,7 +142,7 @@ __global__ __launch_bounds__(1024) void eikonal_fsm_kernel(const float *__restri
     float *tile = lds + 4 * nx;           // relayout tile, 64 x 65
     const int B = blockDim.x, tid = threadIdx.x;
     const size_t nd = (size_t)(nx + ny - 1) * nx;  // diagonal-major array (padding cells never read)
-    float *TA = work + (size_t)blockIdx.x * 2 * nd, *TM = TA + nd;
+    float *TA = work + (size_t)blockIdx.x * (2 * nd + kTrash), *TM = TA + nd;  // then kTrash words
     float *T = tt + (size_t)blockIdx.x * nx * ny;
     for (size_t e = tid; e < nd; e += B) TA[e] = kBig;
     __syncthreads();
@@ -157,6 +163,7 @@ __global__ __launch_bounds__(1024) void eikonal_fsm_kernel(const float *__restri
         const bool fwd = sweep <= 2;
         float *Ts = (sweep == 1 || sweep == 3) ? TA : TM;
         const float *Vs = (sweep == 1 || sweep == 3) ? va : vm;
+        const unsigned sink = (unsigned)(Ts == TA ? 2 * nd : nd);  // the trash words, from Ts
         const int dmax = nx + ny - 2;
         float tq[K][R], vq[K][R], prv[K];
 #pragma unroll
@@ -164,62 +171,93 @@ __global__ __launch_bounds__(1024) void eikonal_fsm_kernel(const float *__restri
             const int c = tid + k * B;
             prv[k] = 0.f;
 #pragma unroll
-            for (int m = 0; m < R; ++m) {
-                const int row = m - c;  // row of ring slot m at step 0 (diagonal m)
+            for (int m = 0; m < R; ++m) {  // slot m: diagonal m (row m - c)
                 tq[k][m] = vq[k][m] = 0.f;
-                if (c < nx && row >= 0 && row < ny) {
-                    const unsigned q = fwd ? (unsigned)(m * nx + c) : (unsigned)((dmax - m) * nx + (nx - 1 - c));
+                if (K <= 2 || (c < nx && m - c >= 0 && m - c < ny)) {
+                    const unsigned q = K <= 2 ? ring_addr(fwd, min(m, dmax), min(c, nx - 1), nx, dmax)
+                                              : ring_addr(fwd, m, c, nx, dmax);
                     tq[k][m] = Ts[q];
                     vq[k][m] = Vs[q];
                 }
             }
         }
         const int nsteps = nx + ny - 1;
-        // the step loop unrolled by the ring size R: in sub-step u the slots are fixed
-        // (row r + m in slot (u + m) % R), so no register of an in-flight load is moved
-        for (int d0 = 0; d0 < nsteps; d0 += R) {
+        // One wavefront step: diagonal d, ring sub-step u (row r + m of a column sits in slot
+        // (u + m) % R, so no register of an in-flight load is ever moved).
+        auto step = [&](const int d, const int u) {
 #pragma unroll
-            for (int u = 0; u < R; ++u) {
-                const int d = d0 + u;
-                if (d >= nsteps) break;  // uniform over the workgroup
+            for (int k = 0; k < K; ++k) {
+                const int c = tid + k * B;
+                if (c < nx) od[(d & 1) * nx + c] = tq[k][(u + 1) % R];
+            }
+            lds_barrier();
 #pragma unroll
-                for (int k = 0; k < K; ++k) {
-                    const int c = tid + k * B;
-                    if (c < nx) od[(d & 1) * nx + c] = tq[k][(u + 1) % R];
-                }
-                lds_barrier();
-#pragma unroll
-                for (int k = 0; k < K; ++k) {
-                    const int c = tid + k * B, r = d - c;
-                    if (c >= nx) continue;
-                    if (r >= 0 && r < ny) {
-                        const float self = tq[k][u];
-                        // neighbours in sweep order: previous column (new, LDS), next column
-                        // (old, its lookahead in LDS), previous row (new, register), next row (old)
-                        const float cp = c > 0 ? nw[((d - 1) & 1) * nx + c - 1] : self;
-                        const float cn = c + 1 < nx ? od[(d & 1) * nx + c + 1] : self;
-                        const float rp = r > 0 ? prv[k] : self;
-                        const float rn = r + 1 < ny ? tq[k][(u + 1) % R] : self;
-                        float tnew = self;
-                        if (self != 0.0f) {  // the source point keeps 0
-                            // a = min(T(j,i-1), T(j,i+1)), b = min(T(j-1,i), T(j+1,i)): min is
-                            // symmetric, so the sweep-order neighbours give the same values
-                            const float a = fminf(cp, cn), b = fminf(rp, rn);
-                            const float t = local_solver(a, b, dx, dy, vq[k][u]);
-                            if (t < self) tnew = t;
-                        }
-                        Ts[fwd ? (unsigned)(d * nx + c) : (unsigned)((dmax - d) * nx + (nx - 1 - c))] = tnew;
-                        prv[k] = tnew;
-                        nw[(d & 1) * nx + c] = tnew;
+            for (int k = 0; k < K; ++k) {
+                const int c = tid + k * B, r = d - c;
+                const bool live = c < nx && r >= 0 && r < ny;
+                float tnew = 0.0f;
+                if (live) {
+                    const float self = tq[k][u];
+                    // neighbours in sweep order: previous column (new, LDS), next column
+                    // (old, its lookahead in LDS), previous row (new, register), next row (old)
+                    const float cp = c > 0 ? nw[((d - 1) & 1) * nx + c - 1] : self;
+                    const float cn = c + 1 < nx ? od[(d & 1) * nx + c + 1] : self;
+                    const float rp = r > 0 ? prv[k] : self;
+                    const float rn = r + 1 < ny ? tq[k][(u + 1) % R] : self;
+                    tnew = self;
+                    if (self != 0.0f) {  // the source point keeps 0
+                        // a = min(T(j,i-1), T(j,i+1)), b = min(T(j-1,i), T(j+1,i)): min is
+                        // symmetric, so the sweep-order neighbours give the same values
+                        const float a = fminf(cp, cn), b = fminf(rp, rn);
+                        const float t = local_solver(a, b, dx, dy, vq[k][u]);
+                        if (t < self) tnew = t;
                     }
-                    // slot u now takes row r + R (diagonal d + R)
+                    prv[k] = tnew;
+                    nw[(d & 1) * nx + c] = tnew;
+                }
+                // slot u now takes row r + R (diagonal d + R)
+                if constexpr (K <= 2) {
+                    // Every lane stores and reloads, outside any branch: with the same memory
+                    // instructions on every path the compiler's vmcnt waits count back through
+                    // the ring (after a lane-conditional load it assumes none were issued
+                    // since, and drains the queue every step).  A lane without a cell stores
+                    // into its trash word; the reload is clamped into the array (a row
+                    // outside the grid is never consumed).  Wider rings (K >= 4) would spill
+                    // with the extra address registers and keep the conditional form.
+                    const int cl = min(c, nx - 1);
+                    Ts[live ? ring_addr(fwd, d, cl, nx, dmax) : sink + (unsigned)tid] = tnew;
+                    const unsigned q = ring_addr(fwd, min(d + R, dmax), cl, nx, dmax);
+                    tq[k][u] = Ts[q];
+                    vq[k][u] = Vs[q];
+                } else {
+                    if (live) Ts[ring_addr(fwd, d, c, nx, dmax)] = tnew;
                     const int rl = r + R;
-                    if (rl >= 0 && rl < ny) {
-                        const unsigned q =
-                            fwd ? (unsigned)((d + R) * nx + c) : (unsigned)((dmax - d - R) * nx + (nx - 1 - c));
+                    if (c < nx && rl >= 0 && rl < ny) {
+                        const unsigned q = ring_addr(fwd, d + R, c, nx, dmax);
                         tq[k][u] = Ts[q];
                         vq[k][u] = Vs[q];
                     }
+                }
+            }
+        };
+        if constexpr (K <= 2) {
+            // Whole blocks of R steps, then the tail: a uniform early exit inside the unrolled
+            // block would join the loop's back edge with paths that issued fewer loads, and
+            // the vmcnt waits of the next block's first steps would drain the ring again.
+            const int nfull = nsteps - nsteps % R;
+            for (int d0 = 0; d0 < nfull; d0 += R) {
+#pragma unroll
+                for (int u = 0; u < R; ++u) step(d0 + u, u);
+            }
+#pragma unroll
+            for (int u = 0; u < R - 1; ++u)
+                if (nfull + u < nsteps) step(nfull + u, u);
+        } else {
+            for (int d0 = 0; d0 < nsteps; d0 += R) {
+#pragma unroll
+                for (int u = 0; u < R; ++u) {
+                    if (d0 + u >= nsteps) break;  // uniform over the workgroup
+                    step(d0 + u, u);
                 }
             }
         }
@@ -293,7 +331,7 @@ extern "C" int psdm_rtm_eikonal(psdm_rtm_ctx *c, const float *d_vel, int ny, int
     const size_t nd = (size_t)(nx + ny - 1) * nx;
     if (int rc = get_buf(c, "eik_src", 2 * (size_t)nsrc * sizeof(int), &d_src)) return rc;
     if (int rc = get_buf(c, "eik_vinv", 2 * nd * sizeof(float), &d_v)) return rc;
-    if (int rc = get_buf(c, "eik_work", 2 * nd * (size_t)nsrc * sizeof(float), &d_work)) return rc;
+    if (int rc = get_buf(c, "eik_work", (2 * nd + kTrash) * (size_t)nsrc * sizeof(float), &d_work)) return rc;
     std::vector<int> h(2 * (size_t)nsrc);
     std::memcpy(h.data(), src_x, nsrc * sizeof(int));
     std::memcpy(h.data() + nsrc, src_z, nsrc * sizeof(int));
