@@ -194,6 +194,9 @@ __global__ __launch_bounds__(1024) void eikonal_fsm_kernel(const float *__restri
 #pragma unroll
             for (int k = 0; k < K; ++k) {
                 const int c = tid + k * B, r = d - c;
+                if constexpr (K > 2) {
+                    if (c >= nx) continue;
+                }
                 const bool live = c < nx && r >= 0 && r < ny;
                 float tnew = 0.0f;
                 if (live) {
