@@ -36,7 +36,9 @@ namespace {
 
 constexpr float kBig = 1.0e9f;  // initial travel time (BIG, solve_eikonal_FSM.f90:11)
 constexpr int kMaxCols = 8;     // model columns per thread (nx <= 8 * 1024)
-constexpr int kTrash = 1024;    // per-solve sink for the stores of lanes without a cell (>= blockDim.x)
+constexpr int kMaxBlock = 1024;  // eikonal workgroup size limit (its __launch_bounds__)
+constexpr int kTrash = 1024;     // per-solve sink for the stores of lanes without a cell, one word per thread
+static_assert(kTrash >= kMaxBlock, "every thread of a workgroup needs its own trash word");
 
 // local_solver (solve_eikonal_FSM.f90:63-79), the reference's FP32 operations in order
 __device__ __forceinline__ float local_solver(float a, float b, float dx, float dy, float v_inv) {
@@ -132,7 +134,7 @@ __device__ __forceinline__ unsigned ring_addr(bool fwd, int D, int c, int nx, in
 // sweeps the travel times are re-laid out through LDS tiles (A -> M -> A -> M), and after
 // the last sweep M -> the model's column-major output.
 template <int K, int P>
-__global__ __launch_bounds__(1024) void eikonal_fsm_kernel(const float *__restrict__ va, const float *__restrict__ vm,
+__global__ __launch_bounds__(kMaxBlock) void eikonal_fsm_kernel(const float *__restrict__ va, const float *__restrict__ vm,
                                                            int ny, int nx, float dx, float dy,
                                                            const int *__restrict__ src_x,
                                                            const int *__restrict__ src_z, float *__restrict__ work,
@@ -321,7 +323,7 @@ extern "C" int psdm_rtm_eikonal(psdm_rtm_ctx *c, const float *d_vel, int ny, int
                                 const int *src_x, const int *src_z, float *d_tt) {
     if (!c) return PSDM_RTM_EINVAL;
     if (!d_vel || !d_tt || !src_x || !src_z) return fail(c, PSDM_RTM_EINVAL, "eikonal: null argument");
-    if (ny < 1 || nx < 1 || nx > kMaxCols * 1024 || nsrc < 1)
+    if (ny < 1 || nx < 1 || nx > kMaxCols * kMaxBlock || nsrc < 1)
         return fail(c, PSDM_RTM_EINVAL, "eikonal: need ny >= 1, 1 <= nx <= 8192, nsrc >= 1");
     if (!(dx > 0.f) || !(dy > 0.f)) return fail(c, PSDM_RTM_EINVAL, "eikonal: dx, dy must be > 0");
     if ((long long)(nx + ny - 1) * nx > 2147483647LL - 1024)
@@ -339,7 +341,7 @@ extern "C" int psdm_rtm_eikonal(psdm_rtm_ctx *c, const float *d_vel, int ny, int
     std::memcpy(h.data(), src_x, nsrc * sizeof(int));
     std::memcpy(h.data() + nsrc, src_z, nsrc * sizeof(int));
     HIP_TRY(c, hipMemcpyAsync(d_src, h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
-    const int B = nx >= 1024 ? 1024 : (nx + 63) / 64 * 64;
+    const int B = nx >= kMaxBlock ? kMaxBlock : (nx + 63) / 64 * 64;
     const int K = (nx + B - 1) / B;
     const unsigned lds = (4u * (unsigned)nx + (unsigned)(kTile * kTileLd)) * (unsigned)sizeof(float);
     const bool timed = c->timing != 0;
