@@ -322,7 +322,9 @@ int psdm_rtm_nav_read(const char *path, int cols, int dir_boat, float d_shot_mod
  * device, column-major (the window Vp_w); for each of nsrc source points (src_x, src_z:
  * host, 1-based column and row in the window) the travel-time field, d_tt (ny, nx, nsrc)
  * device, column-major = TT_r(ny, nx_w, NumRec) (or TT_s for one source).  Four fast
- * sweeps from T = 1e9, T(source) = 0, bit for bit.  nx <= 8192.  Synchronous. */
+ * sweeps from T = 1e9, T(source) = 0, bit for bit.  nx <= 8192.  Synchronous.
+ * Device workspace held by the ctx: 4 * (2 * (nx + ny - 1) * nx + 1024) bytes per source
+ * plus 8 * (nx + ny - 1) * nx for 1/v (3.6 GB for 481 sources in a 601 x 715 window). */
 int psdm_rtm_eikonal(psdm_rtm_ctx *ctx, const float *d_vel, int ny, int nx, float dx, float dy, int nsrc,
                      const int *src_x, const int *src_z, float *d_tt);
 /* Host-array form of psdm_rtm_eikonal for the Fortran seam (dynamic_window_tt_forward /
