@@ -42,6 +42,7 @@ namespace {
 
 constexpr int kSuHdr = 60;  // header words per trace (240 bytes, size_su_header)
 constexpr int kSuBlock = 1024;  // output words per workgroup: 4 per thread, lane-contiguous
+constexpr int kSuMaxNs = (2147483647 - 4 * kSuHdr) / 4;  // trace words and bytes stay int
 
 __device__ __forceinline__ unsigned order(unsigned w, int big) { return big ? __builtin_bswap32(w) : w; }
 __device__ __forceinline__ unsigned order16(unsigned h, int big) {  // a 16-bit field at the word's low address
@@ -144,8 +145,8 @@ extern "C" int psdm_rtm_su_decode(psdm_rtm_ctx *c, const void *d_su, long long n
     if (!c) return PSDM_RTM_EINVAL;
     if (!d_su || !d_out) return fail(c, PSDM_RTM_EINVAL, "su_decode: null argument");
     if (!aligned4(d_su)) return fail(c, PSDM_RTM_EINVAL, "su_decode: the SU bytes must be 4-byte aligned");
-    if (ntr < 1 || ns < 1 || nt < 1 || nt > ns)
-        return fail(c, PSDM_RTM_EINVAL, "su_decode: need ntr >= 1 and 1 <= nt <= ns");
+    if (ntr < 1 || ns < 1 || nt < 1 || nt > ns || ns > kSuMaxNs)
+        return fail(c, PSDM_RTM_EINVAL, "su_decode: need ntr >= 1 and 1 <= nt <= ns < 2^29");
     if (nbytes < (long long)ntr * (kSuHdr + ns) * 4)
         return fail(c, PSDM_RTM_EINVAL, "su_decode: nbytes < ntr * (240 + 4*ns)");
     if ((long long)ntr * nt > 2147483647LL - 4 * kSuBlock)
@@ -175,8 +176,8 @@ extern "C" int psdm_rtm_su_read_host(psdm_rtm_ctx *c, const char *path, long lon
                                      int big_endian, int shift, float *h_out) {
     if (!c) return PSDM_RTM_EINVAL;
     if (!path || !h_out) return fail(c, PSDM_RTM_EINVAL, "su_read_host: null argument");
-    if (ntr < 1 || ns < 1 || nt < 1 || nt > ns || offset < 0)
-        return fail(c, PSDM_RTM_EINVAL, "su_read_host: need ntr >= 1, 1 <= nt <= ns, offset >= 0");
+    if (ntr < 1 || ns < 1 || nt < 1 || nt > ns || ns > kSuMaxNs || offset < 0)
+        return fail(c, PSDM_RTM_EINVAL, "su_read_host: need ntr >= 1, 1 <= nt <= ns < 2^29, offset >= 0");
     const size_t nbytes = (size_t)ntr * (kSuHdr + ns) * 4;
     FILE *f = std::fopen(path, "rb");
     if (!f) return fail(c, PSDM_RTM_EINVAL, std::string("su_read_host: cannot open ") + path);
@@ -211,7 +212,8 @@ extern "C" int psdm_rtm_su_header_i32(psdm_rtm_ctx *c, const void *d_su, long lo
                                       int big_endian, int pos, int *d_out) {
     if (!c) return PSDM_RTM_EINVAL;
     if (!d_su || !d_out) return fail(c, PSDM_RTM_EINVAL, "su_header_i32: null argument");
-    if (ntr < 1 || ns < 1) return fail(c, PSDM_RTM_EINVAL, "su_header_i32: need ntr >= 1, ns >= 1");
+    if (ntr < 1 || ns < 1 || ns > kSuMaxNs)
+        return fail(c, PSDM_RTM_EINVAL, "su_header_i32: need ntr >= 1, 1 <= ns < 2^29");
     if (pos < 1 || pos > (kSuHdr + ns) * 4 - 3)
         return fail(c, PSDM_RTM_EINVAL, "su_header_i32: byte position outside the trace");
     if (nbytes < (long long)ntr * (kSuHdr + ns) * 4)

@@ -131,5 +131,10 @@ def test_rejects_bad_arguments(ctx):
         su.decode(ctx, d[1:], 1, 100)
     with pytest.raises(RtmError, match="byte position"):
         su.header_i32(ctx, d, 10, 100, 638)
+    # a trace of 240 + 4*ns bytes must stay below 2^31 (int word and byte offsets)
+    with pytest.raises(RtmError, match="2\\^29"):
+        su.decode(ctx, d, 1, 1 << 29, nt=1)
+    with pytest.raises(RtmError, match="2\\^29"):
+        su.header_i32(ctx, d, 1, 1 << 29, 9)
     with pytest.raises(RtmError, match="65535"):
         su.encode(ctx, torch.zeros((1, 65536), device="cuda:0"), 1.0, 1.0)
