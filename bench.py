@@ -29,21 +29,41 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def host_cores():
+    """(cores to use, this host's physical cores, logical CPUs): every CPU this process may
+    run on, at most 16 (the host cores allotted to one GPU of the box)."""
+    avail = len(os.sched_getaffinity(0))
+    phys = set()
+    try:
+        pid = core = None
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("physical id"):
+                pid = line.split(":")[1].strip()
+            elif line.startswith("core id"):
+                core = line.split(":")[1].strip()
+                phys.add((pid, core))
+    except OSError:
+        pass
+    return min(avail, 16), len(phys) or None, os.cpu_count()
+
+
 def cpu_leg(ny, nx_w, dPML, dx, dt, nprocs, nt_sample, nt_full):
     from oracle import cpu_baseline
 
     r = cpu_baseline.run(ny, nx_w, nt_sample, dPML, dx, dt, nprocs)
     dn = dPML + 6
     upd_shot = 2 * (ny + 2 * dn) * (nx_w + 2 * dn) * (nt_full - 1)
+    _, phys, logical = host_cores()
     return {
         "value": round(r["gupd_per_s"], 4),
         "unit": "Gpt-updates/s",
         "shots_per_s": round(r["gupd_per_s"] * 1e9 / upd_shot, 6),
         "cores": r["cores"],
         "kind": r["kind"],
-        "sample": (f"{r['cores']} independent processes, each forward+backward of one {ny}x{nx_w} window "
-                   f"(dPML {dPML}) over {nt_sample} of the {nt_full} steps; rate = updates / slowest rank time "
-                   f"({r['rank_s']:.1f} s)"),
+        "host": {"cpus_allowed": len(os.sched_getaffinity(0)), "physical_cores": phys, "logical_cpus": logical},
+        "sample": (f"{r['cores']} independent single-threaded processes (one per core, as mpirun ranks), each "
+                   f"forward+backward of one {ny}x{nx_w} window (dPML {dPML}) over {nt_sample} of the {nt_full} "
+                   f"steps; rate = updates / slowest rank time ({r['rank_s']:.1f} s)"),
     }
 
 
@@ -87,6 +107,58 @@ def accuracy_check(ctx):
     return {"workload": f"single401 shot, {ny}x{nx} window, {nt} steps", "reference": "FP64 oracle fixture",
             "gather_matches_fixture": bool(same_gather), "image_rel_l2_vs_fp64": rel_l2(k, fx["image_fp64"]),
             "traces_rel_l2_vs_fp64": rel_l2(Rec, fx["traces_fp64"])}
+
+
+ACC_BAR = 1e-4  # image rel-L2 against the FP64 oracle that the batched/served forms are held to
+
+
+def accuracy_line(ctx, torch, dev, W, geos, d_mig, d_src, obs, cfg):
+    """Shot 0 of the line at its full record length (601 x 715 window, nt 6000): the gather
+    modelled on the device (true minus water-only model), migrated by the plain form and by
+    the engine (launch shape `cfg`, one resident shot), and the images and the plain form's
+    traces compared with the FP64 oracle's (tests/golden/accuracy_mcs2001_fp64.npz)."""
+    import numpy as np
+
+    from psdm_amd.engine import Engine
+    from psdm_amd.solvers import backward_device, forward_device
+    from tests.cases import rel_l2
+
+    fx = np.load(os.path.join(ROOT, "tests", "golden", "accuracy_mcs2001_fp64.npz"))
+    m, sh, nt = W["model"], W["shots"][0], W["nt"]
+    assert int(fx["nt"]) == nt and int(fx["x_i"]) == sh.x_i and int(fx["nx_w"]) == sh.nx_w
+    ny = m.nz
+    d = obs[0].cpu().numpy()  # (NumRec, nt): Data(nt, NumRec) in Fortran order
+    sumsq = float(np.sum(d.astype(np.float64) ** 2))
+    same_gather = (abs(sumsq - float(fx["data_sumsq"])) <= 1e-12 * float(fx["data_sumsq"]) and
+                   np.array_equal(d.reshape(-1)[fx["data_pick"]], fx["data_sample"]))
+    win = d_mig[sh.x_i - 1:sh.x_f]
+    d_rec = torch.empty((sh.nxRec.size, nt), device=dev)
+    d_img = torch.zeros((sh.nx_w, ny), device=dev)
+    forward_device(ctx, geos[0], win, d_src, d_rec, key=-7)
+    backward_device(ctx, geos[0], win, obs[0].flip(1).contiguous(), d_img, key=-7)
+    ctx.sync()
+    img_plain = d_img.cpu().numpy().T
+    rec = d_rec.cpu().numpy().T[:, fx["trace_channels"]]
+    d_img2 = torch.zeros((sh.nx_w, ny), device=dev)
+    eng = Engine(ctx, d_mig, ny, m.nx, nslots=1, max_nx=sh.nx_w, max_nt=nt, dPML=W["dPML"], fs=W["fs"], dmodel=m.dx,
+                 dt=W["dt"], tile_x=cfg["tile_x"], stream_w=cfg.get("stream_w", 0), groups=1,
+                 chunk_steps=cfg.get("chunk_steps", 64), **({"steps_per_launch": cfg["steps_per_launch"]}
+                                                          if cfg.get("steps_per_launch", 1) != 1 else {}))
+    eng.submit(**job_of(W, 0, d_src, obs), d_image_out=d_img2)
+    eng.drain()
+    eng.close()
+    ctx.sync()
+    img_batched = d_img2.cpu().numpy().T
+    e_plain, e_batched = rel_l2(img_plain, fx["image_fp64"]), rel_l2(img_batched, fx["image_fp64"])
+    return {"workload": f"mcs2001 shot 0, {ny}x{sh.nx_w} window, nt {nt} (full record)",
+            "reference": "FP64 oracle fixture (tests/golden/accuracy_mcs2001_fp64.npz)",
+            "gather_matches_fixture": bool(same_gather),
+            "plain": {"image_rel_l2_vs_fp64": e_plain, "traces_rel_l2_vs_fp64": rel_l2(rec, fx["traces_fp64"])},
+            "batched": {"image_rel_l2_vs_fp64": e_batched},
+            "batched_image_equals_plain": bool(np.array_equal(img_plain, img_batched)),
+            "bar": ACC_BAR, "within_bar": bool(max(e_plain, e_batched) <= ACC_BAR),
+            "note": "both forms reproduce the reference's FP32 arithmetic bit for bit; the distance to FP64 is the "
+                    "FP32 rounding the reference's own solver accumulates over the record"}
 
 
 def model_gathers(ctx, torch, dev, W, geos_model, d_true, d_wat, d_src, nshots):
@@ -764,7 +836,7 @@ def bench_served(args):
     from psdm_amd._native import Context
     from psdm_amd.engine import Engine
     from psdm_amd.serve import Request, Server, poisson_arrivals
-    from psdm_amd.solvers import ricker_mcs
+    from psdm_amd.solvers import ricker_mcs, ricker_was
     from psdm_amd.workloads import _mcs_shot, velocity_model, was_gathers
 
     dev = torch.device("cuda:0")
@@ -783,7 +855,7 @@ def bench_served(args):
     d_vm = torch.from_numpy(np.ascontiguousarray(mcs_model.migration.T)).to(dev)
     d_vw = torch.from_numpy(np.ascontiguousarray(was_model.migration.T)).to(dev)
     s_m = torch.from_numpy(ricker_mcs(nt_m, dt_m, 15.0, 1.2 / 15.0)).to(dev)
-    s_w = torch.from_numpy(ricker_mcs(nt_w, dt_w, 8.0, 1.2 / 8.0)).to(dev)
+    s_w = torch.from_numpy(ricker_was(nt_w, dt_w, 8.0)).to(dev)  # Ricker_was: t0 = 1.5/f0
     obs_m = random_gathers(torch, dev, mcs_shots, [nt_m] * nmcs, seed=3)
     obs_w = random_gathers(torch, dev, was_shots, [nt_w] * len(was_shots), seed=4)
     torch.cuda.synchronize()
@@ -852,11 +924,16 @@ def main():
     ap.add_argument("--warmup", type=int, default=1, help="untimed shots before the plain timed region")
     ap.add_argument("--tile-x", type=int, default=8, help="plain form CPML tile width (8, 16 or 32)")
     ap.add_argument("--timing-every", type=int, default=16, help="sample every n-th step kernel with HIP events")
-    ap.add_argument("--cpu-procs", type=int, default=16)
-    ap.add_argument("--cpu-steps", type=int, default=200)
+    ap.add_argument("--cpu-procs", type=int, default=0,
+                    help="CPU baseline processes (0: every core this process may run on, at most 16, the host "
+                         "cores one GPU of the box is allotted)")
+    ap.add_argument("--cpu-steps", type=int, default=1200, help="CPU baseline: time steps of the sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="batched form with the default launch shape")
+    ap.add_argument("--engine-cfg", default=None,
+                    help="batched form: JSON launch shape (keys nslots, tile_x, stream_w, grid, groups, "
+                         "steps_per_launch) used instead of the autotuner's (e.g. to profile a tuned run's shape)")
     ap.add_argument("--forms", default="plain,batched")
     ap.add_argument("--workload", default="mcs2001",
                     choices=["mcs2001", "uneven", "mixed", "served", "condition", "poststack", "su", "kirchhoff"],
@@ -900,8 +977,9 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        log(f"[bench] cpu baseline: {args.cpu_procs} procs x {args.cpu_steps} steps")
-        cpu = cpu_leg(ny, nxw, dPML, m.dx, dt, args.cpu_procs, args.cpu_steps, nt)
+        procs = args.cpu_procs or host_cores()[0]
+        log(f"[bench] cpu baseline: {procs} procs x {args.cpu_steps} steps")
+        cpu = cpu_leg(ny, nxw, dPML, m.dx, dt, procs, args.cpu_steps, nt)
         log(f"[bench] cpu baseline {cpu['value']} Gupd/s ({cpu['kind']})")
 
     import numpy as np
@@ -952,7 +1030,9 @@ def main():
     if "batched" in forms:
         cfg = dict(nslots=16, tile_x=16, stream_w=4, grid=0, groups=2, chunk_steps=64)
         tuned = None
-        if not args.no_tune:
+        if args.engine_cfg:
+            cfg.update(json.loads(args.engine_cfg))
+        elif not args.no_tune:
             from psdm_amd.autotune import tune
 
             jobs = [job_of(W, k, d_src, obs) for k in range(nshots)]
@@ -966,7 +1046,8 @@ def main():
         # device time per batched step (events around the captured chunks); one step is `groups`
         # concurrent dispatches of the batch kernel, one per slot group
         per_launch_ms = B["launch_ms"]
-        bytes_launch = step_b * st["slot_steps"] / max(st["steps_issued"], 1)
+        shot_steps_launch = st["slot_steps"] / max(st["steps_issued"], 1)
+        bytes_launch = step_b * shot_steps_launch
         ach = bytes_launch / (per_launch_ms * 1e-3) / 1e9
         batched = {"shots_per_s": round(args.steps / B["elapsed"], 4),
                    "gpt_updates_per_s": round(args.steps * upd_shot / B["elapsed"] / 1e9, 3),
@@ -976,6 +1057,7 @@ def main():
                                 "mean_kernel_ms": round(per_launch_ms, 5), "timed_launches": B["n_launch"],
                                 "concurrent_dispatches_per_launch": cfg["groups"],
                                 "algorithmic_bytes_per_launch": round(bytes_launch),
+                                "shot_steps_per_launch": round(shot_steps_launch, 4),
                                 "wall_ms_per_launch": round(1e3 * B["elapsed"] / max(st["steps_issued"], 1), 5),
                                 "note": "one launch = one step of the resident shots, issued as concurrent_dispatches_"
                                         "per_launch dispatches (slot groups); mean_kernel_ms: HIP events around each "
@@ -987,28 +1069,37 @@ def main():
             batched["fold_equal"] = bool(torch.equal(B["fold"], P["fold"]))
         log(f"[bench] batched {batched['shots_per_s']} shots/s")
 
-    def traffic_of(name):
-        """HBM bytes per launch measured with rocprofv3 counters (profiles/<name>)."""
+    def traffic_of(name, key):
+        """HBM bytes measured with rocprofv3 counters (profiles/<name>), per `key` unit."""
         import re
 
         tpath = os.path.join(ROOT, "profiles", name)
         if not os.path.exists(tpath):
             return None
-        mt = re.search(r"^traffic_bytes_per_launch = (\d+)", open(tpath).read(), re.M)
+        mt = re.search(rf"^{key} = (\d+)", open(tpath).read(), re.M)
         return int(mt.group(1)) if mt else None
-
 
     acc = None
     if not args.no_accuracy:
-        log("[bench] accuracy check vs the FP64 oracle fixture")
-        acc = accuracy_check(ctx)
+        log("[bench] accuracy check vs the FP64 oracle fixtures")
+        acc = {"short": accuracy_check(ctx)}
+        if 0 in obs:
+            acc["full"] = accuracy_line(ctx, torch, dev, W, geos, d_mig, d_src, obs,
+                                        batched["config"] if batched else dict(tile_x=16, stream_w=4))
+            log(f"[bench] full-record accuracy: plain {acc['full']['plain']['image_rel_l2_vs_fp64']:.3g}, "
+                f"batched {acc['full']['batched']['image_rel_l2_vs_fp64']:.3g} (bar {ACC_BAR})")
 
     head = batched or plain
     form = "batched" if batched else "plain"
     if plain is not None:
-        plain["roofline"]["traffic"] = traffic_of("traffic_rtm_step_kernel_mcs2001.md")
+        plain["roofline"]["traffic"] = traffic_of("traffic_rtm_step_kernel_mcs2001.md", "traffic_bytes_per_launch")
     if batched is not None:
-        batched["roofline"]["traffic"] = traffic_of("traffic_rtm_batch_kernel_mcs2001.md")
+        # counters give bytes per shot-step (one launch's bytes / its busy slots): scaled to the
+        # shots of the launches timed here, like algorithmic_bytes_per_launch
+        per_shot_step = traffic_of("traffic_rtm_batch_kernel_mcs2001.md", "traffic_bytes_per_shot_step")
+        spl = batched["roofline"]["shot_steps_per_launch"]
+        batched["roofline"]["traffic"] = round(per_shot_step * spl) if per_shot_step else None
+        batched["roofline"]["traffic_per_shot_step"] = per_shot_step
     roof = dict(head["roofline"])
     out = {
         "metric": "migrated shots/s + Gpt-updates/s, batched vs plain, at rel-L2<=1e-4; p99 latency",
@@ -1023,8 +1114,10 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded 2001x601 model; gathers modelled on device, direct wave removed)",
-        "config": {"workload": "mcs2001 line: 2001x601 @10 m, 64 shots @25 m, 480 ch @12.5 m, near offset 150 m, "
-                               "dt 0.8 ms, nt 6000, dPML 40, FP32 snapshots every step",
+        "config": {"workload": (f"mcs2001 line: 2001x601 @10 m, 64 shots @25 m, 480 ch @12.5 m, near offset 150 m, "
+                                f"dt 0.8 ms, nt 6000, dPML 40, FP32 snapshots every step; timed: {args.steps} shots "
+                                f"(shots {args.warmup % len(shots)}..{(args.warmup + args.steps - 1) % len(shots)} of "
+                                f"the line, after {args.warmup} untimed warmup shot(s) in the plain form)"),
                    "form": form, "window": f"{ny}x{nxw}"},
         "gpt_updates_per_s": head["gpt_updates_per_s"],
         "roofline": roof,

@@ -10,10 +10,15 @@ CPU restatement of psdm2twt (auxiliar_PSDM.f90:23-91) without its file writers:
   at both ends; inside, the first interval [x1(i), x1(i+1)] that contains the abscissa and
   whose two ordinates are both non-zero, y = y(i) + (a - x(i)) * ((y(i+1)-y(i))/(x(i+1)-x(i)))
   in FP32; 0 where no interval qualifies;
-* ``psdm2twt`` -- velocity and image of every column on the time axis t(j) = (j-1)*dt.
+* ``psdm2twt`` -- velocity and image of every column on the time axis t(j) = (j-1)*dt;
+* ``normalize_f32`` -- normalize_psdm (auxiliar_PSDM.f90:1-20) as the reference computes it:
+  FP32 PSDM_sum / FP32 Cov_sum where Cov_sum > 0, else 0;
+* ``normalize`` -- the same division on the FP64 stack (FP64 PSDM_sum, int32 fold) that the
+  device keeps, rounded once to FP32.
 
 Pinned by tests/test_poststack_oracle.py against the reference's own
-linear_interpolation_1D compiled into oracle/_ref and against closed forms.
+linear_interpolation_1D and normalize_psdm compiled into oracle/_ref and against closed
+forms.
 """
 from __future__ import annotations
 
@@ -88,3 +93,42 @@ def ref_linear_interpolation_1d(x1, y1, x2) -> np.ndarray:
     L.ref_linear_interpolation_1d(x1.ctypes.data_as(C.c_void_p), y1.ctypes.data_as(C.c_void_p), x1.size,
                                   x2.ctypes.data_as(C.c_void_p), y2.ctypes.data_as(C.c_void_p), x2.size)
     return y2
+
+
+def normalize_f32(psdm_sum, cov_sum) -> np.ndarray:
+    """normalize_psdm with FP32 PSDM_sum and FP32 Cov_sum (auxiliar_PSDM.f90:9-18)."""
+    s = np.asarray(psdm_sum, F32)
+    c = np.asarray(cov_sum, F32)
+    out = np.zeros(s.shape, F32)
+    pos = c > 0
+    out[pos] = s[pos] / c[pos]
+    return out
+
+
+def normalize(psdm_sum, fold) -> np.ndarray:
+    """PSDM = float32(PSDM_sum / fold) where fold > 0, else 0, from the FP64 sum and the
+    int32 fold.  For an FP32-representable sum this equals normalize_f32: FP64 division
+    followed by one rounding to FP32 is the correctly rounded FP32 quotient (53 >= 2*24+2
+    significand bits, so the double rounding is innocuous)."""
+    s = np.asarray(psdm_sum, np.float64)
+    f = np.asarray(fold, np.int32)
+    out = np.zeros(s.shape, F32)
+    pos = f > 0
+    out[pos] = (s[pos] / f[pos].astype(np.float64)).astype(F32)
+    return out
+
+
+def ref_normalize_psdm(psdm_sum, cov_sum) -> np.ndarray:
+    """The reference's normalize_psdm (oracle/_ref) on (ny, nx) FP32 arrays."""
+    L = _po.ref_lib()
+    if not getattr(L, "_norm_bound", False):
+        L.ref_normalize_psdm.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.ref_normalize_psdm.restype = None
+        L._norm_bound = True
+    s = np.asfortranarray(psdm_sum, F32)
+    c = np.asfortranarray(cov_sum, F32)
+    out = np.zeros(s.shape, F32, order="F")
+    ny, nx = s.shape
+    L.ref_normalize_psdm(nx, ny, s.ctypes.data_as(C.c_void_p), c.ctypes.data_as(C.c_void_p),
+                         out.ctypes.data_as(C.c_void_p))
+    return out

@@ -137,6 +137,19 @@ def ref_deriv(which: int, p: np.ndarray) -> np.ndarray:
     return d
 
 
+def ref_ricker_mcs(nt: int, dt: float, f0: float, t0: float) -> np.ndarray:
+    """The reference's Ricker_mcs (get_mcs_data.f90:318-351) with nt_prop = nt,
+    dt_prop = dt, freq_ricker = f0, delay_ricker = t0."""
+    L = ref_lib()
+    if not getattr(L, "_ricker_bound", False):
+        L.ref_ricker_mcs.argtypes = [_I, _F, _F, _F, _P]
+        L.ref_ricker_mcs.restype = None
+        L._ricker_bound = True
+    s = np.zeros(nt, np.float32)
+    L.ref_ricker_mcs(nt, dt, f0, t0, _ptr(s))
+    return s
+
+
 def fd_coef(num: int, precision: str = "f64"):
     dt_ = np.float64 if precision == "f64" else np.float32
     c = np.zeros(4, dt_)

@@ -312,4 +312,27 @@ contains
     call solve_eikonal_FSM(a, b, ddx, ddy, v, px, pz, T)
   end subroutine
 
+  ! normalize_psdm (auxiliar_PSDM.f90:1-20): PSDM = PSDM_sum / Cov_sum where Cov_sum > 0,
+  ! else 0, in the reference's FP32.
+  subroutine ref_normalize_psdm(nx, ny, psdm_sum, cov_sum, psdm) bind(C, name="ref_normalize_psdm")
+    integer(c_int), value :: nx, ny
+    real(c_float) :: psdm_sum(ny, nx), cov_sum(ny, nx), psdm(ny, nx)
+    integer :: a, b
+    real :: dx, dy
+    a = nx; b = ny; dx = 1.0; dy = 1.0
+    call normalize_psdm(a, b, dx, dy, psdm_sum, cov_sum, psdm)
+  end subroutine
+
+  ! Ricker_mcs (get_mcs_data.f90:318-351): the wavelet reads nt_prop, dt_prop,
+  ! freq_ricker and delay_ricker from mod_parfile.
+  subroutine ref_ricker_mcs(nt, dt, f0, t0, s0) bind(C, name="ref_ricker_mcs")
+    use mod_parfile, only: nt_prop, dt_prop, freq_ricker, delay_ricker
+    integer(c_int), value :: nt
+    real(c_float), value :: dt, f0, t0
+    real(c_float) :: s0(nt)
+    integer :: rank
+    nt_prop = nt; dt_prop = dt; freq_ricker = f0; delay_ricker = t0; rank = 0
+    call Ricker_mcs(rank, s0)
+  end subroutine
+
 end module ref_harness

@@ -155,16 +155,25 @@ def dynamic_window(nxSou, nxRec, dd: float, nx: int, dd_space: float):
 
 def ricker_mcs(nt_prop: int, dt_prop: float, freq_ricker: float, delay_ricker: float = 0.0) -> np.ndarray:
     """Ricker_mcs (get_mcs_data.f90:318-351) in FP32: S0 = (1-2a)exp(-a),
-    a = (pi f0 (t-t0))^2, t = (i-1) dt, zero for t >= 10/f0."""
+    a = (pi f0 (t-t0))^2, t = (i-1) dt, zero for t >= 10/f0.  exp is evaluated in FP64
+    and rounded once to FP32 (the correctly rounded FP32 exponential, as the Fortran
+    runtime's expf returns it)."""
     f32 = np.float32
     pi, f0, t0 = f32(3.14159265), f32(freq_ricker), f32(delay_ricker)
     tmp = f32(10.0) / f0
     t = np.arange(nt_prop, dtype=np.float32) * f32(dt_prop)
     x = (pi * f0 * (t - t0)).astype(np.float32)
     a = (x * x).astype(np.float32)
-    S0 = ((f32(1.0) - f32(2.0) * a) * np.exp(f32(-1.0) * a).astype(np.float32)).astype(np.float32)
+    e = np.exp(-a.astype(np.float64)).astype(np.float32)
+    S0 = ((f32(1.0) - f32(2.0) * a) * e).astype(np.float32)
     S0[t >= tmp] = 0.0
     return S0
+
+
+def ricker_was(nt_prop: int, dt_prop: float, freq_ricker: float) -> np.ndarray:
+    """Ricker_was (get_was_data.f90:466-495): the Ricker_mcs expression with the delay
+    fixed at t0 = 1.5/f0 (FP32 quotient)."""
+    return ricker_mcs(nt_prop, dt_prop, freq_ricker, np.float32(1.5) / np.float32(freq_ricker))
 
 
 # ----------------------------------------------------------------------------------

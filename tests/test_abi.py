@@ -103,6 +103,38 @@ def test_dynamic_window_matches_reference_rule():
     assert (xi2, xf2, nx_w2) == (1, 15, 15)
 
 
+@pytest.mark.skipif(not __import__("oracle.pyoracle", fromlist=["x"]).ref_available(), reason="oracle/_ref not built")
+@pytest.mark.parametrize("f0,dt,t0,nt", [(15.0, 0.001, 0.0, 3000), (15.0, 0.001, 0.08, 900), (8.0, 0.001, 0.1875, 4000),
+                                         (25.0, 0.0008, 0.0, 6000), (10.0, 0.0005, 0.12, 2500),
+                                         (8.0, 0.001, 0.1875, 12000), (15.0, 0.0008, 0.08, 6000)])
+def test_ricker_mcs_vs_reference(f0, dt, t0, nt):
+    """ricker_mcs against the reference's own Ricker_mcs (oracle/_ref) with the module
+    inputs freq_ricker, delay_ricker, nt_prop, dt_prop set: bit for bit, zero tail from
+    t >= 10/f0 included."""
+    from oracle.pyoracle import ref_ricker_mcs
+    from psdm_amd.solvers import ricker_mcs
+
+    got, ref = ricker_mcs(nt, dt, f0, t0), ref_ricker_mcs(nt, dt, f0, t0)
+    np.testing.assert_array_equal(got, ref)
+    # a wrong factor in (1 - 2a) exp(-a), e.g. (1 - a) exp(-a), moves the side lobes far off
+    t = np.arange(nt, dtype=np.float32) * np.float32(dt)
+    a = (np.pi * f0 * (t - t0)) ** 2
+    bad = ((1 - a) * np.exp(-a)).astype(np.float32)
+    bad[t >= 10.0 / f0] = 0
+    assert np.max(np.abs(bad - ref)) > 0.1
+
+
+@pytest.mark.skipif(not __import__("oracle.pyoracle", fromlist=["x"]).ref_available(), reason="oracle/_ref not built")
+@pytest.mark.parametrize("f0,dt,nt", [(8.0, 0.001, 12000), (7.0, 0.0008, 3000)])
+def test_ricker_was_vs_reference_expression(f0, dt, nt):
+    """Ricker_was is Ricker_mcs's expression with t0 = 1.5/f0 (get_was_data.f90:484): the
+    reference's Ricker_mcs run with delay_ricker = FP32(1.5/f0) gives it bit for bit."""
+    from oracle.pyoracle import ref_ricker_mcs
+    from psdm_amd.solvers import ricker_was
+
+    np.testing.assert_array_equal(ricker_was(nt, dt, f0), ref_ricker_mcs(nt, dt, f0, np.float32(1.5) / np.float32(f0)))
+
+
 def test_ricker_mcs():
     from psdm_amd.solvers import ricker_mcs
 

@@ -85,3 +85,38 @@ def test_psdm2twt_length_query_and_errors(ctx):
     with pytest.raises(RtmError, match="nt_cap"):
         ctx.check(ctx._L.psdm_rtm_psdm2twt(*args, 4, C.c_void_p(out.data_ptr()), C.c_void_p(out.data_ptr()),
                                            C.byref(nt)), "psdm2twt")
+
+
+@pytest.mark.parametrize("ny,nx,seed", [(601, 37, 0), (1, 1, 1), (255, 2001, 2)])
+def test_normalize_bit_exact_vs_oracle(ctx, ny, nx, seed):
+    """psdm_rtm_normalize (normalize_psdm, auxiliar_PSDM.f90:1-20) against the oracle's FP64
+    normalize, bit for bit: folds 0 (image 0), 1 (the sum itself, rounded to FP32) and > 1,
+    sums of every magnitude.  On FP32-representable sums the oracle equals the reference's
+    own FP32 division (tests/test_poststack_oracle.py), so the device does too."""
+    import ctypes as C
+
+    import torch
+
+    rng = np.random.default_rng(seed)
+    s = rng.standard_normal((ny, nx)) * 10.0 ** rng.integers(-20, 20, (ny, nx))
+    m32 = rng.random((ny, nx)) < 0.3
+    s[m32] = s[m32].astype(np.float32)  # FP32-representable sums
+    fold = rng.integers(0, 300, (ny, nx)).astype(np.int32)
+    fold[rng.random((ny, nx)) < 0.25] = 0
+    fold[rng.random((ny, nx)) < 0.25] = 1
+    if ny * nx == 1:
+        s[:], fold[:] = 1.0 / 3.0, 0
+    say(f"[gpu] normalize {ny}x{nx} seed {seed}")
+    ds = torch.from_numpy(np.ascontiguousarray(s.T)).to("cuda:0")
+    df = torch.from_numpy(np.ascontiguousarray(fold.T)).to("cuda:0")
+    out = torch.full((nx, ny), float("nan"), device="cuda:0")
+    ctx.check(ctx._L.psdm_rtm_normalize(ctx.h, C.c_void_p(ds.data_ptr()), C.c_void_p(df.data_ptr()), ny, nx,
+                                        C.c_void_p(out.data_ptr())), "psdm_rtm_normalize")
+    torch.cuda.synchronize()
+    got = out.cpu().numpy().T
+    want = P.normalize(s, fold)
+    np.testing.assert_array_equal(got, want)
+    assert np.all(got[fold == 0] == 0)
+    s32 = s.astype(np.float32)
+    exact = s32.astype(np.float64) == s
+    np.testing.assert_array_equal(got[exact], P.normalize_f32(s32, fold.astype(np.float32))[exact])

@@ -6,6 +6,7 @@ grids, exact invariants, and the golden vectors written by the reference Fortran
 (tests/golden/make_golden.py).
 """
 import os
+import sys
 from fractions import Fraction
 
 import numpy as np
@@ -279,24 +280,33 @@ def test_aligned_decimated_imaging_equals_masked_reference_imaging():
 
 
 def test_fp64_accuracy_fixture_is_the_oracle():
-    """The FP64 image / traces fixture the accuracy figures are stated against is what the
-    FP64 oracle computes for its workload (regenerated here and compared)."""
-    from psdm_amd.solvers import ricker_mcs
-    from psdm_amd.workloads import line_workload
+    """The FP64 image / traces fixture the short accuracy figures are stated against is what
+    the FP64 oracle computes for its workload (regenerated here with the fixture's own
+    generator and compared)."""
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden"))
+    import make_accuracy_fixture as MF
 
     fx = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "accuracy_single401_fp64.npz"))
-    W = line_workload("single401")
-    m, sh = W["model"], W["shots"][0]
-    nt, dt, dPML = int(fx["nt"]), W["dt"], W["dPML"]
-    ctmp = np.asfortranarray(m.migration[:, sh.x_i - 1:sh.x_f])
-    ctrue = np.asfortranarray(m.true[:, sh.x_i - 1:sh.x_f])
-    src = ricker_mcs(nt, dt, W["f0"], 1.2 / W["f0"]).reshape(nt, 1)
-    geo = ([sh.nxSou - sh.x_i + 1], [sh.nySou], sh.nxRec - sh.x_i + 1, sh.nyRec)
-    d, _ = O.forward(*geo, src, m.dx, dt, ctrue, 1, nt, nt, dPML, 0)
+    d = MF.observed("single401")
     assert float(np.sum(d.astype(np.float64) ** 2)) == float(fx["data_sumsq"])
-    R64, F64 = O.forward(*geo, src, m.dx, dt, ctmp, 1, nt, nt, dPML, 0, precision="f64")
-    k64 = O.backward(*geo, np.asfortranarray(d[::-1, :]), m.dx, dt, ctmp, 1, nt, nt, dPML, 0, Fields=F64,
-                     precision="f64")
+    k64, R64 = MF.fp64("single401", d)
     np.testing.assert_array_equal(k64.astype(np.float32), fx["image_fp64"])
-    np.testing.assert_array_equal(R64.astype(np.float32), fx["traces_fp64"])
+    np.testing.assert_array_equal(R64[:, fx["trace_channels"]].astype(np.float32), fx["traces_fp64"])
     assert np.abs(fx["image_fp64"]).max() > 0
+
+
+def test_fp64_full_record_fixture_matches_its_workload():
+    """The full-record mcs2001 fixture (minutes to regenerate, so not redone here) belongs
+    to shot 0 of the line at nt 6000, holds a non-zero image of the window's shape and the
+    traces of every 10th channel; its gather sample is the FP32 oracle's first samples'
+    shape (the GPU bench confirms the whole gather by its sum of squares)."""
+    from psdm_amd.workloads import line_workload
+
+    fx = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "accuracy_mcs2001_fp64.npz"))
+    W = line_workload("mcs2001")
+    sh = W["shots"][0]
+    assert (int(fx["nt"]), int(fx["x_i"]), int(fx["nx_w"])) == (W["nt"], sh.x_i, sh.nx_w)
+    assert fx["image_fp64"].shape == (W["model"].nz, sh.nx_w) and np.abs(fx["image_fp64"]).max() > 0
+    np.testing.assert_array_equal(fx["trace_channels"], np.arange(0, sh.nxRec.size, 10))
+    assert fx["traces_fp64"].shape == (W["nt"], fx["trace_channels"].size)
+    assert bool(fx["direct_wave_removed"]) and float(fx["data_sumsq"]) > 0
