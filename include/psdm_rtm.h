@@ -246,12 +246,22 @@ int psdm_rtm_condition_timing(psdm_rtm_ctx *ctx, double *filter_ms, double *resa
  * nt = 1 + ceiling(max|t| / dt) and d_vel_twt, d_psdm_twt (nt, nx) device: the velocity
  * and the image at t = (j-1)*dt by linear_interpolation_1D (linear_interpolation.f90:
  * 72-103), bit for bit.  With d_vel_twt or d_psdm_twt NULL the call only returns nt;
- * otherwise nt_cap (the outputs' rows) must be >= nt.  A full call that directly follows
- * a length query with the same arguments reuses the query's times (d_vel must not change
- * in between).  Synchronous.
+ * otherwise nt_cap (the outputs' rows) must be >= nt.  Each call computes the times anew.
+ * Synchronous.
  * ------------------------------------------------------------------------------ */
 int psdm_rtm_psdm2twt(psdm_rtm_ctx *ctx, int nx, int nz, float dd, float dt, const float *d_vel,
                       const float *d_psdm, int nt_cap, float *d_vel_twt, float *d_psdm_twt, int *nt);
+/* psdm_rtm_psdm2twt that allocates nothing (no hipMalloc / hipFree, no allocating library
+ * call): for callers that cannot allocate device memory (code that will be captured into a
+ * HIP graph, tight serving loops, harnesses that forbid allocation).  No counterpart in the
+ * reference.  Same arguments, results (bit for bit) and errors as psdm_rtm_psdm2twt; its
+ * workspace -- the two-way times (nz*nx floats) and the per-column flags (nx+1 words) --
+ * is a static 32 MiB device array per GPU, kept until the process ends: nz*nx + nx up to
+ * about 8.3 M cells, larger sizes fail with PSDM_RTM_EINVAL.  Every call on one GPU shares
+ * that array: calls that overlap on one GPU (several threads or contexts) race.  Not
+ * timed by psdm_rtm_set_timing.  Synchronous. */
+int psdm_rtm_psdm2twt_noalloc(psdm_rtm_ctx *ctx, int nx, int nz, float dd, float dt, const float *d_vel,
+                              const float *d_psdm, int nt_cap, float *d_vel_twt, float *d_psdm_twt, int *nt);
 /* With psdm_rtm_set_timing(ctx, n != 0): mean device time per full conversion of the
  * time-accumulation kernel and of the interpolation kernel, and the conversions timed. */
 int psdm_rtm_psdm2twt_timing(psdm_rtm_ctx *ctx, double *times_ms, double *interp_ms, long long *n_calls);

@@ -45,6 +45,7 @@ SIGNATURES = {
     "psdm_rtm_condition": ([_P, _P, _I, _P, _P], _I),
     "psdm_rtm_condition_host": ([_P, _P, _I, _P, _P], _I),
     "psdm_rtm_psdm2twt": ([_P, _I, _I, _F, _F, _P, _P, _I, _P, _P, C.POINTER(_I)], _I),
+    "psdm_rtm_psdm2twt_noalloc": ([_P, _I, _I, _F, _F, _P, _P, _I, _P, _P, C.POINTER(_I)], _I),
     "psdm_rtm_psdm2twt_timing": ([_P, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(_LL)], _I),
     "psdm_rtm_condition_timing": ([_P, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(_LL)], _I),
     "psdm_rtm_engine_stats": ([_P, C.POINTER(_LL), C.POINTER(_LL), C.POINTER(_LL), C.POINTER(_LL)], _I),

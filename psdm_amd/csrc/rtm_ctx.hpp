@@ -52,9 +52,6 @@ struct psdm_rtm_ctx {
     hipEvent_t twt_ev[4] = {};
     double twt_ms[2] = {0.0, 0.0};
     long long twt_n = 0;
-    std::vector<unsigned long long> twt_key;  // arguments of the last length query (times kept)
-    int twt_nt = 0;
-    float twt_times_ms = 0.f;
     // SU codec: device time of the decode [0] and encode [1] kernels per call (timing on)
     hipEvent_t su_ev[4] = {};
     double su_ms[2] = {0.0, 0.0};

@@ -19,6 +19,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "rtm_ctx.hpp"
@@ -117,6 +118,86 @@ __global__ __launch_bounds__(256) void twt_interp_kernel(const float *__restrict
     psdm_twt[(size_t)i * nt + j] = lin_interp(t + c, psdm + c, nz, a3, up);
 }
 
+__global__ void twt_clear_kernel(unsigned *p) {
+    if (threadIdx.x == 0) *p = 0u;
+}
+
+// Scratch of psdm_rtm_psdm2twt_noalloc, one copy per GPU held until the process ends: the
+// two-way times (nz*nx floats), then at the next 256-byte boundary the per-column
+// monotonicity flags (nx ints) and the max |t| bits (one word).  Host code reaches it only
+// through the per-device address from hipGetSymbolAddress (scratch_base).
+constexpr size_t kTwtScratchBytes = size_t(32) << 20;
+__device__ __align__(256) unsigned char scratch[kTwtScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of this GPU's scratch (the context's device is current): looked up once per
+// device and kept.
+int scratch_base(psdm_rtm_ctx *c, unsigned char **out) {
+    if (c->device < 0 || c->device >= kMaxDevices) return fail(c, PSDM_RTM_EINVAL, "psdm2twt_noalloc: device id");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[c->device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return psdm::host::hip_fail(c, e, "hipGetSymbolAddress(scratch)");
+        g_scratch_ptr[c->device] = p;
+    }
+    *out = (unsigned char *)g_scratch_ptr[c->device];
+    return 0;
+}
+
+size_t twt_scratch_need(int nx, int nz, size_t *mono_off) {
+    *mono_off = ((size_t)nz * nx * sizeof(float) + 255) & ~(size_t)255;
+    return *mono_off + ((size_t)nx + 1) * sizeof(int);
+}
+
+int check_twt_args(psdm_rtm_ctx *c, int nx, int nz, float dd, float dt, const float *d_vel, const float *d_psdm,
+                   const int *nt) {
+    if (!d_vel || !d_psdm || !nt) return fail(c, PSDM_RTM_EINVAL, "psdm2twt: null argument");
+    if (nx < 1 || nx > 65535 || nz < 1) return fail(c, PSDM_RTM_EINVAL, "psdm2twt: need 1 <= nx <= 65535, nz >= 1");
+    if (!(dd > 0.f) || !(dt > 0.f)) return fail(c, PSDM_RTM_EINVAL, "psdm2twt: dd and dt must be > 0");
+    return 0;
+}
+
+// The conversion with its workspace at d_t (nz*nx floats) and d_mono (nx ints, then the max
+// bits): the times, nt, and, unless d_vel_twt or d_psdm_twt is null, the interpolation.
+// `timed`: device times of both kernels into the context.  Synchronous.
+int twt_run(psdm_rtm_ctx *c, int nx, int nz, float dd, float dt, const float *d_vel, const float *d_psdm, int nt_cap,
+            float *d_vel_twt, float *d_psdm_twt, int *nt, float *d_t, int *d_mono, bool timed) {
+    unsigned *d_max = (unsigned *)(d_mono + nx);
+    hipLaunchKernelGGL(twt_clear_kernel, dim3(1), dim3(64), 0, c->stream, d_max);
+    hipExtLaunchKernelGGL(twt_times_kernel, dim3((nx + kTC - 1) / kTC), dim3(256), 0, c->stream,
+                          timed ? c->twt_ev[0] : nullptr, timed ? c->twt_ev[1] : nullptr, 0, d_vel, nz, nx, dd, d_t,
+                          d_mono, d_max);
+    HIP_TRY(c, hipGetLastError());
+    unsigned bits = 0;
+    HIP_TRY(c, hipMemcpyAsync(&bits, d_max, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    float tmax;
+    std::memcpy(&tmax, &bits, sizeof(float));
+    const double nd = 1.0 + std::ceil((double)(tmax / dt));  // nt = 1 + ceiling(time_max/dt), quotient in FP32
+    if (!(nd < 2147483647.0)) return fail(c, PSDM_RTM_EINVAL, "psdm2twt: time axis too long (is dt or v wrong?)");
+    *nt = (int)nd;
+    if (!d_vel_twt || !d_psdm_twt) return 0;  // length query
+    if (*nt > nt_cap) return fail(c, PSDM_RTM_EINVAL, "psdm2twt: outputs need nt rows (nt_cap too small)");
+    hipExtLaunchKernelGGL(twt_interp_kernel, dim3((*nt + 255) / 256, nx), dim3(256), 0, c->stream,
+                          timed ? c->twt_ev[2] : nullptr, timed ? c->twt_ev[3] : nullptr, 0, (const float *)d_t,
+                          (const int *)d_mono, d_vel, d_psdm, nz, *nt, dt, d_vel_twt, d_psdm_twt);
+    HIP_TRY(c, hipGetLastError());
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (timed) {
+        float a, b;
+        HIP_TRY(c, hipEventElapsedTime(&a, c->twt_ev[0], c->twt_ev[1]));
+        HIP_TRY(c, hipEventElapsedTime(&b, c->twt_ev[2], c->twt_ev[3]));
+        c->twt_ms[0] += a;
+        c->twt_ms[1] += b;
+        c->twt_n++;
+    }
+    return 0;
+}
+
 }  // namespace
 }  // namespace psdm
 
@@ -125,66 +206,32 @@ using namespace psdm;
 extern "C" int psdm_rtm_psdm2twt(psdm_rtm_ctx *c, int nx, int nz, float dd, float dt, const float *d_vel,
                                  const float *d_psdm, int nt_cap, float *d_vel_twt, float *d_psdm_twt, int *nt) {
     if (!c) return PSDM_RTM_EINVAL;
-    if (!d_vel || !d_psdm || !nt) return fail(c, PSDM_RTM_EINVAL, "psdm2twt: null argument");
-    if (nx < 1 || nx > 65535 || nz < 1) return fail(c, PSDM_RTM_EINVAL, "psdm2twt: need 1 <= nx <= 65535, nz >= 1");
-    if (!(dd > 0.f) || !(dt > 0.f)) return fail(c, PSDM_RTM_EINVAL, "psdm2twt: dd and dt must be > 0");
+    if (int rc = check_twt_args(c, nx, nz, dd, dt, d_vel, d_psdm, nt)) return rc;
     if (int rc = set_device(c)) return rc;
     const bool timed = c->timing != 0;
     if (timed)
         for (auto &e : c->twt_ev)
             if (!e) HIP_TRY(c, hipEventCreate(&e));
-    unsigned bd, bt;
-    std::memcpy(&bd, &dd, 4);
-    std::memcpy(&bt, &dt, 4);
-    const std::vector<unsigned long long> key = {(unsigned long long)(uintptr_t)d_vel, (unsigned long long)nx,
-                                                 (unsigned long long)nz, bd, bt};
-    const bool query = !d_vel_twt || !d_psdm_twt;
-    void *vt, *vm;
-    if (int rc = get_buf(c, "twt_t", (size_t)nz * nx * sizeof(float), &vt)) return rc;
-    if (int rc = get_buf(c, "twt_mono", ((size_t)nx + 1) * sizeof(int), &vm)) return rc;
-    int *d_mono = (int *)vm;
-    unsigned *d_max = (unsigned *)(d_mono + nx);
-    float times_ms = 0.f;
-    if (!query && c->twt_key == key) {  // the times of the preceding length query
-        *nt = c->twt_nt;
-        times_ms = c->twt_times_ms;
-    } else {
-        HIP_TRY(c, hipMemsetAsync(d_max, 0, sizeof(unsigned), c->stream));
-        hipExtLaunchKernelGGL(twt_times_kernel, dim3((nx + kTC - 1) / kTC), dim3(256), 0, c->stream,
-                              timed ? c->twt_ev[0] : nullptr, timed ? c->twt_ev[1] : nullptr, 0, d_vel, nz, nx, dd,
-                              (float *)vt, d_mono, d_max);
-        HIP_TRY(c, hipGetLastError());
-        unsigned bits = 0;
-        HIP_TRY(c, hipMemcpyAsync(&bits, d_max, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
-        HIP_TRY(c, hipStreamSynchronize(c->stream));
-        if (timed) HIP_TRY(c, hipEventElapsedTime(&times_ms, c->twt_ev[0], c->twt_ev[1]));
-        float tmax;
-        std::memcpy(&tmax, &bits, sizeof(float));
-        const double nd = 1.0 + std::ceil((double)(tmax / dt));  // nt = 1 + ceiling(time_max/dt), quotient in FP32
-        if (!(nd < 2147483647.0)) return fail(c, PSDM_RTM_EINVAL, "psdm2twt: time axis too long (is dt or v wrong?)");
-        *nt = (int)nd;
-    }
-    c->twt_key.clear();
-    if (query) {  // keep the times for the full call that follows
-        c->twt_key = key;
-        c->twt_nt = *nt;
-        c->twt_times_ms = times_ms;
-        return 0;
-    }
-    if (*nt > nt_cap) return fail(c, PSDM_RTM_EINVAL, "psdm2twt: outputs need nt rows (nt_cap too small)");
-    hipExtLaunchKernelGGL(twt_interp_kernel, dim3((*nt + 255) / 256, nx), dim3(256), 0, c->stream,
-                          timed ? c->twt_ev[2] : nullptr, timed ? c->twt_ev[3] : nullptr, 0, (const float *)vt,
-                          d_mono, d_vel, d_psdm, nz, *nt, dt, d_vel_twt, d_psdm_twt);
-    HIP_TRY(c, hipGetLastError());
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
-    if (timed) {
-        float b;
-        HIP_TRY(c, hipEventElapsedTime(&b, c->twt_ev[2], c->twt_ev[3]));
-        c->twt_ms[0] += times_ms;
-        c->twt_ms[1] += b;
-        c->twt_n++;
-    }
-    return 0;
+    size_t mono_off;
+    void *ws;
+    if (int rc = get_buf(c, "twt_ws", twt_scratch_need(nx, nz, &mono_off), &ws)) return rc;
+    return twt_run(c, nx, nz, dd, dt, d_vel, d_psdm, nt_cap, d_vel_twt, d_psdm_twt, nt, (float *)ws,
+                   (int *)((unsigned char *)ws + mono_off), timed);
+}
+
+extern "C" int psdm_rtm_psdm2twt_noalloc(psdm_rtm_ctx *c, int nx, int nz, float dd, float dt, const float *d_vel,
+                                         const float *d_psdm, int nt_cap, float *d_vel_twt, float *d_psdm_twt,
+                                         int *nt) {
+    if (!c) return PSDM_RTM_EINVAL;
+    if (int rc = check_twt_args(c, nx, nz, dd, dt, d_vel, d_psdm, nt)) return rc;
+    size_t mono_off;
+    if (twt_scratch_need(nx, nz, &mono_off) > kTwtScratchBytes)
+        return fail(c, PSDM_RTM_EINVAL, "psdm2twt_noalloc: nz*nx + nx exceeds the scratch (32 MiB: about 8.3 M cells)");
+    if (int rc = set_device(c)) return rc;
+    unsigned char *base;
+    if (int rc = scratch_base(c, &base)) return rc;
+    return twt_run(c, nx, nz, dd, dt, d_vel, d_psdm, nt_cap, d_vel_twt, d_psdm_twt, nt, (float *)base,
+                   (int *)(base + mono_off), false);
 }
 
 extern "C" int psdm_rtm_psdm2twt_timing(psdm_rtm_ctx *c, double *times_ms, double *interp_ms, long long *n) {
