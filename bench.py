@@ -917,6 +917,27 @@ def bench_served(args):
     ctx.close()
 
 
+def bench_copy(args):
+    """Calibration of the HBM traffic counters (rocprofv3 FETCH_SIZE / WRITE_SIZE): four
+    device-to-device copies of 256 MiB, whose bytes read and written are known."""
+    import torch
+
+    dev = torch.device("cuda:0")
+    a = torch.ones(64 * 1024 * 1024, device=dev)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(4):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    gbs = 4 * 2 * a.numel() * 4 / el / 1e9
+    print(json.dumps({"metric": "device copy GB/s (read + write)", "value": round(gbs, 1), "unit": "GB/s", "n_gpus": 1,
+                      "steps": 4, "warmup": 0, "ms_per_step": round(1e3 * el / 4, 3), "higher_is_better": True,
+                      "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+                      "config": {"workload": "copy: 4 x 256 MiB device to device (counter calibration)"}}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -932,15 +953,21 @@ def main():
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="batched form with the default launch shape")
     ap.add_argument("--engine-cfg", default=None,
-                    help="batched form: JSON launch shape (keys nslots, tile_x, stream_w, grid, groups, "
-                         "steps_per_launch) used instead of the autotuner's (e.g. to profile a tuned run's shape)")
+                    help="batched form: launch shape as key=value pairs, e.g. nslots=20,tile_x=16,stream_w=4,groups=2 "
+                         "(keys nslots, tile_x, stream_w, grid, groups, steps_per_launch), used instead of the "
+                         "autotuner's (to profile a tuned run's shape)")
+    ap.add_argument("--synthetic-gathers", action="store_true",
+                    help="mcs2001: seeded noise gathers instead of gathers modelled on the device (same cost per "
+                         "migrated shot; keeps profiled runs short)")
     ap.add_argument("--forms", default="plain,batched")
     ap.add_argument("--workload", default="mcs2001",
-                    choices=["mcs2001", "uneven", "mixed", "served", "condition", "poststack", "su", "kirchhoff"],
+                    choices=["mcs2001", "uneven", "mixed", "served", "condition", "poststack", "su", "kirchhoff",
+                             "copy"],
                     help="mcs2001: the 64-shot line (default line); uneven: 256 uneven shots, batched vs plain; "
                          "mixed: uneven + FP16/BF16 snapshots every 4th step; served: Poisson-arrival stream; "
                          "condition / poststack / su: the steps before and after the migration; "
-                         "kirchhoff: the Kirchhoff migration of the line's shots")
+                         "kirchhoff: the Kirchhoff migration of the line's shots; copy: a 256 MiB device copy, "
+                         "the calibration of the HBM traffic counters")
     ap.add_argument("--shots", type=int, default=256, help="uneven/mixed: shots of the 256-shot line used")
     ap.add_argument("--slots", type=int, default=16, help="uneven/mixed/served: engine slots")
     ap.add_argument("--hbm-gb", type=float, default=200.0, help="uneven/mixed/served: snapshot budget (GiB)")
@@ -966,6 +993,8 @@ def main():
         return bench_su(args)
     if args.workload == "kirchhoff":
         return bench_kirchhoff(args)
+    if args.workload == "copy":
+        return bench_copy(args)
     forms = set(args.forms.split(","))
 
     from psdm_amd.workloads import line_workload, step_bytes, updates_per_shot
@@ -1006,8 +1035,12 @@ def main():
                     sh.nxRec - sh.x_i + 1, sh.nyRec)
         geos.append(DeviceShot(*args_geo))                   # migration: snapshots every step
         geos_model.append(DeviceShot(*args_geo, nt_snap=0))  # data modelling: no snapshots
-    log(f"[bench] modelling {nshots} gathers")
-    obs = model_gathers(ctx, torch, dev, W, geos_model, d_true, d_wat, d_src, nshots)
+    if args.synthetic_gathers:
+        log(f"[bench] {nshots} seeded noise gathers")
+        obs = random_gathers(torch, dev, shots[:nshots], [nt] * nshots, seed=0)
+    else:
+        log(f"[bench] modelling {nshots} gathers")
+        obs = model_gathers(ctx, torch, dev, W, geos_model, d_true, d_wat, d_src, nshots)
 
     bfw = step_bytes(ny, nxw, dPML, 4.0, 1, backward=False)
     bbw = step_bytes(ny, nxw, dPML, 4.0, 1, backward=True)
@@ -1031,7 +1064,7 @@ def main():
         cfg = dict(nslots=16, tile_x=16, stream_w=4, grid=0, groups=2, chunk_steps=64)
         tuned = None
         if args.engine_cfg:
-            cfg.update(json.loads(args.engine_cfg))
+            cfg.update({k: int(v) for k, v in (kv.split("=") for kv in args.engine_cfg.split(","))})
         elif not args.no_tune:
             from psdm_amd.autotune import tune
 
@@ -1080,7 +1113,7 @@ def main():
         return int(mt.group(1)) if mt else None
 
     acc = None
-    if not args.no_accuracy:
+    if not args.no_accuracy and not args.synthetic_gathers:
         log("[bench] accuracy check vs the FP64 oracle fixtures")
         acc = {"short": accuracy_check(ctx)}
         if 0 in obs:
@@ -1113,7 +1146,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded 2001x601 model; gathers modelled on device, direct wave removed)",
+        "data": ("synthetic (seeded 2001x601 model; seeded noise gathers)" if args.synthetic_gathers else
+                 "synthetic (seeded 2001x601 model; gathers modelled on device, direct wave removed)"),
         "config": {"workload": (f"mcs2001 line: 2001x601 @10 m, 64 shots @25 m, 480 ch @12.5 m, near offset 150 m, "
                                 f"dt 0.8 ms, nt 6000, dPML 40, FP32 snapshots every step; timed: {args.steps} shots "
                                 f"(shots {args.warmup % len(shots)}..{(args.warmup + args.steps - 1) % len(shots)} of "
