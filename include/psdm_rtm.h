@@ -160,6 +160,10 @@ typedef struct {
     long long hbm_budget;    /* bytes for snapshot stores (0 = nslots x the largest shot) */
     double *d_psdm_sum;      /* optional caller-owned stack (ny, nx_prop) FP64, zeroed; NULL = engine-owned */
     int *d_cov_sum;          /* optional caller-owned fold (ny, nx_prop) int32, zeroed */
+    int steps_per_launch;    /* time steps per launch: 0 or 1, or 2 (temporal blocking: the first step is
+                                computed redundantly over a halo, so each launch reads and writes a shot's
+                                wavefield once per two steps; FP32 snapshots only, ny and windows >= 16);
+                                results are bit-identical to 1 */
 } psdm_rtm_engine_cfg;
 
 typedef struct {

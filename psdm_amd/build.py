@@ -14,7 +14,7 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libpsdm_rtm.so")
 
-SOURCES = ["rtm_kernels.hip", "rtm_api.hip", "rtm_engine.hip", "rtm_cond.hip", "rtm_post.hip", "rtm_su.hip", "rtm_kirch.hip"]
+SOURCES = ["rtm_kernels.hip", "rtm_tblock.hip", "rtm_api.hip", "rtm_engine.hip", "rtm_cond.hip", "rtm_post.hip", "rtm_su.hip", "rtm_kirch.hip"]
 
 # -ffp-contract=off: no fused multiply-add, so each product and sum rounds separately
 #   in the reference's order (the FP32 parity contract).

@@ -196,6 +196,86 @@ inline void set_tiling(StepArgs &a, const Tiling &T) {
     a.fd_cpml = make_fastdiv(T.ct_pml);
 }
 
+// Two time steps per launch (temporal blocking, engine only).  One launch advances a shot
+// from levels (k-1, k-2) to (k+1, k) with no dependency between its work items: every item
+// computes the first step redundantly over a halo wide enough for the second.
+//   core items   the interior rows [kTbTop, ny-3) and columns [3, nx-3): wavefront stream
+//                items of 4 rows per lane whose lanes 2..61 own kTbRows rows (lanes 0-1 and
+//                62-63 hold the halo), walking sw columns with the first step 3 columns
+//                ahead of the second; their halo never leaves the interior;
+//   edge tiles   everything else -- the CPML strips and a band of 3 (kTbTop on top) interior
+//                cells along them -- in tiles of at most kTbTY rows x kTbTX columns, each
+//                computing the first step on its tile +-3 (+-6 for the CPML memory) in LDS.
+// Regions of the edge (padded grid rows x columns): top [0, dn+kTbTop) x all, bottom
+// [dn+ny-3, nyt) x all, left [dn+kTbTop, dn+ny-3) x [0, dn+3), right (same rows) x
+// [dn+nx-3, nxt).  Each region splits evenly into ceil(h/kTbTY) x ceil(w/kTbTX) tiles.
+constexpr int kTbTop = 4;    // interior rows above the core (the core's first row is 16-byte aligned)
+constexpr int kTbBand = 3;   // interior columns left/right of the core and rows below it
+constexpr int kTbRows = 240; // core rows owned per wavefront
+constexpr int kTbTY = 52, kTbTX = 16;  // edge tile: at most 52 rows x 16 columns owned
+constexpr int kTbH = 9;      // edge tile LDS halo: the first step's memory needs +-6, its stencil +-3
+
+struct TbTiling {
+    int er_top, er_mid, er_bot;  // edge row tiles of the top, middle (left/right) and bottom regions
+    int ec_all, ec_side;         // edge column tiles across the full width, and of a side region
+    int ne;                      // edge tiles: (er_top + er_bot) * ec_all + 2 * er_mid * ec_side
+    int cr, cs, sw, ncore, ncblk;  // core row tiles, column segments, width, items, workgroups
+    int nitems;                  // workgroups of one launch: ne + ncblk
+};
+
+// First interior row read by lane 0 + 8 of core row tile R: kTbTop + kTbRows * R, the last
+// tile moved up (to a multiple of 4) so it ends at or below ny - 3.
+__host__ __device__ inline int tb_origin(int R, int ny) {
+    const int o = kTbTop + kTbRows * R, last = cdiv(ny - 3 - kTbRows, 4) * 4;
+    return o < last || last < kTbTop ? o : last;
+}
+
+__host__ __device__ inline TbTiling make_tb_tiling(int ny, int nx, int dn, int sw) {
+    TbTiling T;
+    T.er_top = cdiv(dn + kTbTop, kTbTY);
+    T.er_bot = cdiv(dn + 3, kTbTY);
+    T.er_mid = cdiv(ny - 3 - kTbTop, kTbTY);
+    T.ec_all = cdiv(nx + 2 * dn, kTbTX);
+    T.ec_side = cdiv(dn + 3, kTbTX);
+    T.ne = (T.er_top + T.er_bot) * T.ec_all + 2 * T.er_mid * T.ec_side;
+    T.cr = cdiv(ny - 3 - kTbTop, kTbRows);
+    T.sw = sw;
+    T.cs = cdiv(nx - 6, sw);
+    T.ncore = T.cr * T.cs;
+    T.ncblk = cdiv(T.ncore, 4);
+    T.nitems = T.ne + T.ncblk;
+    return T;
+}
+
+// [lo, hi) of the r-th of n even parts of [a, b).
+__host__ __device__ inline void tb_split(int r, int n, int a, int b, int *lo, int *hi) {
+    *lo = a + (int)((long long)r * (b - a) / n);
+    *hi = a + (int)((long long)(r + 1) * (b - a) / n);
+}
+
+// Owned rectangle [j0, j1) x [i0, i1) of edge tile e (padded grid).
+__host__ __device__ inline void tb_edge_tile(const TbTiling &T, int e, int ny, int nx, int dn, int *j0, int *j1,
+                                             int *i0, int *i1) {
+    const int nyt = ny + 2 * dn, nxt = nx + 2 * dn;
+    const int ntop = T.er_top * T.ec_all, nbot = T.er_bot * T.ec_all, nside = T.er_mid * T.ec_side;
+    if (e < ntop) {
+        tb_split(e / T.ec_all, T.er_top, 0, dn + kTbTop, j0, j1);
+        tb_split(e % T.ec_all, T.ec_all, 0, nxt, i0, i1);
+    } else if ((e -= ntop) < nbot) {
+        tb_split(e / T.ec_all, T.er_bot, dn + ny - 3, nyt, j0, j1);
+        tb_split(e % T.ec_all, T.ec_all, 0, nxt, i0, i1);
+    } else {
+        e -= nbot;
+        const int right = e >= nside;
+        if (right) e -= nside;
+        tb_split(e / T.ec_side, T.er_mid, dn + kTbTop, dn + ny - 3, j0, j1);
+        if (right)
+            tb_split(e % T.ec_side, T.ec_side, dn + nx - 3, nxt, i0, i1);
+        else
+            tb_split(e % T.ec_side, T.ec_side, 0, dn + 3, i0, i1);
+    }
+}
+
 // One slot of the batched engine: a shot resident on the device for its forward and
 // backward phases.  `a` holds the static fields; the per-step pointers and flags are
 // resolved on the device from the global step (rtm_batch_kernel).
@@ -218,6 +298,14 @@ struct SlotDesc {
     const int *ev_ptr_b, *ev_cell_b, *ev_inj_b;
     const float *inj_b;
     int ninj_b;
+    // two steps per launch: the second pair of wavefield levels (pA/pB hold levels
+    // (k-1, k-2) of even launches of a phase, pC/pD those of odd ones), second buffers of
+    // the chi memories and the x-strip history, the two-step tiling, and the injection
+    // events per two-step work item (the cells each item computes, halo included)
+    float *pC, *pD, *chxB, *chyB, *hsC, *hsD;
+    TbTiling tb;
+    const int *tev_ptr_f, *tev_cell_f, *tev_inj_f;
+    const int *tev_ptr_b, *tev_cell_b, *tev_inj_b;
 };
 
 // One launch group: slots[0 .. nslots) (their SlotDesc woff / witems give the group's
@@ -240,6 +328,9 @@ void launch_step(const StepArgs &a, int tile_x, hipStream_t s, hipEvent_t e0 = n
 void launch_batch(const BatchArgs &b, int tile_x, int sv, int grid, hipStream_t s, hipEvent_t e0 = nullptr,
                   hipEvent_t e1 = nullptr);
 void launch_advance(long long *gbase, long long by, hipStream_t s);
+// Two steps per launch (rtm_tblock.hip); sv: kSvF32 or kSvF32Kahan.
+void launch_tb_batch(const BatchArgs &b, int sv, int grid, hipStream_t s, hipEvent_t e0 = nullptr,
+                     hipEvent_t e1 = nullptr);
 void launch_record(const float *p, const int *rec_cell, float *rec, int rec_ld, int row, int nrec, hipStream_t s);
 void launch_c2(const float *vel, float *c2, int ny, int nx, int ldc, float dt, float dx, hipStream_t s);
 void launch_stack(float *img, float *comp, int ld_img, int ny, int nx_w, int x0, double *sum, int ld_sum, int *fold,

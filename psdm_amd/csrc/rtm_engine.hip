@@ -108,13 +108,14 @@ struct psdm_rtm_engine {
     psdm_rtm_ctx *c = nullptr;
     psdm_rtm_engine_cfg cfg{};
     int dn = 0, ld = 0, lds = 0, off = 0, sw = 16, nyt = 0, nxt_max = 0, cap_tiles = 0, nwork = 0, per_xcd = 0, grid = 0;
+    int spl = 1;  // time steps per launch (2: temporal blocking, rtm_tblock.hip)
     size_t win = 0;  // per-slot length of the c2 / image / comp buffers
     // slots split into `groups` independent launch chains inside a captured chunk, so one
     // group's next step can start while the other's drains
     int groups = 1;
     hipStream_t side[kMaxGroups - 1] = {};
     hipEvent_t ev_fork = nullptr, ev_join[kMaxGroups - 1] = {};
-    size_t n_p = 0, n_x = 0, n_y = 0, state_stride = 0, esize = 4;
+    size_t n_p = 0, n_x = 0, n_y = 0, state_base = 0, state_stride = 0, esize = 4;
     // device
     SlotDesc *d_slots = nullptr;
     long long *d_gbase = nullptr;
@@ -165,6 +166,32 @@ BatchArgs group_args(const psdm_rtm_engine *e, int q, int goff, int compact, int
     return BatchArgs{e->d_slots + s0, n, e->d_gbase, goff, compact, e->cap_tiles, make_fastdiv(e->cap_tiles)};
 }
 
+// One launch of a slot group: one time step, or two with temporal blocking.
+void launch_group(const psdm_rtm_engine *e, const BatchArgs &b, int grid, hipStream_t st) {
+    const int sv = snap_variant(e->cfg.snap_format, e->cfg.kahan);
+    if (e->spl == 2)
+        launch_tb_batch(b, sv, grid, st);
+    else
+        launch_batch(b, e->cfg.tile_x, sv, grid, st);
+}
+
+// Launches a slot occupies: one per time step of each phase, or, with two steps per
+// launch, nt/2 per phase (the phase's nt-1 steps two at a time).
+long long slot_launches(const psdm_rtm_engine *e, int nt) { return e->spl == 2 ? 2LL * (nt / 2) : 2LL * (nt - 1); }
+
+// Time steps that launches [l0, l1) of a slot (relative to its first) advance.
+long long slot_steps_in(const psdm_rtm_engine *e, int nt, long long l0, long long l1) {
+    if (l1 <= l0) return 0;
+    if (e->spl != 2) return l1 - l0;
+    long long n = 2 * (l1 - l0);
+    if ((nt - 1) & 1) {  // the last launch of each phase advances one step
+        const long long M = nt / 2;
+        if (l0 <= M - 1 && M - 1 < l1) n--;
+        if (l0 <= 2 * M - 1 && 2 * M - 1 < l1) n--;
+    }
+    return n;
+}
+
 int issue_chunk(psdm_rtm_engine *e) {
     psdm_rtm_ctx *c = e->c;
     const int C = e->cfg.chunk_steps;
@@ -175,7 +202,7 @@ int issue_chunk(psdm_rtm_engine *e) {
             for (int q = 0; q < e->groups; q++) {
                 int grid;
                 const BatchArgs b = group_args(e, q, g, e->compact, &grid);
-                launch_batch(b, e->cfg.tile_x, snap_variant(e->cfg.snap_format, e->cfg.kahan), grid, c->stream);
+                launch_group(e, b, grid, c->stream);
             }
         launch_advance(e->d_gbase, C, c->stream);
     }
@@ -194,8 +221,7 @@ int capture_graph(psdm_rtm_engine *e, int compact) {
         for (int q = 0; q < e->groups; q++) {
             int grid;
             const BatchArgs b = group_args(e, q, g, compact, &grid);
-            launch_batch(b, e->cfg.tile_x, snap_variant(e->cfg.snap_format, e->cfg.kahan), grid,
-                         q == 0 ? c->stream : e->side[q - 1]);
+            launch_group(e, b, grid, q == 0 ? c->stream : e->side[q - 1]);
         }
     for (int q = 1; q < e->groups; q++) {
         HIP_TRY(c, hipEventRecord(e->ev_join[q - 1], e->side[q - 1]));
@@ -249,10 +275,58 @@ int admit(psdm_rtm_engine *e, int s, Job &job, size_t arena_off) {
             ptr[u + 1] = (int)cell.size();
         }
     };
+    // two steps per launch: per work item (edge tiles, then core items) every injection cell
+    // the item computes, halo included; a cell several items compute has one value column
+    const TbTiling TB = make_tb_tiling(ny, nx, dn, e->sw);
+    auto build_tb = [&](const int *px, const int *py, int n, std::vector<int> &ptr, std::vector<int> &cell,
+                        std::vector<int> &inj, std::vector<int> &cols) {
+        std::map<int, int> last;  // (iw << 16 | jw) -> trace column; ordered by column, then row
+        for (int q = 0; q < n; q++) {
+            const int jw = py[q] - 1, iw = px[q] - 1;
+            if (jw < 0 || jw >= ny || iw < 0 || iw >= nx) continue;
+            last[(iw << 16) | jw] = q;
+        }
+        std::vector<int> keys, col_of;
+        for (auto &kv : last) {
+            keys.push_back(kv.first);
+            col_of.push_back((int)cols.size());
+            cols.push_back(kv.second);
+        }
+        // the cells with iw in [a, b) and jw in [r0, r1), in key order
+        auto add = [&](int a, int b, int r0, int r1) {
+            auto it = std::lower_bound(keys.begin(), keys.end(), a < 0 ? 0 : a << 16);
+            for (; it != keys.end() && (*it >> 16) < b; ++it) {
+                const int jw = *it & 0xffff;
+                if (jw >= r0 && jw < r1) {
+                    cell.push_back(*it);
+                    inj.push_back(col_of[it - keys.begin()]);
+                }
+            }
+        };
+        ptr.assign(TB.nitems + 1, 0);  // items ne .. ne+ncore-1 are the core items
+        for (int t = 0; t < TB.ne; t++) {
+            int j0, j1, i0, i1;
+            tb_edge_tile(TB, t, ny, nx, dn, &j0, &j1, &i0, &i1);
+            add(i0 - 3 - dn, i1 + 3 - dn, j0 - 3 - dn, j1 + 3 - dn);
+            ptr[t + 1] = (int)cell.size();
+        }
+        for (int u = 0; u < TB.ncore; u++) {
+            const int CS = u / TB.cr, R = u - CS * TB.cr, o = tb_origin(R, ny);
+            const int c0 = 3 + CS * TB.sw, cend = std::min(c0 + TB.sw, nx - 3);
+            add(c0 - 3, cend + 3, o - 8, o + 248);
+            ptr[TB.ne + u + 1] = (int)cell.size();
+        }
+        ptr.resize(TB.ne + TB.ncore + 1);
+    };
     const int sx = J.nxSou, sy = J.nySou;
     std::vector<int> fp, fc, fi, fcol, bp, bc, bi, bcol;
-    build(&sx, &sy, 1, fp, fc, fi, fcol);
-    build(job.nxRec.data(), job.nyRec.data(), J.NumRec, bp, bc, bi, bcol);
+    if (e->spl == 2) {
+        build_tb(&sx, &sy, 1, fp, fc, fi, fcol);
+        build_tb(job.nxRec.data(), job.nyRec.data(), J.NumRec, bp, bc, bi, bcol);
+    } else {
+        build(&sx, &sy, 1, fp, fc, fi, fcol);
+        build(job.nxRec.data(), job.nyRec.data(), J.NumRec, bp, bc, bi, bcol);
+    }
     std::vector<int> ints;
     auto put = [&](const std::vector<int> &v) {
         const size_t o = ints.size();
@@ -341,9 +415,27 @@ int admit(psdm_rtm_engine *e, int s, Job &job, size_t arena_off) {
     d.ev_inj_b = sl.d_ints + obi;
     d.inj_b = inj_b;
     d.ninj_b = ninj_b;
+    if (e->spl == 2) {
+        d.tb = TB;
+        a.nitems = TB.nitems;  // the slot's work items in the launch group's list
+        d.tev_ptr_f = d.ev_ptr_f;
+        d.tev_cell_f = d.ev_cell_f;
+        d.tev_inj_f = d.ev_inj_f;
+        d.tev_ptr_b = d.ev_ptr_b;
+        d.tev_cell_b = d.ev_cell_b;
+        d.tev_inj_b = d.ev_inj_b;
+        float *st2 = st + e->state_base;  // the second set follows the one-step state
+        d.pC = st2 + e->off;
+        d.pD = st2 + e->n_p + e->off;
+        float *x2 = st2 + 2 * e->n_p;
+        d.chxB = x2;
+        d.chyB = x2 + e->n_x;
+        d.hsC = d.chyB + e->n_y;
+        d.hsD = d.hsC + 2 * dn;
+    }
     sl.busy = true;
     sl.g0 = e->G;
-    sl.g_end = e->G + 2LL * (nt - 1);
+    sl.g_end = e->G + slot_launches(e, nt);
     sl.arena_off = arena_off;
     sl.job = std::move(job);
     e->slots_dirty = true;
@@ -480,6 +572,12 @@ int psdm_rtm_engine_create(psdm_rtm_ctx *c, const psdm_rtm_engine_cfg *cfg, psdm
     if (cfg->stream_w < 0 || cfg->stream_w > 256) return fail(c, PSDM_RTM_EINVAL, "stream_w must be in 0..256");
     if (cfg->groups < 0 || cfg->groups > kMaxGroups) return fail(c, PSDM_RTM_EINVAL, "groups must be in 0..4");
     if (cfg->chunk_steps < 1 || cfg->chunk_steps > 4096) return fail(c, PSDM_RTM_EINVAL, "chunk_steps out of range");
+    if (cfg->steps_per_launch < 0 || cfg->steps_per_launch > 2)
+        return fail(c, PSDM_RTM_EINVAL, "steps_per_launch must be 0, 1 or 2");
+    if (cfg->steps_per_launch == 2 && cfg->snap_format != kSnapF32)
+        return fail(c, PSDM_RTM_EINVAL, "steps_per_launch 2 needs FP32 snapshots");
+    if (cfg->steps_per_launch == 2 && cfg->ny < 16)
+        return fail(c, PSDM_RTM_EINVAL, "steps_per_launch 2 needs ny >= 16");
     if (!cfg->d_vel) return fail(c, PSDM_RTM_EINVAL, "null velocity model");
     if (!(cfg->dt > 0.f) || !(cfg->dmodel > 0.f)) return fail(c, PSDM_RTM_EINVAL, "dt and dmodel must be > 0");
     auto *e = new psdm_rtm_engine();
@@ -488,11 +586,16 @@ int psdm_rtm_engine_create(psdm_rtm_ctx *c, const psdm_rtm_engine_cfg *cfg, psdm
     e->dn = cfg->dPML + 6;
     e->nyt = cfg->ny + 2 * e->dn;
     e->ld = state_ld(cfg->ny, e->dn);
-    e->lds = psdm::interior_ld(cfg->ny);
+    e->spl = cfg->steps_per_launch == 2 ? 2 : 1;
+    // two steps per launch: core items read 4 rows past the one-step items' last (interior
+    // rows up to ny + 7), so the interior arrays hold 4 more rows
+    e->lds = e->spl == 2 ? cdiv((cfg->ny > kStreamRows ? cfg->ny : kStreamRows) + 12, 64) * 64
+                         : psdm::interior_ld(cfg->ny);
     e->off = state_off(e->dn);
-    e->sw = cfg->stream_w > 0 ? cfg->stream_w : 4;
+    e->sw = cfg->stream_w > 0 ? cfg->stream_w : (e->spl == 2 ? 8 : 4);
     e->nxt_max = cfg->max_nx + 2 * e->dn;
-    e->cap_tiles = make_tiling(cfg->ny, cfg->max_nx, e->dn, cfg->tile_x, e->sw).nitems;
+    e->cap_tiles = e->spl == 2 ? make_tb_tiling(cfg->ny, cfg->max_nx, e->dn, e->sw).nitems
+                               : make_tiling(cfg->ny, cfg->max_nx, e->dn, cfg->tile_x, e->sw).nitems;
     e->groups = cfg->groups > 0 ? cfg->groups : 2;
     if (!cfg->use_graph) e->groups = 1;
     e->groups = cdiv(cfg->nslots, cdiv(cfg->nslots, e->groups));  // every group holds at least one slot
@@ -505,7 +608,11 @@ int psdm_rtm_engine_create(psdm_rtm_ctx *c, const psdm_rtm_engine_cfg *cfg, psdm
     e->n_p = ((size_t)e->ld * e->nxt_max + 32 + 63) / 64 * 64;  // + room for the row offset
     e->n_x = (size_t)e->ld * dn2;
     e->n_y = dn2 * e->nxt_max;
-    e->state_stride = (2 * e->n_p + 3 * e->n_x + 3 * e->n_y + 2 * dn2 + 63) / 64 * 64;
+    // p (2 levels), phi_x (2), chi_x, phi_y (2), chi_y, the x-strip history (2); with two
+    // steps per launch a second pair of p levels, second chi buffers and history pair
+    e->state_base = (2 * e->n_p + 3 * e->n_x + 3 * e->n_y + 2 * dn2 + 63) / 64 * 64;
+    e->state_stride = e->state_base;
+    if (e->spl == 2) e->state_stride += (2 * e->n_p + e->n_x + e->n_y + 2 * dn2 + 63) / 64 * 64;
     const size_t S = cfg->nslots, win = e->win;
     size_t budget = cfg->hbm_budget > 0 ? (size_t)cfg->hbm_budget : S * snap_bytes_of(e, cfg->max_nx, cfg->max_nt);
     auto bail = [&](int rc) {
@@ -618,6 +725,7 @@ int psdm_rtm_engine_submit(psdm_rtm_engine *e, const psdm_rtm_job *job, long lon
         return efail(e, PSDM_RTM_EINVAL, "window outside the velocity model");
     if (e->cfg.variant == 0 && job->nx_w < e->dn)
         return efail(e, PSDM_RTM_EINVAL, "the _ram padding needs nx_w >= dPML+6");
+    if (e->spl == 2 && job->nx_w < 16) return efail(e, PSDM_RTM_EINVAL, "steps_per_launch 2 needs nx_w >= 16");
     if (job->NumRec < 0 || (job->NumRec > 0 && (!job->nxRec || !job->nyRec || !job->d_data)) || !job->d_wavelet)
         return efail(e, PSDM_RTM_EINVAL, "missing gather, receivers or wavelet");
     Job J;
@@ -673,7 +781,7 @@ int psdm_rtm_engine_run(psdm_rtm_engine *e, int max_chunks, int *n_active, int *
         HIP_TRY(c, hipEventRecord(e->ev_cstart[e->chunks % psdm_rtm_engine::kRing], c->stream));
         if (int rc = issue_chunk(e)) return rc;
         for (auto &sl : e->slots)
-            if (sl.busy) e->active_slot_steps += std::min<long long>(C, sl.g_end - e->G);
+            if (sl.busy) e->active_slot_steps += slot_steps_in(e, sl.job.j.nt, e->G - sl.g0, std::min(e->G + C, sl.g_end) - sl.g0);
         e->G += C;
         HIP_TRY(c, hipEventRecord(e->ev_chunk[e->chunks % psdm_rtm_engine::kRing], c->stream));
         e->chunks++;

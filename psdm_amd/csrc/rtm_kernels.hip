@@ -11,50 +11,9 @@
 #include <hip/hip_ext.h>
 
 #include "rtm_core.hpp"
+#include "rtm_device.hpp"
 
 namespace psdm {
-
-namespace {
-
-__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
-
-// c2 of padded cell (j,i): the reference fills the pads with the nearest edge
-// velocity of ctmp (solver_ac_forward_ram.f90:282-309), except that its _ram loops
-// fill the top and bottom pads only over the first dn interior columns
-// (`do j=1,dnx`, :288-302); the rest of those pads keeps c = 0.
-__device__ __forceinline__ float c2_at(const StepArgs &a, int j, int i) {
-    const int iw = i - a.dn;
-    const bool pad_row = j < a.dn || j >= a.dn + a.ny;
-    if (a.ram && pad_row && iw >= a.dn && iw < a.nx) return 0.f;
-    const int jw = clampi(j - a.dn, 0, a.ny - 1);
-    return ((const float __attribute__((address_space(1))) *)a.c2)[clampi(iw, 0, a.nx - 1) * a.ldc + jw];
-}
-
-__device__ __forceinline__ unsigned short f32_to_bf16(float f) {
-    unsigned u = __float_as_uint(f);
-    if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x7fffffu)) return 0x7fc0;  // NaN
-    u += 0x7fffu + ((u >> 16) & 1u);                                           // round to nearest even
-    return (unsigned short)(u >> 16);
-}
-__device__ __forceinline__ float bf16_to_f32(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
-
-__device__ __forceinline__ float wave_max(float v) {
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
-    return v;
-}
-
-}  // namespace
-
-// Global-memory views of the step arguments: pointers read from a StepArgs in memory
-// carry no address space of their own, and would otherwise be accessed with flat
-// instructions (which wait on both the LDS and the vector-memory counters).
-template <class T>
-using gptr = T __attribute__((address_space(1))) *;
-template <class T>
-__device__ __forceinline__ gptr<T> gp(T *p) {
-    return (gptr<T>)p;
-}
 
 // Stages p(k-1) rows j0-3..j0+66 and columns i0-3..i0+TX+2 of the padded grid into
 // LDS tile[(i-i0+3)*(TY+6) + (j-j0+3)] (zero outside the grid or when `zero`).
@@ -77,29 +36,6 @@ __device__ __forceinline__ void stage_p(gptr<const float> pc, int ld, int nyt, i
     }
 }
 
-// Native 4 x float / 2 x uint vectors (16- and 8-byte lane accesses).
-typedef float v4f __attribute__((ext_vector_type(4)));
-typedef unsigned v2u __attribute__((ext_vector_type(2)));
-
-// Lane L receives lane L-1's value (DPP wave_shr:1; lane 0 receives 0).
-__device__ __forceinline__ float from_prev_lane(float v) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xf, 0xf, false));
-}
-// Lane L receives lane L+1's value (DPP wave_shl:1; lane 63 receives 0).
-__device__ __forceinline__ float from_next_lane(float v) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xf, 0xf, false));
-}
-
-// D2 of one row of the 6th-order stencil, in the reference's order (solver_space_derivs_clean.f90:1-69):
-// ((((p+p)*c0 + (a1+b1)*c1) + (a2+b2)*c2) + (a3+b3)*c3)
-__device__ __forceinline__ float d2(float p, float a1, float b1, float a2, float b2, float a3, float b3) {
-    float d = (p + p) * kC2_0;
-    d = d + (a1 + b1) * kC2_1;
-    d = d + (a2 + b2) * kC2_2;
-    d = d + (a3 + b3) * kC2_3;
-    return d;
-}
-
 // Interior stream item u: interior rows o-4 .. o+251 of one stream row tile (4 per lane;
 // lanes 0 and 63 hold the 4-row halos) and columns [c0, cend), walked left to right with
 // p(k-1) columns i-3 .. i+3 in registers.  x neighbours come from the register window,
@@ -107,18 +43,6 @@ __device__ __forceinline__ float d2(float p, float a1, float b1, float a2, float
 // update (:443-444, source injection :348-358), free surface (:463-466), then the
 // snapshot (forward, :476-500) or the imaging term (backward,
 // solver_ac_backward_ram.f90:455-481).  Only the rows the item owns are stored.
-// Lane address = wave-uniform base + this lane's 32-bit byte offset (global loads and
-// stores then take the base in scalar registers and one offset register).
-// (__builtin_amdgcn_readfirstlane is int -> int: the 64-bit base is read as two halves)
-template <class T>
-__device__ __forceinline__ gptr<T> lanep(const void *base, unsigned off) {
-    const unsigned long long v = (unsigned long long)base;
-    const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)v);
-    const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(v >> 32));
-    return gp((T *)((const char *)(((unsigned long long)hi << 32) | lo) + off));
-}
-__device__ __forceinline__ v4f ldv(const float *base, unsigned off) { return *lanep<const v4f>(base, off); }
-
 template <int SV>
 __device__ __forceinline__ void stream_item(const StepArgs &a, int u, v4f *stage) {
     constexpr bool kF32 = SV == kSvF32 || SV == kSvF32Kahan;  // else 16-bit (FP16 or BF16)
