@@ -110,16 +110,25 @@ def updates_per_shot(ny: int, nx_w: int, nt: int, dPML: int) -> int:
     return 2 * (ny + 2 * dn) * (nx_w + 2 * dn) * (nt - 1)
 
 
-def step_bytes(ny: int, nx_w: int, dPML: int, snap_bytes: float = 4.0, snap_every: int = 1, backward=False):
+def step_bytes(ny: int, nx_w: int, dPML: int, snap_bytes: float = 4.0, snap_every: int = 1, backward=False,
+               steps_per_launch: int = 1):
     """Algorithmic HBM bytes of one time step: 16 B per padded point (read p, p_prev,
     c2, write p), 16 B per strip point and 32 B per corner point (CPML phi/chi read +
     write per direction), plus the snapshot write (forward) or snapshot read + image
-    read-modify-write (backward) over the interior every `snap_every` steps."""
+    read-modify-write (backward) over the interior every `snap_every` steps.
+
+    With two steps per launch the wavefield and the memories cross HBM once per launch:
+    per two steps 20 B per padded point (read p(k-1), p(k-2), c2; write p(k+1), p(k)),
+    16 B per strip point and 32 B per corner point, the image read-modify-write once per
+    launch that images; the snapshots stay per step."""
     dn = dPML + 6
     n_pad = (ny + 2 * dn) * (nx_w + 2 * dn)
     n_strip = 2 * dn * ny + 2 * dn * nx_w
     n_corner = 4 * dn * dn
     n_int = ny * nx_w
+    if steps_per_launch == 2:
+        img = 8.0 / max(2, snap_every) if backward else 0.0  # once per launch that images
+        return (20.0 * n_pad + 16.0 * n_strip + 32.0 * n_corner) / 2 + (snap_bytes / snap_every + img) * n_int
     per = (snap_bytes + (8.0 if backward else 0.0)) / snap_every
     return 16.0 * n_pad + 16.0 * n_strip + 32.0 * n_corner + per * n_int
 
