@@ -238,7 +238,8 @@ def run_batched(ctx, torch, dev, W, d_mig, d_src, obs, steps, warmup, cfg):
                  stream_w=cfg.get("stream_w", 0), groups=cfg.get("groups", 0),
                  snap_format=cfg.get("snap_format", "f32"),
                  store_snap=cfg.get("store_snap", 1), kahan=cfg.get("kahan", False),
-                 chunk_steps=cfg.get("chunk_steps", 64), d_psdm_sum=psdm_sum, d_cov_sum=fold)
+                 chunk_steps=cfg.get("chunk_steps", 64), d_psdm_sum=psdm_sum, d_cov_sum=fold,
+                 steps_per_launch=cfg.get("steps_per_launch", 1))
     for k in range(warmup, warmup + steps):  # the shots the plain form times
         eng.submit(**job_of(W, k % len(shots), d_src, obs))
     ctx.sync()
@@ -1070,17 +1071,22 @@ def main():
 
             jobs = [job_of(W, k, d_src, obs) for k in range(nshots)]
             best, trials = tune(ctx, d_mig, ny, m.nx, jobs, max(s.nx_w for s in shots), nt, dPML, W["fs"], m.dx, dt,
-                                nshots=args.steps, log=log)
+                                nshots=args.steps, steps_per_launch=(1, 2), log=log)
             cfg.update(nslots=best["nslots"], tile_x=best["tile_x"], stream_w=best["stream_w"], grid=best["grid"],
-                       groups=best["groups"])
+                       groups=best["groups"], steps_per_launch=best["steps_per_launch"])
             tuned = {"best": best, "trials": trials}
         B = run_batched(ctx, torch, dev, W, d_mig, d_src, obs, args.steps, args.warmup, cfg)
         st = B["stats"]
+        spl = cfg.get("steps_per_launch", 1)
+        # algorithmic bytes of one shot-step in the launch form that ran (two steps per launch
+        # move the wavefield once per two steps)
+        step_bb = 0.5 * (step_bytes(ny, nxw, dPML, 4.0, 1, backward=False, steps_per_launch=spl) +
+                         step_bytes(ny, nxw, dPML, 4.0, 1, backward=True, steps_per_launch=spl))
         # device time per batched step (events around the captured chunks); one step is `groups`
         # concurrent dispatches of the batch kernel, one per slot group
         per_launch_ms = B["launch_ms"]
         shot_steps_launch = st["slot_steps"] / max(st["steps_issued"], 1)
-        bytes_launch = step_b * shot_steps_launch
+        bytes_launch = step_bb * shot_steps_launch
         ach = bytes_launch / (per_launch_ms * 1e-3) / 1e9
         batched = {"shots_per_s": round(args.steps / B["elapsed"], 4),
                    "gpt_updates_per_s": round(args.steps * upd_shot / B["elapsed"] / 1e9, 3),
@@ -1092,9 +1098,10 @@ def main():
                                 "algorithmic_bytes_per_launch": round(bytes_launch),
                                 "shot_steps_per_launch": round(shot_steps_launch, 4),
                                 "wall_ms_per_launch": round(1e3 * B["elapsed"] / max(st["steps_issued"], 1), 5),
-                                "note": "one launch = one step of the resident shots, issued as concurrent_dispatches_"
-                                        "per_launch dispatches (slot groups); mean_kernel_ms: HIP events around each "
-                                        "captured chunk of launches / launches per chunk"}}
+                                "steps_per_launch": spl,
+                                "note": "one launch = steps_per_launch time steps of the resident shots, issued as "
+                                        "concurrent_dispatches_per_launch dispatches (slot groups); mean_kernel_ms: HIP "
+                                        "events around each captured chunk of launches / launches per chunk"}}
         if plain is not None:
             a = B["psdm_sum"].cpu().numpy()
             b = P["psdm_sum"].cpu().numpy()

@@ -7,8 +7,8 @@ the record length), lets one chunk run (admission and first-step effects), then 
 following chunks, which cover forward and backward steps.
 
 The search has two stages.  (1) The launch shape -- CPML tile width, interior
-stream-item width, concurrent slot groups, and steps advanced per launch -- is chosen at
-the first slot count.  (2) With that shape, the number of resident slots S is chosen by
+stream-item width, concurrent slot groups, and time steps per launch (1, or 2 with
+temporal blocking) -- is chosen at the first slot count.  (2) With that shape, the number of resident slots S is chosen by
 the time the engine would take for the workload's `nshots` shots of equal record length:
 floor(nshots / S) rounds with all S slots busy plus, when S does not divide nshots, one
 round with the remainder busy, each timed at its own occupancy (a step with few busy
@@ -68,7 +68,7 @@ def round_time(t_step, nshots, nslots, t_part=None):
 
 def tune(ctx, d_vel, ny, nx_prop, jobs, max_nx, nt, dPML, fs, dx, dt, snap_format="f32", store_snap=1, kahan=False,
          slots=None, tiles=(16, 32), stream_ws=(4, 8), grids=(0,), groups=(2,), steps_per_launch=(1,),
-         chunk_steps=32, chunks=5, nshots=None, log=None):
+         tb_stream_ws=(8, 16), chunk_steps=32, chunks=5, nshots=None, log=None):
     """jobs: list of Engine.submit keyword dicts.  Returns (best config dict, trials)."""
     slots = list(slots) if slots else slot_candidates(nshots)
     say = log or (lambda *a: None)
@@ -83,10 +83,18 @@ def tune(ctx, d_vel, ny, nx_prop, jobs, max_nx, nt, dPML, fs, dx, dt, snap_forma
             say(f"[autotune] slots={nslots} busy={busy} {cfg}: skipped ({exc})")
             return None
 
-    # (1) launch shape at the first slot count, every slot busy
+    # (1) launch shape at the first slot count, every slot busy.  Two steps per launch have
+    # fixed edge tiles (tile_x does not apply) and wider interior items (each computes its
+    # first step over 3 extra columns on either side)
     s0 = 16 if 16 in slots else slots[0]
     shapes = []
-    for tile_x, sw, grid, ng, spl in itertools.product(tiles, stream_ws, grids, groups, steps_per_launch):
+    cands = []
+    for spl in steps_per_launch:
+        if spl == 2:
+            cands += [(tiles[0], sw, grid, ng, 2) for sw, grid, ng in itertools.product(tb_stream_ws, grids, groups)]
+        else:
+            cands += [(t, sw, grid, ng, spl) for t, sw, grid, ng in itertools.product(tiles, stream_ws, grids, groups)]
+    for tile_x, sw, grid, ng, spl in cands:
         cfg = dict(tile_x=tile_x, stream_w=sw, grid=grid, groups=ng, steps_per_launch=spl)
         t = run(s0, s0, cfg)
         if t is None:

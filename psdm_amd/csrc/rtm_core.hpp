@@ -304,6 +304,10 @@ struct SlotDesc {
     // events per two-step work item (the cells each item computes, halo included)
     float *pC, *pD, *chxB, *chyB, *hsC, *hsD;
     TbTiling tb;
+    // the core kernel's compact work list (as woff ... wfd, over tb.ncblk workgroups per slot);
+    // woff ... wfd then hold the edge kernel's (tb.ne tiles per slot)
+    int cwoff, cwtotal, cwuni;
+    FastDiv cwfd;
     const int *tev_ptr_f, *tev_cell_f, *tev_inj_f;
     const int *tev_ptr_b, *tev_cell_b, *tev_inj_b;
 };
@@ -328,8 +332,10 @@ void launch_step(const StepArgs &a, int tile_x, hipStream_t s, hipEvent_t e0 = n
 void launch_batch(const BatchArgs &b, int tile_x, int sv, int grid, hipStream_t s, hipEvent_t e0 = nullptr,
                   hipEvent_t e1 = nullptr);
 void launch_advance(long long *gbase, long long by, hipStream_t s);
-// Two steps per launch (rtm_tblock.hip); sv: kSvF32 or kSvF32Kahan.
-void launch_tb_batch(const BatchArgs &b, int sv, int grid, hipStream_t s, hipEvent_t e0 = nullptr,
+// Two steps per launch (rtm_tblock.hip): part 0 the edge tiles (list woff..., cap_tiles =
+// edge tiles per slot at capacity), part 1 the core items (list cwoff..., cap_tiles = core
+// workgroups per slot); sv: kSvF32 or kSvF32Kahan.
+void launch_tb_batch(const BatchArgs &b, int part, int sv, int grid, hipStream_t s, hipEvent_t e0 = nullptr,
                      hipEvent_t e1 = nullptr);
 void launch_record(const float *p, const int *rec_cell, float *rec, int rec_ld, int row, int nrec, hipStream_t s);
 void launch_c2(const float *vel, float *c2, int ny, int nx, int ldc, float dt, float dx, hipStream_t s);

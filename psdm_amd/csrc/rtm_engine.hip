@@ -109,7 +109,8 @@ struct psdm_rtm_engine {
     psdm_rtm_ctx *c = nullptr;
     psdm_rtm_engine_cfg cfg{};
     int dn = 0, ld = 0, lds = 0, off = 0, sw = 16, nyt = 0, nxt_max = 0, cap_tiles = 0, nwork = 0, per_xcd = 0, grid = 0;
-    int spl = 1;  // time steps per launch (2: temporal blocking, rtm_tblock.hip)
+    int spl = 1;        // time steps per launch (2: temporal blocking, rtm_tblock.hip)
+    int cap_core = 0;   // two steps per launch: core workgroups of a slot at capacity (cap_tiles: edge tiles)
     size_t win = 0;  // per-slot length of the c2 / image / comp buffers
     // slots split into `groups` independent launch chains inside a captured chunk, so one
     // group's next step can start while the other's drains
@@ -159,21 +160,30 @@ int efail(psdm_rtm_engine *e, int code, const std::string &m) { return fail(e->c
 
 // Launches one chunk of time steps (graph replay or eager launches).
 // Launch arguments of slot group q (slots [q*n0, min((q+1)*n0, nslots)), n0 = ceil(nslots/groups)).
-BatchArgs group_args(const psdm_rtm_engine *e, int q, int goff, int compact, int *grid) {
+// part 1: the core kernel of two steps per launch (its own items per slot at capacity)
+BatchArgs group_args(const psdm_rtm_engine *e, int q, int goff, int compact, int *grid, int part = 0) {
     const int n0 = cdiv(e->cfg.nslots, e->groups);
     const int s0 = q * n0, n = std::min(n0, e->cfg.nslots - s0);
-    const int nwork = n * e->cap_tiles, per_xcd = cdiv(nwork, 8);
+    const int cap = part == 1 ? e->cap_core : e->cap_tiles;
+    const int nwork = n * cap, per_xcd = cdiv(nwork, 8);
     *grid = e->cfg.grid > 0 ? std::min(e->cfg.grid, per_xcd * 8) : per_xcd * 8;
-    return BatchArgs{e->d_slots + s0, n, e->d_gbase, goff, compact, e->cap_tiles, make_fastdiv(e->cap_tiles)};
+    return BatchArgs{e->d_slots + s0, n, e->d_gbase, goff, compact, cap, make_fastdiv(cap)};
 }
 
-// One launch of a slot group: one time step, or two with temporal blocking.
-void launch_group(const psdm_rtm_engine *e, const BatchArgs &b, int grid, hipStream_t st) {
+// One launch of slot group q at chunk offset g: one time step, or two with temporal
+// blocking (the edge-tile kernel, then the core kernel, on the same stream).
+void launch_group(const psdm_rtm_engine *e, int q, int g, int compact, hipStream_t st) {
     const int sv = snap_variant(e->cfg.snap_format, e->cfg.kahan);
-    if (e->spl == 2)
-        launch_tb_batch(b, sv, grid, st);
-    else
+    int grid;
+    const BatchArgs b = group_args(e, q, g, compact, &grid);
+    if (e->spl == 2) {
+        launch_tb_batch(b, 0, sv, grid, st);
+        int grid2;
+        const BatchArgs b2 = group_args(e, q, g, compact, &grid2, 1);
+        launch_tb_batch(b2, 1, sv, grid2, st);
+    } else {
         launch_batch(b, e->cfg.tile_x, sv, grid, st);
+    }
 }
 
 // Launches a slot occupies: one per time step of each phase, or, with two steps per
@@ -200,11 +210,7 @@ int issue_chunk(psdm_rtm_engine *e) {
         HIP_TRY(c, hipGraphLaunch(e->exec[e->compact], c->stream));
     } else {
         for (int g = 0; g < C; g++)
-            for (int q = 0; q < e->groups; q++) {
-                int grid;
-                const BatchArgs b = group_args(e, q, g, e->compact, &grid);
-                launch_group(e, b, grid, c->stream);
-            }
+            for (int q = 0; q < e->groups; q++) launch_group(e, q, g, e->compact, c->stream);
         launch_advance(e->d_gbase, C, c->stream);
     }
     HIP_TRY(c, hipGetLastError());
@@ -219,11 +225,7 @@ int capture_graph(psdm_rtm_engine *e, int compact) {
         for (int q = 1; q < e->groups; q++) HIP_TRY(c, hipStreamWaitEvent(e->side[q - 1], e->ev_fork, 0));
     }
     for (int g = 0; g < e->cfg.chunk_steps; g++)
-        for (int q = 0; q < e->groups; q++) {
-            int grid;
-            const BatchArgs b = group_args(e, q, g, compact, &grid);
-            launch_group(e, b, grid, q == 0 ? c->stream : e->side[q - 1]);
-        }
+        for (int q = 0; q < e->groups; q++) launch_group(e, q, g, compact, q == 0 ? c->stream : e->side[q - 1]);
     for (int q = 1; q < e->groups; q++) {
         HIP_TRY(c, hipEventRecord(e->ev_join[q - 1], e->side[q - 1]));
         HIP_TRY(c, hipStreamWaitEvent(c->stream, e->ev_join[q - 1], 0));
@@ -382,7 +384,7 @@ int admit(psdm_rtm_engine *e, int s, Job &job, size_t arena_off) {
     d.ninj_b = ninj_b;
     if (e->spl == 2) {
         d.tb = TB;
-        a.nitems = TB.nitems;  // the slot's work items in the launch group's list
+        a.nitems = TB.ne;  // the slot's edge tiles in the launch group's first list
         d.tev_ptr_f = d.ev_ptr_f;
         d.tev_cell_f = d.ev_cell_f;
         d.tev_inj_f = d.ev_inj_f;
@@ -457,6 +459,21 @@ int upload_slots(psdm_rtm_engine *e) {
         src[s0].wuni = uni ? src[s0].witems : 0;
         src[s0].wfd = make_fastdiv(std::max(1, run / n));
         if (src[s0].wuni != e->cap_tiles) e->compact = 1;
+        if (e->spl == 2) {  // the core kernel's list
+            int run2 = 0, first = -1;
+            bool uni2 = true;
+            for (int s = s0; s < s0 + n; ++s) {
+                const int w2 = src[s].g0 >= 0 ? src[s].tb.ncblk : 0;
+                src[s].cwoff = run2;
+                run2 += w2;
+                if (first < 0) first = w2;
+                uni2 = uni2 && w2 == first;
+            }
+            src[s0].cwtotal = run2;
+            src[s0].cwuni = uni2 ? first : 0;
+            src[s0].cwfd = make_fastdiv(std::max(1, run2 / n));
+            if (src[s0].cwuni != e->cap_core) e->compact = 1;
+        }
     }
     HIP_TRY(c, hipMemcpyAsync(e->d_slots, src, S * sizeof(SlotDesc), hipMemcpyHostToDevice, c->stream));
     HIP_TRY(c, hipEventRecord(e->h_ring_ev[e->h_ring], c->stream));
@@ -543,6 +560,8 @@ int psdm_rtm_engine_create(psdm_rtm_ctx *c, const psdm_rtm_engine_cfg *cfg, psdm
         return fail(c, PSDM_RTM_EINVAL, "steps_per_launch 2 needs FP32 snapshots");
     if (cfg->steps_per_launch == 2 && cfg->ny < 16)
         return fail(c, PSDM_RTM_EINVAL, "steps_per_launch 2 needs ny >= 16");
+    if (cfg->steps_per_launch == 2 && cfg->dPML + 6 > 256)
+        return fail(c, PSDM_RTM_EINVAL, "steps_per_launch 2 needs dPML <= 250");
     if (!cfg->d_vel) return fail(c, PSDM_RTM_EINVAL, "null velocity model");
     if (!(cfg->dt > 0.f) || !(cfg->dmodel > 0.f)) return fail(c, PSDM_RTM_EINVAL, "dt and dmodel must be > 0");
     auto *e = new psdm_rtm_engine();
@@ -559,8 +578,13 @@ int psdm_rtm_engine_create(psdm_rtm_ctx *c, const psdm_rtm_engine_cfg *cfg, psdm
     e->off = state_off(e->dn);
     e->sw = cfg->stream_w > 0 ? cfg->stream_w : (e->spl == 2 ? 8 : 4);
     e->nxt_max = cfg->max_nx + 2 * e->dn;
-    e->cap_tiles = e->spl == 2 ? make_tb_tiling(cfg->ny, cfg->max_nx, e->dn, e->sw).nitems
-                               : make_tiling(cfg->ny, cfg->max_nx, e->dn, cfg->tile_x, e->sw).nitems;
+    if (e->spl == 2) {
+        const TbTiling T = make_tb_tiling(cfg->ny, cfg->max_nx, e->dn, e->sw);
+        e->cap_tiles = T.ne;
+        e->cap_core = T.ncblk;
+    } else {
+        e->cap_tiles = make_tiling(cfg->ny, cfg->max_nx, e->dn, cfg->tile_x, e->sw).nitems;
+    }
     e->groups = cfg->groups > 0 ? cfg->groups : 2;
     if (!cfg->use_graph) e->groups = 1;
     e->groups = cdiv(cfg->nslots, cdiv(cfg->nslots, e->groups));  // every group holds at least one slot

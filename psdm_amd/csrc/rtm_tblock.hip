@@ -94,16 +94,20 @@ __device__ __forceinline__ bool resolve_tb(const SlotDesc &sd, long long G, TbSt
 
 constexpr int kLH = 72;                 // LDS rows: owned rows (<= 52) + 2 * kTbH
 constexpr int kLW = kTbTX + 2 * kTbH;   // LDS columns: 34
-constexpr int kCR = 64, kCC = kTbTX + 12;  // cells a thread block computes: rows +-6, columns +-6
-constexpr int kCPW = kCC / 4;           // columns per wavefront (7)
-constexpr int kEdgeLds = 3 * kLH * kLW + (kCR * kCC + 3) / 4 * 4;  // p, phi_x, phi_y + the event map (floats)
+constexpr int kCR = 64, kCC = kTbTX + 12;  // cells a tile computes: rows +-6 (one per lane), columns +-6
+constexpr int kEB = 512, kEW = kEB / 64;   // edge-tile workgroup: 8 wavefronts
+constexpr int kCPW = (kCC + kEW - 1) / kEW;  // columns per wavefront (4)
+constexpr int kTabMax = 256;            // CPML cells per side the edge kernel supports (dPML + 6)
+// LDS of an edge tile (floats): p, phi_x, phi_y of the current level; the injection values of
+// the two steps and the event flags per computed cell; the damping tables
+constexpr int kEdgeLds = 3 * kLH * kLW + 3 * kCR * kCC + 2 * kTabMax;
 
 // Level n+1 of padded cell (j, i) from level n: p, phi_x, phi_y of level n at grid positions
 // in LDS (column-major, kLH rows from row lj0, columns from li0); p1 (level n-1), c2 and
 // the cell's chi memories (in/out) from the caller.  The formulas are those of the
 // one-step kernels (cpml_tile and stream_item in rtm_kernels.hip), cell type by cell type.
 // wantP: the new p (and chi) are needed; otherwise only the new phi (a cell of the ring
-// that only the next step's memory derivative reads).
+// that only the next step's memory derivative reads).  tA, tB: the damping tables in LDS.
 struct EdgeView {
     const float *P, *X, *Y;
     int lj0, li0;
@@ -112,9 +116,9 @@ struct EdgeView {
     __device__ __forceinline__ float y(int j, int i) const { return Y[(i - li0) * kLH + (j - lj0)]; }
 };
 
-__device__ __forceinline__ void edge_cell(const StepArgs &a, const EdgeView &L, int j, int i, bool wantP, float p1,
-                                          float c2, bool has_src, float src, float &chx, float &chy, float &hsl,
-                                          float &hsp, float &np, float &nphx, float &nphy) {
+__device__ __forceinline__ void edge_cell(const StepArgs &a, const EdgeView &L, const float *tA, const float *tB, int j,
+                                          int i, bool wantP, float p1, float c2, bool has_src, float src, float &chx,
+                                          float &chy, float &hsl, float &hsp, float &np, float &nphx, float &nphy) {
     const int dn = a.dn;
     const int xs = i < dn ? 1 : (i >= dn + a.nx ? 2 : 0), ys = j < dn ? 1 : (j >= dn + a.ny ? 2 : 0);
     const float p = L.p(j, i);
@@ -138,7 +142,7 @@ __device__ __forceinline__ void edge_cell(const StepArgs &a, const EdgeView &L, 
     const int distx = xs == 0 ? 0 : (xs == 1 ? dn - i : i - (dn + a.nx) + 1);
     const int disty = ys == 0 ? 0 : (ys == 1 ? dn - j : j - (dn + a.ny) + 1);
     const int dist = distx > disty ? distx : disty;  // corners: F(max(dist_x, dist_y))
-    const float A = gp(a.fA)[dist - 1], B = gp(a.fB)[dist - 1];
+    const float A = tA[dist - 1], B = tB[dist - 1];
     const float px = L.p(j, ic), xp1 = L.p(j, ic + 1), xm1 = L.p(j, ic - 1), xp2 = L.p(j, ic + 2),
                 xm2 = L.p(j, ic - 2), xp3 = L.p(j, ic + 3), xm3 = L.p(j, ic - 3);
     const float py = L.p(jc, i), yp1 = L.p(jc + 1, i), ym1 = L.p(jc - 1, i), yp2 = L.p(jc + 2, i),
@@ -204,27 +208,6 @@ __device__ __forceinline__ void edge_cell(const StepArgs &a, const EdgeView &L, 
     np = v;
 }
 
-// Forward snapshot / backward imaging of one interior cell at one step (FP32):
-// F = ((p(k) - p(k-2)) * 0.5) / dt; kernel += ((0.5 * (p(k) - p(k-2))) * F) * store_snap.
-template <bool kKahan>
-__device__ __forceinline__ void snap_or_image(const StepArgs &a, int mode, int slot, size_t co, float v, float p1,
-                                              float &img, float &cmp) {
-    const float dv = v - p1;
-    if (mode == 0) {
-        __builtin_nontemporal_store((dv * 0.5f) / a.dt, gp((float *)a.snaps) + (size_t)slot * a.snap_stride + co);
-    } else {
-        const float F = __builtin_nontemporal_load(gp((const float *)a.snaps) + (size_t)slot * a.snap_stride + co);
-        const float term = ((0.5f * dv) * F) * a.ss_f;
-        if (kKahan) {
-            const float y = term - cmp, t = img + y;
-            cmp = (t - img) - y;
-            img = t;
-        } else {
-            img = img + term;
-        }
-    }
-}
-
 template <int SV>
 __device__ void tb_edge_tile(const SlotDesc &sd, const TbStep &s, int e, float *lds) {
     constexpr bool kKahan = SV == kSvF32Kahan;
@@ -235,12 +218,15 @@ __device__ void tb_edge_tile(const SlotDesc &sd, const TbStep &s, int e, float *
     int j0, j1, i0, i1;
     tb_edge_tile(sd.tb, e, a.ny, a.nx, dn, &j0, &j1, &i0, &i1);
     float *sP = lds, *sX = lds + kLH * kLW, *sY = lds + 2 * kLH * kLW;
-    int *evm = (int *)(lds + 3 * kLH * kLW);
+    float *sV0 = lds + 3 * kLH * kLW, *sV1 = sV0 + kCR * kCC;
+    int *evm = (int *)(sV1 + kCR * kCC);
+    float *tA = (float *)(evm + kCR * kCC), *tB = tA + kTabMax;
     const int lj0 = j0 - kTbH, li0 = i0 - kTbH;
     const bool zin = s.first != 0;
 
-    // stage level k0-1 of p and of the CPML memories over the tile +-9 (zero outside the grid)
-    for (int c = w; c < kLW; c += 4) {
+    // stage level k0-1 of p and of the CPML memories over the tile +-9 (zero outside the
+    // grid), the damping tables, and clear the event flags
+    for (int c = w; c < kLW; c += kEW) {
         const int i = li0 + c;
         const bool colok = i >= 0 && i < nxt;
         const int xs = i < dn ? 1 : (i >= dn + a.nx ? 2 : 0);
@@ -254,30 +240,37 @@ __device__ void tb_edge_tile(const SlotDesc &sd, const TbStep &s, int e, float *
             sY[c * kLH + r] = ok && ys != 0 ? gp(s.phy_i)[(size_t)i * dn2 + (ys == 1 ? j : j - a.ny)] : 0.f;
         }
     }
-    for (int q = threadIdx.x; q < kCR * kCC; q += kBlock) evm[q] = -1;
+    for (int q = threadIdx.x; q < dn; q += kEB) {
+        tA[q] = gp(a.fA)[q];
+        tB[q] = gp(a.fB)[q];
+    }
+    for (int q = threadIdx.x; q < kCR * kCC; q += kEB) evm[q] = -1;
     __syncthreads();
-    // injection cells of this tile (its step-1 cells +-3): event index by computed cell
+    // injection cells of this tile (its step-k0 cells +-3): both steps' values by computed cell
     const int ev0 = gp(s.ev_ptr)[e], ev1 = gp(s.ev_ptr)[e + 1];
-    for (int q = ev0 + (int)threadIdx.x; q < ev1; q += kBlock) {
-        const int cell = gp(s.ev_cell)[q];
-        const int j = dn + (cell & 0xffff), i = dn + (cell >> 16);
-        evm[(i - (i0 - 6)) * kCR + (j - (j0 - 6))] = q;
+    gptr<const float> inj0 = gp(s.inj_val) + (size_t)(s.k0 - 2) * s.ninj;
+    gptr<const float> inj1 = gp(s.inj_val) + (size_t)(s.k0 - 1) * s.ninj;
+    for (int q = ev0 + (int)threadIdx.x; q < ev1; q += kEB) {
+        const int cell = gp(s.ev_cell)[q], ij = gp(s.ev_inj)[q];
+        const int m = (dn + (cell >> 16) - (i0 - 6)) * kCR + (dn + (cell & 0xffff) - (j0 - 6));
+        evm[m] = q;
+        sV0[m] = inj0[ij];
+        sV1[m] = s.two ? inj1[ij] : 0.f;
     }
     __syncthreads();
 
-    // this thread's cells: row j0-6+lane, columns i0-6+w+4q (q < kCPW)
+    // this thread's cells: row j0-6+lane, columns i0-6+w+kEW*q (q < kCPW)
     const int j = j0 - 6 + lane;
     const bool rowin = j >= 0 && j < nyt && j < j1 + 6;
-    float p1[kCPW], c2v[kCPW], chx[kCPW], chy[kCPW], hsl[kCPW], hsp[kCPW], np1[kCPW], nx1[kCPW], ny1[kCPW];
-    float img[kCPW], cmp[kCPW];
-    int evi[kCPW];
+    const int ys = j < dn ? 1 : (j >= dn + a.ny ? 2 : 0);
+    float p1[kCPW], c2v[kCPW], chx[kCPW], chy[kCPW], hsl[kCPW], hsp[kCPW], np1[kCPW], nx1[kCPW], ny1[kCPW], tm1[kCPW];
     const EdgeView L{sP, sX, sY, lj0, li0};
 #pragma unroll
     for (int q = 0; q < kCPW; ++q) {
-        const int i = i0 - 6 + w + 4 * q;
-        const bool in = rowin && i >= 0 && i < nxt && i < i1 + 6;
-        const int xs = i < dn ? 1 : (i >= dn + a.nx ? 2 : 0), ys = j < dn ? 1 : (j >= dn + a.ny ? 2 : 0);
-        const bool near = j >= j0 - 3 && j < j1 + 3 && i >= i0 - 3 && i < i1 + 3;  // p computed at step 1
+        const int i = i0 - 6 + w + kEW * q;
+        const bool in = rowin && w + kEW * q < kCC && i >= 0 && i < nxt && i < i1 + 6;
+        const int xs = i < dn ? 1 : (i >= dn + a.nx ? 2 : 0);
+        const bool near = j >= j0 - 3 && j < j1 + 3 && i >= i0 - 3 && i < i1 + 3;  // p computed at step k0
         p1[q] = (in && near && !zin) ? gp(s.pp)[(size_t)i * ld + j] : 0.f;
         c2v[q] = (in && near) ? c2_at(a, j, i) : 0.f;
         const int xc = i - (xs == 2 ? a.nx : 0);
@@ -286,46 +279,40 @@ __device__ void tb_edge_tile(const SlotDesc &sd, const TbStep &s, int e, float *
         const bool own = in && a.ram && a.fs && j == dn && xs != 0;
         hsl[q] = (own && !zin) ? gp(s.hs_l)[xc] : 0.f;
         hsp[q] = (own && !zin) ? gp(s.hs_p)[xc] : 0.f;
-        evi[q] = in ? evm[(w + 4 * q) * kCR + lane] : -1;
-        img[q] = cmp[q] = 0.f;
-        np1[q] = nx1[q] = ny1[q] = 0.f;
+        np1[q] = nx1[q] = ny1[q] = tm1[q] = 0.f;
     }
     const bool imaging = s.mode == 1;
 
     // step k0: p and chi over the tile +-3, phi over the tile +-6
-    gptr<const float> inj0 = gp(s.inj_val) + (size_t)(s.k0 - 2) * s.ninj;
 #pragma unroll
     for (int q = 0; q < kCPW; ++q) {
-        const int i = i0 - 6 + w + 4 * q;
-        if (!(rowin && i >= 0 && i < nxt && i < i1 + 6)) continue;
+        const int i = i0 - 6 + w + kEW * q;
+        if (!(rowin && w + kEW * q < kCC && i >= 0 && i < nxt && i < i1 + 6)) continue;
         const bool near = j >= j0 - 3 && j < j1 + 3 && i >= i0 - 3 && i < i1 + 3;
-        const int xs = i < dn ? 1 : (i >= dn + a.nx ? 2 : 0), ys = j < dn ? 1 : (j >= dn + a.ny ? 2 : 0);
+        const int xs = i < dn ? 1 : (i >= dn + a.nx ? 2 : 0);
+        const int m = (w + kEW * q) * kCR + lane;
         nx1[q] = L.x(j, i);
         ny1[q] = L.y(j, i);
-        const bool src = evi[q] >= 0;
-        edge_cell(a, L, j, i, near, p1[q], c2v[q], src, src ? inj0[gp(s.ev_inj)[evi[q]]] : 0.f, chx[q], chy[q], hsl[q],
-                  hsp[q], np1[q], nx1[q], ny1[q]);
+        edge_cell(a, L, tA, tB, j, i, near, p1[q], c2v[q], evm[m] >= 0, sV0[m], chx[q], chy[q], hsl[q], hsp[q],
+                  np1[q], nx1[q], ny1[q]);
         const bool owned = j >= j0 && j < j1 && i >= i0 && i < i1;
         if (owned && xs == 0 && ys == 0 && s.snap[0] >= 0) {
             const size_t co = (size_t)(i - dn) * a.lds + (j - dn);
-            if (imaging) {
-                img[q] = gp(a.image)[co];
-                if (kKahan) cmp[q] = gp(a.image_comp)[co];
+            const float dv = np1[q] - p1[q];
+            if (!imaging) {
+                __builtin_nontemporal_store((dv * 0.5f) / a.dt, gp((float *)a.snaps) + (size_t)s.snap[0] * a.snap_stride + co);
+            } else {
+                const float F = __builtin_nontemporal_load(gp((const float *)a.snaps) + (size_t)s.snap[0] * a.snap_stride + co);
+                tm1[q] = ((0.5f * dv) * F) * a.ss_f;
             }
-            snap_or_image<kKahan>(a, s.mode, s.snap[0], co, np1[q], p1[q], img[q], cmp[q]);
-        } else if (owned && xs == 0 && ys == 0 && imaging) {
-            const size_t co = (size_t)(i - dn) * a.lds + (j - dn);
-            img[q] = gp(a.image)[co];
-            if (kKahan) cmp[q] = gp(a.image_comp)[co];
         }
     }
-    float np2[kCPW], nx2[kCPW], ny2[kCPW];
     if (s.two) {
         __syncthreads();  // every cell of level k0-1 has been read
 #pragma unroll
         for (int q = 0; q < kCPW; ++q) {
-            const int i = i0 - 6 + w + 4 * q;
-            if (!(rowin && i >= 0 && i < nxt && i < i1 + 6)) continue;
+            const int i = i0 - 6 + w + kEW * q;
+            if (!(rowin && w + kEW * q < kCC && i >= 0 && i < nxt && i < i1 + 6)) continue;
             const int c = i - li0, r = j - lj0;
             const bool near = j >= j0 - 3 && j < j1 + 3 && i >= i0 - 3 && i < i1 + 3;
             const float pold = sP[c * kLH + r];
@@ -335,55 +322,70 @@ __device__ void tb_edge_tile(const SlotDesc &sd, const TbStep &s, int e, float *
             p1[q] = pold;  // level k0-1: the previous level of step k0+1
         }
         __syncthreads();
-        // step k0+1 on the owned cells
-        gptr<const float> inj1 = gp(s.inj_val) + (size_t)(s.k0 - 1) * s.ninj;
-#pragma unroll
-        for (int q = 0; q < kCPW; ++q) {
-            const int i = i0 - 6 + w + 4 * q;
-            np2[q] = nx2[q] = ny2[q] = 0.f;
-            if (!(j >= j0 && j < j1 && i >= i0 && i < i1)) continue;
-            const int xs = i < dn ? 1 : (i >= dn + a.nx ? 2 : 0), ys = j < dn ? 1 : (j >= dn + a.ny ? 2 : 0);
-            nx2[q] = L.x(j, i);
-            ny2[q] = L.y(j, i);
-            const bool src = evi[q] >= 0;
-            edge_cell(a, L, j, i, true, p1[q], c2v[q], src, src ? inj1[gp(s.ev_inj)[evi[q]]] : 0.f, chx[q], chy[q],
-                      hsl[q], hsp[q], np2[q], nx2[q], ny2[q]);
-            if (xs == 0 && ys == 0 && s.snap[1] >= 0)
-                snap_or_image<kKahan>(a, s.mode, s.snap[1], (size_t)(i - dn) * a.lds + (j - dn), np2[q], p1[q], img[q],
-                                      cmp[q]);
-        }
     }
-    // outputs of the owned cells: the last two levels of p, the memories and the history
+    // step k0+1 on the owned cells (or, with a single step, step k0's results), and the
+    // outputs: the last two levels of p, the memories and the history, the image
 #pragma unroll
     for (int q = 0; q < kCPW; ++q) {
-        const int i = i0 - 6 + w + 4 * q;
+        const int i = i0 - 6 + w + kEW * q;
         if (!(j >= j0 && j < j1 && i >= i0 && i < i1)) continue;
-        const int xs = i < dn ? 1 : (i >= dn + a.nx ? 2 : 0), ys = j < dn ? 1 : (j >= dn + a.ny ? 2 : 0);
+        const int xs = i < dn ? 1 : (i >= dn + a.nx ? 2 : 0);
+        const int m = (w + kEW * q) * kCR + lane;
+        float np2 = np1[q], nx2 = nx1[q], ny2 = ny1[q], tm2 = 0.f;
+        if (s.two) {
+            nx2 = L.x(j, i);
+            ny2 = L.y(j, i);
+            edge_cell(a, L, tA, tB, j, i, true, p1[q], c2v[q], evm[m] >= 0, sV1[m], chx[q], chy[q], hsl[q], hsp[q],
+                      np2, nx2, ny2);
+            if (xs == 0 && ys == 0 && s.snap[1] >= 0) {
+                const size_t co = (size_t)(i - dn) * a.lds + (j - dn);
+                const float dv = np2 - p1[q];
+                if (!imaging) {
+                    __builtin_nontemporal_store((dv * 0.5f) / a.dt,
+                                                gp((float *)a.snaps) + (size_t)s.snap[1] * a.snap_stride + co);
+                } else {
+                    const float F =
+                        __builtin_nontemporal_load(gp((const float *)a.snaps) + (size_t)s.snap[1] * a.snap_stride + co);
+                    tm2 = ((0.5f * dv) * F) * a.ss_f;
+                }
+            }
+        }
         const size_t o = (size_t)i * ld + j;
         const int xc = i - (xs == 2 ? a.nx : 0);
-        if (s.two) {
-            gp(s.ol)[o] = np2[q];
-            gp(s.op)[o] = np1[q];
-        } else {
-            gp(s.ol)[o] = np1[q];
-        }
+        gp(s.ol)[o] = np2;
+        if (s.two) gp(s.op)[o] = np1[q];
         if (xs != 0) {
-            gp(s.phx_o)[(size_t)xc * ld + j] = s.two ? nx2[q] : nx1[q];
+            gp(s.phx_o)[(size_t)xc * ld + j] = nx2;
             gp(s.chx_o)[(size_t)xc * ld + j] = chx[q];
         }
         if (ys != 0) {
             const size_t yo = (size_t)i * dn2 + (ys == 1 ? j : j - a.ny);
-            gp(s.phy_o)[yo] = s.two ? ny2[q] : ny1[q];
+            gp(s.phy_o)[yo] = ny2;
             gp(s.chy_o)[yo] = chy[q];
         }
         if (a.ram && a.fs && j == dn && xs != 0) {
             gp(s.hs_ol)[xc] = hsl[q];
             gp(s.hs_op)[xc] = hsp[q];
         }
-        if (xs == 0 && ys == 0 && imaging) {
+        if (xs == 0 && ys == 0 && imaging && (s.snap[0] >= 0 || s.snap[1] >= 0)) {
+            // the image gets step k0's term, then step k0+1's (stream_item's sum or Kahan update)
             const size_t co = (size_t)(i - dn) * a.lds + (j - dn);
-            gp(a.image)[co] = img[q];
-            if (kKahan) gp(a.image_comp)[co] = cmp[q];
+            float im = gp(a.image)[co], cm = kKahan ? gp(a.image_comp)[co] : 0.f;
+            const float terms[2] = {tm1[q], tm2};
+            const int has[2] = {s.snap[0] >= 0, s.snap[1] >= 0};
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                if (!has[h]) continue;
+                if (kKahan) {
+                    const float y = terms[h] - cm, t = im + y;
+                    cm = (t - im) - y;
+                    im = t;
+                } else {
+                    im = im + terms[h];
+                }
+            }
+            gp(a.image)[co] = im;
+            if (kKahan) gp(a.image_comp)[co] = cm;
         }
     }
 }
@@ -686,19 +688,41 @@ __device__ void tb_core_item(const SlotDesc &sd, const TbStep &s, int u) {
     }
 }
 
-// One launch (two time steps) of every active slot of a launch group; the work list and
-// the slot lookup are those of rtm_batch_kernel (rtm_kernels.hip), with a slot's items
-// the edge tiles then the groups of four core items.
+// Slot and item of work item wk of a launch group's list: every slot holding `uni` items
+// (uni > 0), or else the compact list of the occupied slots (offsets woff per slot), as in
+// rtm_batch_kernel (rtm_kernels.hip).
+template <bool CMP, int LIST>
+__device__ __forceinline__ void tb_locate(const SlotDesc *slots, int n, int wk, int uni, const FastDiv &wfd,
+                                          const int *woffs, int *s, int *t) {
+    if (!CMP || uni > 0) {
+        *s = fdiv(wk, wfd);
+        *t = wk - *s * uni;
+    } else if (n <= 64) {
+        const int mine = woffs[threadIdx.x & 63];
+        *s = __popcll(__ballot(mine <= wk)) - 1;
+        *t = wk - __builtin_amdgcn_readlane(mine, *s);
+    } else {
+        int q = min(fdiv(wk, wfd), n - 1);
+        auto off = [&](int k) { return LIST == 0 ? slots[k].woff : slots[k].cwoff; };
+        while (q > 0 && off(q) > wk) --q;
+        while (q + 1 < n && off(q + 1) <= wk) ++q;
+        *s = q;
+        *t = wk - off(q);
+    }
+}
+
+// One launch (two time steps) of the edge tiles of every active slot of a launch group
+// (list 0: SlotDesc woff / wtotal / wuni / wfd); each XCD takes a contiguous range.
 template <int SV, bool CMP>
-__global__ __launch_bounds__(kBlock) void rtm_tb_kernel(BatchArgs b, const SlotDesc *__restrict__ slots,
-                                                        const long long *__restrict__ gbase) {
+__global__ __launch_bounds__(kEB) void rtm_tb_edge_kernel(BatchArgs b, const SlotDesc *__restrict__ slots,
+                                                          const long long *__restrict__ gbase) {
     __shared__ __attribute__((aligned(16))) float lds[kEdgeLds];
+    __shared__ int woffs[64];
     const long long G = *gbase + b.goff;
     const int n = b.nslots;
     const int x = blockIdx.x & 7, nbx = gridDim.x >> 3;
     int total = n * b.cap_tiles, uni = b.cap_tiles;
     FastDiv wfd = b.fd_cap;
-    __shared__ int woffs[64];
     if (CMP) {
         total = slots[0].wtotal;
         uni = slots[0].wuni;
@@ -711,50 +735,67 @@ __global__ __launch_bounds__(kBlock) void rtm_tb_kernel(BatchArgs b, const SlotD
         const int wk = x * per_xcd + m;
         if (wk >= total) break;
         int s, t;
-        if (!CMP || uni > 0) {
-            s = fdiv(wk, wfd);
-            t = wk - s * uni;
-        } else if (n <= 64) {
-            const int mine = woffs[threadIdx.x & 63];
-            s = __popcll(__ballot(mine <= wk)) - 1;
-            t = wk - __builtin_amdgcn_readlane(mine, s);
-        } else {
-            s = min(fdiv(wk, wfd), n - 1);
-            while (s > 0 && slots[s].woff > wk) --s;
-            while (s + 1 < n && slots[s + 1].woff <= wk) ++s;
-            t = wk - slots[s].woff;
-        }
+        tb_locate<CMP, 0>(slots, n, wk, uni, wfd, woffs, &s, &t);
         const SlotDesc &sd = slots[s];
         TbStep st;
-        if (!resolve_tb(sd, G, st) || t >= sd.tb.nitems) continue;
-        if (t < sd.tb.ne) {
-            tb_edge_tile<SV>(sd, st, t, lds);
-        } else {
-            const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-            const int u = 4 * (t - sd.tb.ne) + w;
-            if (u < sd.tb.ncore) tb_core_item<SV>(sd, st, u);
-        }
-        __syncthreads();  // the LDS is reused by the next item
+        if (!resolve_tb(sd, G, st) || t >= sd.tb.ne) continue;
+        tb_edge_tile<SV>(sd, st, t, lds);
+        __syncthreads();  // the LDS is reused by the next tile
+    }
+}
+
+// One launch (two time steps) of the core items of every active slot of a launch group,
+// four per workgroup (list 1: SlotDesc cwoff / cwtotal / cwuni / cwfd).
+template <int SV, bool CMP>
+__global__ __launch_bounds__(kBlock) void rtm_tb_core_kernel(BatchArgs b, const SlotDesc *__restrict__ slots,
+                                                             const long long *__restrict__ gbase) {
+    __shared__ int woffs[64];
+    const long long G = *gbase + b.goff;
+    const int n = b.nslots;
+    const int x = blockIdx.x & 7, nbx = gridDim.x >> 3;
+    int total = n * b.cap_tiles, uni = b.cap_tiles;
+    FastDiv wfd = b.fd_cap;
+    if (CMP) {
+        total = slots[0].cwtotal;
+        uni = slots[0].cwuni;
+        wfd = slots[0].cwfd;
+        if (n <= 64 && (int)threadIdx.x < 64) woffs[threadIdx.x] = (int)threadIdx.x < n ? slots[threadIdx.x].cwoff : 0x7fffffff;
+        __syncthreads();
+    }
+    const int per_xcd = (total + 7) >> 3;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    for (int m = blockIdx.x >> 3; m < per_xcd; m += nbx) {
+        const int wk = x * per_xcd + m;
+        if (wk >= total) break;
+        int s, t;
+        tb_locate<CMP, 1>(slots, n, wk, uni, wfd, woffs, &s, &t);
+        const SlotDesc &sd = slots[s];
+        TbStep st;
+        if (!resolve_tb(sd, G, st) || t >= sd.tb.ncblk) continue;
+        const int u = 4 * t + w;
+        if (u < sd.tb.ncore) tb_core_item<SV>(sd, st, u);
     }
 }
 
 template <int SV>
-void launch_tb_sv(const BatchArgs &b, int grid, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
-    auto k = b.compact ? rtm_tb_kernel<SV, true> : rtm_tb_kernel<SV, false>;
+void launch_tb_sv(const BatchArgs &b, int part, int grid, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+    auto k = part == 0 ? (b.compact ? rtm_tb_edge_kernel<SV, true> : rtm_tb_edge_kernel<SV, false>)
+                       : (b.compact ? rtm_tb_core_kernel<SV, true> : rtm_tb_core_kernel<SV, false>);
+    const int block = part == 0 ? kEB : kBlock;
     if (!e0 && !e1)
-        hipLaunchKernelGGL(k, dim3(grid), dim3(kBlock), 0, s, b, b.slots, b.gbase);
+        hipLaunchKernelGGL(k, dim3(grid), dim3(block), 0, s, b, b.slots, b.gbase);
     else
-        hipExtLaunchKernelGGL(k, dim3(grid), dim3(kBlock), 0, s, e0, e1, 0, b, b.slots, b.gbase);
+        hipExtLaunchKernelGGL(k, dim3(grid), dim3(block), 0, s, e0, e1, 0, b, b.slots, b.gbase);
 }
 
 }  // namespace
 
-void launch_tb_batch(const BatchArgs &b, int sv, int grid, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+void launch_tb_batch(const BatchArgs &b, int part, int sv, int grid, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
     grid = (grid + 7) / 8 * 8;
     if (sv == kSvF32Kahan)
-        launch_tb_sv<kSvF32Kahan>(b, grid, s, e0, e1);
+        launch_tb_sv<kSvF32Kahan>(b, part, grid, s, e0, e1);
     else
-        launch_tb_sv<kSvF32>(b, grid, s, e0, e1);
+        launch_tb_sv<kSvF32>(b, part, grid, s, e0, e1);
 }
 
 }  // namespace psdm
