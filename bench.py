@@ -224,6 +224,50 @@ def run_plain(ctx, torch, dev, W, geos, d_mig, d_src, obs, steps, warmup, timing
                 psdm_sum=psdm_sum, fold=fold)
 
 
+def run_plain_wide(ctx, torch, dev, nt, timing_every):
+    """The plain form on the widest window of the served stream: one WAS gather (801 x 5102
+    window of the 801 x 6001 @ 10 m model, 1000 receivers, dt 1 ms, FP32 snapshots every
+    step) over its first nt steps, forward + backward.  One step of this window is large
+    enough to fill the chip; returns the step kernel's roofline."""
+    import numpy as np
+
+    from psdm_amd.solvers import DeviceShot, backward_device, forward_device, ricker_was
+    from psdm_amd.workloads import step_bytes, updates_per_shot, was_gathers
+
+    G = was_gathers(1, seed=2)
+    m, sh = G["model"], G["shots"][0]
+    ny, nxw, dt, dPML = m.nz, sh.nx_w, G["dt"], G["dPML"]
+    geo = DeviceShot(ny, nxw, nt, dPML, 0, m.dx, dt, [sh.nxSou - sh.x_i + 1], [sh.nySou], sh.nxRec - sh.x_i + 1,
+                     sh.nyRec)
+    d_mig = torch.from_numpy(np.ascontiguousarray(m.migration[:, sh.x_i - 1:sh.x_f].T)).to(dev)
+    d_src = torch.from_numpy(ricker_was(nt, dt, G["f0"]).reshape(1, nt)).to(dev)
+    g = torch.Generator(device=dev)
+    g.manual_seed(9)
+    d_obs = 1e-3 * torch.randn((sh.nxRec.size, nt), generator=g, device=dev)
+    d_rec = torch.empty((sh.nxRec.size, nt), device=dev)
+    d_img = torch.zeros((nxw, ny), device=dev)
+    torch.cuda.synchronize()
+    forward_device(ctx, geo, d_mig, d_src, d_rec, key=-11)  # warm-up (allocations)
+    backward_device(ctx, geo, d_mig, d_obs, d_img, key=-11)
+    ctx.sync()
+    ctx.set_timing(timing_every)
+    t0 = time.perf_counter()
+    forward_device(ctx, geo, d_mig, d_src, d_rec, key=-11)
+    backward_device(ctx, geo, d_mig, d_obs, d_img, key=-11)
+    ctx.sync()
+    el = time.perf_counter() - t0
+    n_ev, mean_ms = ctx.step_stats()
+    ctx.set_timing(0)
+    b = 0.5 * (step_bytes(ny, nxw, dPML) + step_bytes(ny, nxw, dPML, backward=True))
+    ach = b / (mean_ms * 1e-3) / 1e9
+    return {"workload": f"WAS gather, {ny}x{nxw} window, first {nt} of 12000 steps (dt 1 ms), FP32 snapshots",
+            "gpt_updates_per_s": round(updates_per_shot(ny, nxw, nt, dPML) / el / 1e9, 3),
+            "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(ach / HBM_PEAK_GBS, 4), "kernel": "rtm_step_kernel",
+                         "mean_kernel_ms": round(mean_ms, 5), "sampled_launches": n_ev,
+                         "algorithmic_bytes_per_launch": round(b)}}
+
+
 def run_batched(ctx, torch, dev, W, d_mig, d_src, obs, steps, warmup, cfg):
     """Many shots per launch: the engine keeps `nslots` shots resident."""
     from psdm_amd.engine import Engine
@@ -397,6 +441,9 @@ def bench_uneven(args, mixed=False):
         plain = {"shots_per_s": round(len(shots) / el, 4), "gpt_updates_per_s": round(upd / el / 1e9, 3),
                  "elapsed_s": round(el, 3), "tile_x": args.tile_x}
         log(f"[bench] plain {plain['shots_per_s']} shots/s")
+        if not args.no_wide:
+            plain["wide_window"] = run_plain_wide(ctx, torch, dev, args.wide_steps, args.timing_every)
+            log(f"[bench] plain, wide window: roofline {plain['wide_window']['roofline']['frac']}")
 
     base = results["f32_ss1"]
     if mixed:
@@ -894,7 +941,7 @@ def bench_served(args):
                 for q in range(n)]
         eng = engines()
         log(f"[bench] served: rate {rate}/s")
-        res = Server(eng).run(reqs, log=log)
+        res = Server(eng, fixed_bytes={k: e.memory()[0] for k, e in eng.items()}).run(reqs, log=log)
         for e in eng.values():
             e.close()
         res.pop("requests")
@@ -951,6 +998,8 @@ def main():
                          "cores one GPU of the box is allotted)")
     ap.add_argument("--cpu-steps", type=int, default=1200, help="CPU baseline: time steps of the sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-wide", action="store_true", help="skip the plain form on the 801 x 5102 window")
+    ap.add_argument("--wide-steps", type=int, default=1500, help="plain form on the 801 x 5102 window: steps timed")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="batched form with the default launch shape")
     ap.add_argument("--engine-cfg", default=None,
@@ -974,7 +1023,8 @@ def main():
     ap.add_argument("--hbm-gb", type=float, default=200.0, help="uneven/mixed/served: snapshot budget (GiB)")
     ap.add_argument("--no-plain", action="store_true", help="uneven/mixed: skip the plain form")
     ap.add_argument("--requests", type=int, default=40, help="served: requests per rate")
-    ap.add_argument("--rates", default="0.5,1,2,4", help="served: Poisson arrival rates (requests/s)")
+    ap.add_argument("--rates", default="1,4,8,16",
+                    help="served: Poisson arrival rates (requests/s); the upper ones exceed the stream's capacity")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -1060,6 +1110,9 @@ def main():
                               "kernel": "rtm_step_kernel", "mean_kernel_ms": round(P["mean_ms"], 5),
                               "sampled_launches": P["n_ev"], "algorithmic_bytes_per_launch": round(step_b)}}
         log(f"[bench] plain {plain['shots_per_s']} shots/s")
+        if not args.no_wide:
+            plain["wide_window"] = run_plain_wide(ctx, torch, dev, args.wide_steps, args.timing_every)
+            log(f"[bench] plain, wide window: roofline {plain['wide_window']['roofline']['frac']}")
 
     if "batched" in forms:
         cfg = dict(nslots=16, tile_x=16, stream_w=4, grid=0, groups=2, chunk_steps=64)

@@ -195,6 +195,13 @@ int psdm_rtm_engine_image(psdm_rtm_engine *engine, double **d_psdm_sum, int **d_
 int psdm_rtm_engine_timing(psdm_rtm_engine *engine, double *mean_launch_ms, long long *n_launches);
 int psdm_rtm_engine_stats(psdm_rtm_engine *engine, long long *steps_issued, long long *slot_steps,
                           long long *hbm_used, long long *hbm_budget);
+/* Device memory of the engine: fixed_bytes, reserved at creation for nslots shots of the
+ * slot capacity (wavefields, CPML memories, images, the model's c2, descriptors, an
+ * engine-owned stack); arena_bytes, the snapshot arena that admission fills shot by shot
+ * (hbm_budget).  A resident shot's footprint is its slot's share of fixed_bytes plus its
+ * snapshot store, so admitting by the snapshot store against the arena admits by the
+ * whole footprint. */
+int psdm_rtm_engine_memory(psdm_rtm_engine *engine, long long *fixed_bytes, long long *arena_bytes);
 
 /* ------------------------------------------------------------------------------
  * Gather conditioning before migration: the per-shot steps of the driver between

@@ -111,7 +111,7 @@ struct psdm_rtm_engine {
     int dn = 0, ld = 0, lds = 0, off = 0, sw = 16, nyt = 0, nxt_max = 0, cap_tiles = 0, nwork = 0, per_xcd = 0, grid = 0;
     int spl = 1;        // time steps per launch (2: temporal blocking, rtm_tblock.hip)
     int cap_core = 0;   // two steps per launch: core workgroups of a slot at capacity (cap_tiles: edge tiles)
-    size_t win = 0;  // per-slot length of the c2 / image / comp buffers
+    size_t win = 0;  // per-slot length of the image / comp buffers
     // slots split into `groups` independent launch chains inside a captured chunk, so one
     // group's next step can start while the other's drains
     int groups = 1;
@@ -320,8 +320,10 @@ int admit(psdm_rtm_engine *e, int s, Job &job, size_t arena_off) {
     float *inj_f = sl.d_inj, *inj_b = sl.d_inj + (size_t)(nt - 1) * ninj_f;
     launch_gather(J.d_wavelet, nt, sl.d_ints + ofcol, ninj_f, inj_f, 0, c->stream);
     launch_gather(J.d_data, nt, sl.d_ints + obcol, ninj_b, inj_b, 1, c->stream);
-    float *c2 = e->d_c2 + (size_t)s * e->win + kIntPad;
-    launch_c2(e->cfg.d_vel + (size_t)(J.x_i - 1) * ny, c2, ny, nx, e->lds, e->cfg.dt, e->cfg.dmodel, c->stream);
+    // the shot's c2 window is a view of the engine's one c2 array over the whole model (the
+    // reference cuts each shot's window from one Vpmodel, calculate_PSDM_RTM.f90:400-402), so
+    // resident shots that overlap share their c2 cache lines
+    const float *c2 = e->d_c2 + (size_t)(J.x_i - 1) * e->lds + kIntPad;
     // the host vectors above are read by the queued copy: wait before they go away
     HIP_TRY(c, hipStreamSynchronize(c->stream));
 
@@ -623,7 +625,9 @@ int psdm_rtm_engine_create(psdm_rtm_ctx *c, const psdm_rtm_engine_cfg *cfg, psdm
     ETRY(hipMalloc(&e->d_gbase, sizeof(long long)));
     ETRY(hipMalloc(&e->d_tabs, 2 * e->dn * sizeof(float)));
     ETRY(hipMalloc(&e->d_state, S * e->state_stride * sizeof(float)));
-    ETRY(hipMalloc(&e->d_c2, S * win * sizeof(float)));
+    // c2 = v^2 dt^2 / dx^2 of the whole model, in the interior layout (lds rows per column)
+    ETRY(hipMalloc(&e->d_c2, ((size_t)e->lds * cfg->nx_prop + 64) * sizeof(float)));
+    launch_c2(cfg->d_vel, e->d_c2 + kIntPad, cfg->ny, cfg->nx_prop, e->lds, cfg->dt, cfg->dmodel, c->stream);
     ETRY(hipMalloc(&e->d_img, S * win * sizeof(float)));
     ETRY(hipMalloc(&e->d_comp, S * win * sizeof(float)));
     ETRY(hipMalloc(&e->d_arena, std::max<size_t>(budget, 256)));
@@ -820,6 +824,19 @@ int psdm_rtm_engine_timing(psdm_rtm_engine *e, double *mean_launch_ms, long long
     const long long n = e->chunk_ms_n * e->cfg.chunk_steps;
     if (n_launches) *n_launches = n;
     if (mean_launch_ms) *mean_launch_ms = n > 0 ? e->chunk_ms_sum / (double)n : 0.0;
+    return 0;
+}
+
+int psdm_rtm_engine_memory(psdm_rtm_engine *e, long long *fixed_bytes, long long *arena_bytes) {
+    if (!e) return PSDM_RTM_EINVAL;
+    const size_t S = e->cfg.nslots;
+    size_t fixed = S * sizeof(SlotDesc) + sizeof(long long) + 2 * e->dn * sizeof(float);
+    fixed += S * e->state_stride * sizeof(float) + 2 * S * e->win * sizeof(float);
+    fixed += ((size_t)e->lds * e->cfg.nx_prop + 64) * sizeof(float);
+    if (e->own_image) fixed += (size_t)e->cfg.ny * e->cfg.nx_prop * (sizeof(double) + sizeof(int));
+    for (auto &sl : e->slots) fixed += sl.n_ints * sizeof(int) + sl.n_inj * sizeof(float);
+    if (fixed_bytes) *fixed_bytes = (long long)fixed;
+    if (arena_bytes) *arena_bytes = (long long)e->arena.size;
     return 0;
 }
 

@@ -132,3 +132,9 @@ class Engine:
         self.ctx.check(self._L.psdm_rtm_engine_stats(self.h, C.byref(a), C.byref(b), C.byref(c), C.byref(d)),
                        "psdm_rtm_engine_stats")
         return dict(steps_issued=a.value, slot_steps=b.value, hbm_used=c.value, hbm_budget=d.value)
+
+    def memory(self):
+        """(fixed device bytes reserved at creation for the slots, snapshot arena bytes)."""
+        a, b = _LL(), _LL()
+        self.ctx.check(self._L.psdm_rtm_engine_memory(self.h, C.byref(a), C.byref(b)), "psdm_rtm_engine_memory")
+        return a.value, b.value
