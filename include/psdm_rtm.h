@@ -108,6 +108,14 @@ int psdm_rtm_sync(psdm_rtm_ctx *ctx);
  * 1 there, and d_kernel is cleared for the next shot.  Ordered on the context stream. */
 int psdm_rtm_stack_image(psdm_rtm_ctx *ctx, float *d_kernel, int ny, int nx_w, int x_i, double *d_psdm_sum,
                          int *d_cov_sum, int nx_prop);
+/* The driver's taper of the forward snapshots (calculate_PSDM_RTM.f90:421-427 with
+ * taper_option != 0: taper_pad_trace, filters.f90:20-48, on every cell's Fields(j,i,:))
+ * applied to the snapshots a _ram forward left in HBM under its Fields address: snapshots
+ * i0 .. nt_prop2 (i0 = max(1, nt_prop2 - N_taper + 1)) are multiplied by
+ * w(n) = 0.5*(1 + cos(pi*n/N_taper)), pi = 4 atan(1), in FP32 as the reference does;
+ * snapshots the forward did not write stay 0 (the padding).  Bit for bit the driver's
+ * result, without copying the snapshots to the host and back.  Synchronous. */
+int psdm_rtm_taper_snapshots(psdm_rtm_ctx *ctx, const float *Fields, int nt_prop2, int N_taper);
 /* normalize_psdm (auxiliar_PSDM.f90:1-20): d_psdm = d_psdm_sum / d_cov_sum where the
  * fold is positive, else 0. */
 int psdm_rtm_normalize(psdm_rtm_ctx *ctx, const double *d_psdm_sum, const int *d_cov_sum, int ny, int nx,

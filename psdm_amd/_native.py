@@ -50,6 +50,7 @@ SIGNATURES = {
     "psdm_rtm_condition_timing": ([_P, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(_LL)], _I),
     "psdm_rtm_engine_stats": ([_P, C.POINTER(_LL), C.POINTER(_LL), C.POINTER(_LL), C.POINTER(_LL)], _I),
     "psdm_rtm_engine_memory": ([_P, C.POINTER(_LL), C.POINTER(_LL)], _I),
+    "psdm_rtm_taper_snapshots": ([_P, _P, _I, _I], _I),
     "psdm_rtm_su_decode": ([_P, _P, _LL, _I, _I, _I, _I, _I, _P], _I),
     "psdm_rtm_su_header_i32": ([_P, _P, _LL, _I, _I, _I, _I, _P], _I),
     "psdm_rtm_su_read_host": ([_P, C.c_char_p, _LL, _I, _I, _I, _I, _I, _P], _I),

@@ -6,9 +6,10 @@
 ! 134-137, 177-179, 221-224) and calculate_PSDM_RTM.f90 unchanged.
 !
 ! Module inputs are read exactly where the reference reads them: dPML and fs from
-! mod_parfile (solver_ac_forward_ram.f90:8); taper_option decides whether the forward
-! snapshots must be materialised in the caller's Fields (the driver tapers them in
-! place, calculate_PSDM_RTM.f90:421-427) or can stay in HBM until the backward call.
+! mod_parfile (solver_ac_forward_ram.f90:8).  The forward snapshots stay in HBM until the
+! backward call; with taper_option != 0 the driver's taper of Fields (calculate_PSDM_RTM.f90:
+! 421-427, N_taper = max(1, int(T_taper/dt_prop)) at :287) is applied to them there at the
+! end of the forward call, so the caller's Fields are never materialised.
 ! Errors keep the reference's fail-fast behaviour: a message, then error stop.
 
 module psdm_rtm_iface
@@ -47,6 +48,13 @@ module psdm_rtm_iface
       integer(c_int) :: nxSou(*), nySou(*), nxRec(*), nyRec(*)
       real(c_float), value :: dmodel, dt
       real(c_float) :: S(*), ctmp(*), Fields(*), Rec_point(*)
+    end function
+    integer(c_int) function psdm_rtm_taper_snapshots(ctx, Fields, nt_prop2, N_taper) &
+        bind(C, name="psdm_rtm_taper_snapshots")
+      import :: c_ptr, c_int, c_float
+      type(c_ptr), value :: ctx
+      real(c_float) :: Fields(*)
+      integer(c_int), value :: nt_prop2, N_taper
     end function
     integer(c_int) function psdm_rtm_backward_ram(ctx, dPML, fs, NumSou, NumRec, nxSou, nySou, nxRec, nyRec, Sadj, &
         dmodel, dt, ny, nx, ctmp, store_snap, nt_snap, nt, Fields, host_fields, kernel) &
@@ -142,36 +150,35 @@ end module psdm_rtm_iface
 subroutine solver_ac_forward_ram(rank,NumSou,NumRec,nxSou,nySou,nxRec,nyRec,&
 S,dmodel,dt,ny,nx,ctmp,&
 store_snap,nt_snap,nt,Fields,Rec_point)
-  use mod_parfile, only: dPML, fs, taper_option
+  use mod_parfile, only: dPML, fs, taper_option, T_taper
   use psdm_rtm_iface
   implicit none
   integer :: rank, NumSou, NumRec, ny, nx, store_snap, nt_snap, nt
   integer :: nxSou(NumSou), nySou(NumSou), nxRec(NumRec), nyRec(NumRec)
   real :: dmodel, dt
   real :: S(nt,NumSou), ctmp(ny,nx), Fields(ny,nx,nt_snap), Rec_point(nt,NumRec)
-  integer(c_int) :: host
-  host = 0
-  if (taper_option /= 0) host = 1
+  integer(c_int) :: N_taper
   call psdm_check(psdm_rtm_forward_ram(psdm_context(), dPML, fs, NumSou, NumRec, nxSou, nySou, nxRec, nyRec, S, &
-                  dmodel, dt, ny, nx, ctmp, store_snap, nt_snap, nt, Fields, host, Rec_point), 'solver_ac_forward_ram')
+                  dmodel, dt, ny, nx, ctmp, store_snap, nt_snap, nt, Fields, 0, Rec_point), 'solver_ac_forward_ram')
+  if (taper_option /= 0) then
+    N_taper = max(1, int(T_taper / dt))
+    call psdm_check(psdm_rtm_taper_snapshots(psdm_context(), Fields, nt, N_taper), 'solver_ac_forward_ram (taper)')
+  endif
 end subroutine
 
 
 subroutine solver_ac_backward_ram(rank,NumSou,NumRec,nxSou,nySou,nxRec,nyRec,&
 Sadj,dmodel,dt,ny,nx,ctmp,&
 store_snap,nt_snap,nt,Fields,kernel)
-  use mod_parfile, only: dPML, fs, taper_option
+  use mod_parfile, only: dPML, fs
   use psdm_rtm_iface
   implicit none
   integer :: rank, NumSou, NumRec, ny, nx, store_snap, nt_snap, nt
   integer :: nxSou(NumSou), nySou(NumSou), nxRec(NumRec), nyRec(NumRec)
   real :: dmodel, dt
   real :: Sadj(nt,NumRec), ctmp(ny,nx), Fields(ny,nx,nt_snap), kernel(ny,nx)
-  integer(c_int) :: host
-  host = 0
-  if (taper_option /= 0) host = 1
   call psdm_check(psdm_rtm_backward_ram(psdm_context(), dPML, fs, NumSou, NumRec, nxSou, nySou, nxRec, nyRec, Sadj, &
-                  dmodel, dt, ny, nx, ctmp, store_snap, nt_snap, nt, Fields, host, kernel), 'solver_ac_backward_ram')
+                  dmodel, dt, ny, nx, ctmp, store_snap, nt_snap, nt, Fields, 0, kernel), 'solver_ac_backward_ram')
 end subroutine
 
 

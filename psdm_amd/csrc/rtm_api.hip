@@ -634,6 +634,34 @@ int psdm_rtm_stack_image(psdm_rtm_ctx *c, float *d_kernel, int ny, int nx_w, int
     return 0;
 }
 
+int psdm_rtm_taper_snapshots(psdm_rtm_ctx *c, const float *Fields, int nt_prop2, int N_taper) {
+    if (!c) return PSDM_RTM_EINVAL;
+    if (nt_prop2 < 1 || N_taper < 1) return fail(c, PSDM_RTM_EINVAL, "taper: need nt_prop2 >= 1 and N_taper >= 1");
+    auto it = c->snaps.find((long long)(uintptr_t)Fields);
+    if (it == c->snaps.end()) return fail(c, PSDM_RTM_ENOENT, "no device-resident snapshots under this key");
+    if (int rc = set_device(c)) return rc;
+    const SnapStore &st = it->second;
+    // taper_pad_trace: i0 = max(1, nt_prop2 - N_taper + 1); trace(i0+n) *= w(n) for
+    // i0+n <= nt_prop2, w(n) = 0.5*(1 + cos(pi*n/N_taper)), pi = 4 atan(1), FP32; snapshot
+    // s (1-based) is slot s-1 of the store, and slots the forward did not write hold zeros
+    const int i0 = std::max(1, nt_prop2 - N_taper + 1);
+    const int last = std::min(nt_prop2, st.nslots);
+    const int nq = last - i0 + 1;
+    if (nq <= 0 || !st.d) return 0;
+    const float pi = 4.0f * std::atan(1.0f);
+    std::vector<float> w(nq);
+    for (int n = 0; n < nq; ++n) w[n] = 0.5f * (1.0f + std::cos(pi * (float)n / (float)N_taper));
+    void *v;
+    if (int rc = get_buf(c, "taper_w", w.size() * sizeof(float), &v)) return rc;
+    HIP_TRY(c, hipMemcpyAsync(v, w.data(), w.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    const int lds = psdm::interior_ld(st.ny);
+    psdm::launch_taper_snaps(st.d + psdm::kIntPad, (size_t)lds * st.nx, lds, st.ny, st.nx, i0 - 1, nq, (const float *)v,
+                             c->stream);
+    HIP_TRY(c, hipGetLastError());
+    HIP_TRY(c, hipStreamSynchronize(c->stream));  // the host weights go out of scope
+    return 0;
+}
+
 int psdm_rtm_normalize(psdm_rtm_ctx *c, const double *d_psdm_sum, const int *d_cov_sum, int ny, int nx,
                        float *d_psdm) {
     if (!c) return PSDM_RTM_EINVAL;

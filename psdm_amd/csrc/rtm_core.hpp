@@ -342,6 +342,9 @@ void launch_c2(const float *vel, float *c2, int ny, int nx, int ldc, float dt, f
 void launch_stack(float *img, float *comp, int ld_img, int ny, int nx_w, int x0, double *sum, int ld_sum, int *fold,
                   hipStream_t s);
 void launch_normalize(const double *sum, const int *fold, float *out, size_t n, hipStream_t s);
+// snaps[(s0+q)*stride + i*lds + j] *= w[q], q < nq, i < nx, j < ny
+void launch_taper_snaps(float *snaps, size_t stride, int lds, int ny, int nx, int s0, int nq, const float *w,
+                        hipStream_t s);
 // out[r*ncols + e] = traces[cols[e]*nt + (reverse ? nt-1-r : r)], r in [0, nt-1)
 void launch_gather(const float *traces, int nt, const int *cols, int ncols, float *out, int reverse, hipStream_t s);
 

@@ -686,6 +686,18 @@ __global__ void rtm_stack_kernel(float *img, float *comp, int ld_img, int ny, in
     }
 }
 
+// taper_pad_trace's half-cosine taper (filters.f90:20-48) on resident snapshots: slot
+// s0 + q (0-based, q < nq) of every interior cell is multiplied by w[q] (FP32, as the
+// driver's trace(i0+n) * w on Fields, calculate_PSDM_RTM.f90:421-427).
+__global__ void rtm_taper_snaps_kernel(float *snaps, size_t stride, int lds, int ny, int nx, int s0, const float *w) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y, q = blockIdx.z;
+    if (j < ny && i < nx) {
+        float *f = snaps + (size_t)(s0 + q) * stride + (size_t)i * lds + j;
+        *f = *f * w[q];
+    }
+}
+
 // normalize_psdm (auxiliar_PSDM.f90:1-20): PSDM = PSDM_sum / Cov_sum where the fold is
 // positive, 0 elsewhere.
 __global__ void rtm_normalize_kernel(const double *sum, const int *fold, float *out, size_t n) {
@@ -762,6 +774,13 @@ void launch_stack(float *img, float *comp, int ld_img, int ny, int nx_w, int x0,
                   hipStream_t s) {
     hipLaunchKernelGGL(rtm_stack_kernel, dim3((ny + 255) / 256, nx_w), dim3(256), 0, s, img, comp, ld_img, ny, nx_w, x0,
                        sum, ld_sum, fold);
+}
+
+void launch_taper_snaps(float *snaps, size_t stride, int lds, int ny, int nx, int s0, int nq, const float *w,
+                        hipStream_t s) {
+    if (nq <= 0) return;
+    hipLaunchKernelGGL(rtm_taper_snaps_kernel, dim3((ny + 255) / 256, nx, nq), dim3(256), 0, s, snaps, stride, lds, ny,
+                       nx, s0, w);
 }
 
 void launch_normalize(const double *sum, const int *fold, float *out, size_t n, hipStream_t s) {

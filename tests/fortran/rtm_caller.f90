@@ -1,14 +1,17 @@
 ! Test caller of the Fortran drop-in: reproduces the solver calls of one RTM shot as
 ! calculate_PSDM_RTM.f90:395-466 makes them (Fields allocated and zeroed by the caller,
 ! forward, BackSource = data reversed in time, PSDM_w zeroed, backward with
-! nt_snap = nt), going through psdm_rtm_fortran.f90 -> libpsdm_rtm.so.
+! nt_snap = nt), going through psdm_rtm_fortran.f90 -> libpsdm_rtm.so.  With taper != 0
+! the driver would next taper its host Fields (calculate_PSDM_RTM.f90:421-427); the shim
+! keeps the snapshots on the device and tapers them there, so the host Fields stay zero
+! and their taper changes nothing: it is left out here.
 ! Usage: rtm_caller in.bin out.bin
-!   in.bin : int32 ny,nx,nt,NumRec,dPML,fs,taper,variant; float32 dmodel,dt;
+!   in.bin : int32 ny,nx,nt,NumRec,dPML,fs,taper,variant; float32 dmodel,dt,T_taper;
 !            int32 nxSou,nySou,nxRec(NumRec),nyRec(NumRec); float32 S(nt), ctmp(ny,nx),
 !            Data(nt,NumRec)
 !   out.bin: float32 Rec_point(nt,NumRec), kernel(ny,nx)
 program rtm_caller
-  use mod_parfile, only: dPML, fs, taper_option
+  use mod_parfile, only: dPML, fs, taper_option, T_taper
   implicit none
   integer :: ny, nx, nt, NumRec, variant, u, ierr
   integer :: nxSou(1), nySou(1)
@@ -23,7 +26,7 @@ program rtm_caller
   open(newunit=u, file=trim(fin), access='stream', form='unformatted', status='old', iostat=ierr)
   if (ierr /= 0) error stop 'cannot open input'
   read(u) ny, nx, nt, NumRec, dPML, fs, taper_option, variant
-  read(u) dmodel, dt
+  read(u) dmodel, dt, T_taper
   allocate(nxRec(NumRec), nyRec(NumRec))
   read(u) nxSou, nySou, nxRec, nyRec
   allocate(S(nt,1), ctmp(ny,nx), Data(nt,NumRec), Rec(nt,NumRec), BackSource(nt,NumRec), kernel(ny,nx))

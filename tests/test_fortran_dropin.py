@@ -43,12 +43,12 @@ def test_shim_links_against_reference_callers():
         assert s in defined
 
 
-def _write_input(path, c, data):
+def _write_input(path, c, data, taper=0, T_taper=0.0):
     ny, nx, nt = c["ny"], c["nx"], c["nt"]
     NumRec = c["nxRec"].size
     with open(path, "wb") as f:
-        np.array([ny, nx, nt, NumRec, c["dPML"], c["fs"], 0, c["variant"]], np.int32).tofile(f)
-        np.array([c["dx"], c["dt"]], np.float32).tofile(f)
+        np.array([ny, nx, nt, NumRec, c["dPML"], c["fs"], taper, c["variant"]], np.int32).tofile(f)
+        np.array([c["dx"], c["dt"], T_taper], np.float32).tofile(f)
         np.array([c["nxSou"][0], c["nySou"][0]], np.int32).tofile(f)
         c["nxRec"].astype(np.int32).tofile(f)
         c["nyRec"].astype(np.int32).tofile(f)
@@ -58,19 +58,26 @@ def _write_input(path, c, data):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", ["ram_fs0", "ram_fs1", "disk_fs1"])
-def test_fortran_caller_through_shim_matches_oracle(name):
+@pytest.mark.parametrize("name,taper", [("ram_fs0", 0), ("ram_fs1", 0), ("disk_fs1", 0), ("ram_fs0", 1),
+                                        ("ram_fs1", 1)])
+def test_fortran_caller_through_shim_matches_oracle(name, taper):
+    """With taper_option != 0 the snapshots are tapered on the device (never copied to the
+    host); the oracle tapers its own snapshots as the driver tapers Fields
+    (calculate_PSDM_RTM.f90:421-427; oracle/conditioning.taper_weights, pinned against the
+    reference's taper_pad_trace), and the image must match it bit for bit."""
+    from oracle import conditioning as OC
     from psdm_amd.build import CALLER
 
     assert os.path.exists(CALLER), "tests/fortran/_build/rtm_caller is built by __graft_entry__.build()"
     c = case(name)
-    print(f"[gpu] fortran caller case={name}", flush=True)
+    T_taper = 0.05
+    print(f"[gpu] fortran caller case={name} taper={taper}", flush=True)
     geo = (c["nxSou"], c["nySou"], c["nxRec"], c["nyRec"])
     rng = np.random.default_rng(3)
     data = np.asfortranarray(rng.standard_normal((c["nt"], c["nxRec"].size)).astype(np.float32) * 1e-3)
     tmp = tempfile.mkdtemp()
     try:
-        _write_input(os.path.join(tmp, "in.bin"), c, data)
+        _write_input(os.path.join(tmp, "in.bin"), c, data, taper, T_taper)
         subprocess.run([CALLER, "in.bin", "out.bin"], cwd=tmp, check=True, timeout=120)
         out = np.fromfile(os.path.join(tmp, "out.bin"), np.float32)
     finally:
@@ -80,6 +87,14 @@ def test_fortran_caller_through_shim_matches_oracle(name):
     kernel = out[nt * NumRec:].reshape((ny, nx), order="F")
     Ro, Fo = O.forward(*geo, c["S"], c["dx"], c["dt"], c["ctmp"], 1, nt, nt, c["dPML"], c["fs"],
                        variant=c["variant"], table_mode=0)
+    if taper:
+        N_taper = max(1, int(np.float32(T_taper) / np.float32(c["dt"])))  # calculate_PSDM_RTM.f90:287
+        w = OC.taper_weights(N_taper)
+        i0 = max(1, nt - N_taper + 1)
+        for n in range(N_taper):
+            if i0 + n <= nt:
+                Fo[:, :, i0 + n - 1] = (Fo[:, :, i0 + n - 1] * w[n]).astype(np.float32)
+        assert N_taper > 1
     ko = O.backward(*geo, np.asfortranarray(data[::-1, :]), c["dx"], c["dt"], c["ctmp"], 1, nt, nt, c["dPML"],
                     c["fs"], Fields=Fo, variant=c["variant"], table_mode=0)
     np.testing.assert_array_equal(Rec, Ro)
