@@ -353,14 +353,22 @@ int psdm_rtm_nav_read(const char *path, int cols, int dir_boat, float d_shot_mod
  * device, column-major = TT_r(ny, nx_w, NumRec) (or TT_s for one source).  Four fast
  * sweeps from T = 1e9, T(source) = 0, bit for bit.  nx <= 8192.  Synchronous.
  * Device workspace held by the ctx: 4 * (2 * (nx + ny - 1) * nx + 1024) bytes per source
- * plus 8 * (nx + ny - 1) * nx for 1/v (3.6 GB for 481 sources in a 601 x 715 window). */
+ * solved at once plus 8 * (nx + ny - 1) * nx for 1/v.  The sources run in batches of as
+ * many as the workspace limit holds (psdm_rtm_set_eikonal_workspace, default 16 GiB: all
+ * 481 of a 601 x 715 window, 71 of an 801 x 5102 one), at least one per batch. */
 int psdm_rtm_eikonal(psdm_rtm_ctx *ctx, const float *d_vel, int ny, int nx, float dx, float dy, int nsrc,
                      const int *src_x, const int *src_z, float *d_tt);
 /* Host-array form of psdm_rtm_eikonal for the Fortran seam (dynamic_window_tt_forward /
  * _backward's arrays): h_vel (ny, nx) and h_tt (ny, nx, nsrc) column-major host arrays.
- * Synchronous. */
+ * The device holds one batch of travel times at a time.  Synchronous. */
 int psdm_rtm_eikonal_host(psdm_rtm_ctx *ctx, const float *h_vel, int ny, int nx, float dx, float dy, int nsrc,
                           const int *src_x, const int *src_z, float *h_tt);
+/* No counterpart in the reference (its solver keeps one source's arrays at a time): the
+ * limit on the eikonal workspace of this ctx, in bytes (> 0; default 16 GiB). */
+int psdm_rtm_set_eikonal_workspace(psdm_rtm_ctx *ctx, size_t max_bytes);
+/* *nb = the sources psdm_rtm_eikonal solves per launch for this shape under the ctx's
+ * workspace limit (1 <= *nb <= nsrc). */
+int psdm_rtm_eikonal_batch(psdm_rtm_ctx *ctx, int ny, int nx, int nsrc, int *nb);
 /* The imaging loop of calculate_PSDM_kirchoff.f90:394-447: d_img (ny, nx) device =
  * sum over receivers k (in order) of weight * (d(it+1) - d(it-1)) / (2*dt) with
  * it = 1 + floor((TT_s + TT_r(k))/dt) (skipped unless 1 < it < nt_prop), Veff = vp0,

@@ -62,6 +62,7 @@ struct psdm_rtm_ctx {
     hipEvent_t kir_ev[4] = {};
     double kir_ms[2] = {0.0, 0.0};
     long long kir_n[2] = {0, 0};
+    size_t eik_work_cap = (size_t)16 << 30;  // eikonal workspace limit (sources run in batches)
 };
 
 namespace psdm {

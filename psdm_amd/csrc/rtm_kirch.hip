@@ -21,6 +21,7 @@
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -159,7 +160,6 @@ __global__ __launch_bounds__(kMaxBlock) void eikonal_fsm_kernel(const float *__r
         // Column c (sweep order) is at row r = d - c in step d.  Per column the thread keeps
         // the old T and 1/v of rows r .. r+P+1 in a ring (tq, vq; the old T(r+1) is what the
         // previous column needs from it this step) and prv = new T(r-1).
-        const bool iup = sweep == 1 || sweep == 4, jup = sweep <= 2;
         // sweep-order cell (r, c) of step d lies at D * nx + X of the sweep's layout, with
         // D = d, X = c (sweeps 1, 2) or D = nx+ny-2-d, X = nx-1-c (sweeps 3, 4)
         const bool fwd = sweep <= 2;
@@ -319,10 +319,10 @@ __global__ __launch_bounds__(256) void kirchhoff_image_kernel(const float *__res
 
 using namespace psdm;
 
-extern "C" int psdm_rtm_eikonal(psdm_rtm_ctx *c, const float *d_vel, int ny, int nx, float dx, float dy, int nsrc,
-                                const int *src_x, const int *src_z, float *d_tt) {
-    if (!c) return PSDM_RTM_EINVAL;
-    if (!d_vel || !d_tt || !src_x || !src_z) return fail(c, PSDM_RTM_EINVAL, "eikonal: null argument");
+namespace {
+
+int check_eikonal(psdm_rtm_ctx *c, int ny, int nx, float dx, float dy, int nsrc, const int *src_x, const int *src_z) {
+    if (!src_x || !src_z) return fail(c, PSDM_RTM_EINVAL, "eikonal: null argument");
     if (ny < 1 || nx < 1 || nx > kMaxCols * kMaxBlock || nsrc < 1)
         return fail(c, PSDM_RTM_EINVAL, "eikonal: need ny >= 1, 1 <= nx <= 8192, nsrc >= 1");
     if (!(dx > 0.f) || !(dy > 0.f)) return fail(c, PSDM_RTM_EINVAL, "eikonal: dx, dy must be > 0");
@@ -331,49 +331,106 @@ extern "C" int psdm_rtm_eikonal(psdm_rtm_ctx *c, const float *d_vel, int ny, int
     for (int s = 0; s < nsrc; ++s)
         if (src_x[s] < 1 || src_x[s] > nx || src_z[s] < 1 || src_z[s] > ny)
             return fail(c, PSDM_RTM_EINVAL, "eikonal: source point outside the model (1-based indices)");
-    if (int rc = set_device(c)) return rc;
+    return 0;
+}
+
+// Sources solved per launch: as many as the workspace limit holds, at least one.
+int eikonal_batch(const psdm_rtm_ctx *c, int ny, int nx, int nsrc) {
+    const size_t per = (2 * (size_t)(nx + ny - 1) * nx + kTrash) * sizeof(float);
+    const size_t fit = c->eik_work_cap / per;
+    return (int)std::max<size_t>(1, std::min<size_t>(fit, (size_t)nsrc));
+}
+
+// Uploads the source points, computes 1/v of the model extended below and grows the
+// workspace to nb solves.  Returns the device source list and the workspace.
+int eikonal_setup(psdm_rtm_ctx *c, const float *d_vel, int ny, int nx, int nsrc, const int *src_x, const int *src_z,
+                  int nb, const int **sx, const int **sz, const float **va, const float **vm, float **work) {
     void *d_src, *d_v, *d_work;
     const size_t nd = (size_t)(nx + ny - 1) * nx;
     if (int rc = get_buf(c, "eik_src", 2 * (size_t)nsrc * sizeof(int), &d_src)) return rc;
     if (int rc = get_buf(c, "eik_vinv", 2 * nd * sizeof(float), &d_v)) return rc;
-    if (int rc = get_buf(c, "eik_work", (2 * nd + kTrash) * (size_t)nsrc * sizeof(float), &d_work)) return rc;
+    if (int rc = get_buf(c, "eik_work", (2 * nd + kTrash) * (size_t)nb * sizeof(float), &d_work)) return rc;
     std::vector<int> h(2 * (size_t)nsrc);
     std::memcpy(h.data(), src_x, nsrc * sizeof(int));
     std::memcpy(h.data() + nsrc, src_z, nsrc * sizeof(int));
     HIP_TRY(c, hipMemcpyAsync(d_src, h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    // the host list goes out of scope at return
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    *sx = (const int *)d_src;
+    *sz = *sx + nsrc;
+    float *a = (float *)d_v;
+    const size_t ncell = (size_t)nx * ny;
+    hipLaunchKernelGGL(reciprocal_kernel, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, c->stream, d_vel, ny, nx,
+                       a, a + nd);
+    HIP_TRY(c, hipGetLastError());
+    *va = a;
+    *vm = a + nd;
+    *work = (float *)d_work;
+    return 0;
+}
+
+// One launch: sources sx[0..n) into tt (ny, nx, n); e0 / e1 stamp its start / end (or null).
+void eikonal_launch(psdm_rtm_ctx *c, const float *va, const float *vm, int ny, int nx, float dx, float dy, int n,
+                    const int *sx, const int *sz, float *work, float *tt, hipEvent_t e0, hipEvent_t e1) {
     const int B = nx >= kMaxBlock ? kMaxBlock : (nx + 63) / 64 * 64;
     const int K = (nx + B - 1) / B;
     const unsigned lds = (4u * (unsigned)nx + (unsigned)(kTile * kTileLd)) * (unsigned)sizeof(float);
+    if (K == 1)
+        hipExtLaunchKernelGGL(eikonal_fsm_kernel<1, 4>, dim3(n), dim3(B), lds, c->stream, e0, e1, 0, va, vm, ny, nx, dx,
+                              dy, sx, sz, work, tt);
+    else if (K == 2)
+        hipExtLaunchKernelGGL(eikonal_fsm_kernel<2, 4>, dim3(n), dim3(B), lds, c->stream, e0, e1, 0, va, vm, ny, nx, dx,
+                              dy, sx, sz, work, tt);
+    else if (K <= 4)
+        hipExtLaunchKernelGGL(eikonal_fsm_kernel<4, 2>, dim3(n), dim3(B), lds, c->stream, e0, e1, 0, va, vm, ny, nx, dx,
+                              dy, sx, sz, work, tt);
+    else
+        hipExtLaunchKernelGGL(eikonal_fsm_kernel<8, 1>, dim3(n), dim3(B), lds, c->stream, e0, e1, 0, va, vm, ny, nx, dx,
+                              dy, sx, sz, work, tt);
+}
+
+int eikonal_events(psdm_rtm_ctx *c) {
+    for (int q = 0; q < 2; ++q)
+        if (!c->kir_ev[q]) HIP_TRY(c, hipEventCreate(&c->kir_ev[q]));
+    return 0;
+}
+
+// Adds the stamped launch's device time to the eikonal total; `call` counts one call.
+int eikonal_time(psdm_rtm_ctx *c, bool call) {
+    float ms;
+    HIP_TRY(c, hipEventElapsedTime(&ms, c->kir_ev[0], c->kir_ev[1]));
+    c->kir_ms[0] += ms;
+    if (call) c->kir_n[0]++;
+    return 0;
+}
+
+}  // namespace
+
+extern "C" int psdm_rtm_eikonal(psdm_rtm_ctx *c, const float *d_vel, int ny, int nx, float dx, float dy, int nsrc,
+                                const int *src_x, const int *src_z, float *d_tt) {
+    if (!c) return PSDM_RTM_EINVAL;
+    if (!d_vel || !d_tt) return fail(c, PSDM_RTM_EINVAL, "eikonal: null argument");
+    if (int rc = check_eikonal(c, ny, nx, dx, dy, nsrc, src_x, src_z)) return rc;
+    if (int rc = set_device(c)) return rc;
+    const int nb = eikonal_batch(c, ny, nx, nsrc);
+    const int *sx, *sz;
+    const float *va, *vm;
+    float *work;
+    if (int rc = eikonal_setup(c, d_vel, ny, nx, nsrc, src_x, src_z, nb, &sx, &sz, &va, &vm, &work)) return rc;
     const bool timed = c->timing != 0;
     if (timed)
-        for (int q = 0; q < 2; ++q)
-            if (!c->kir_ev[q]) HIP_TRY(c, hipEventCreate(&c->kir_ev[q]));
-    const int *sx = (const int *)d_src, *sz = sx + nsrc;
+        if (int rc = eikonal_events(c)) return rc;
     const size_t ncell = (size_t)nx * ny;
-    float *va = (float *)d_v, *vm = va + nd, *work = (float *)d_work;
-    hipLaunchKernelGGL(reciprocal_kernel, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, c->stream, d_vel, ny, nx,
-                       va, vm);
-    hipEvent_t e0 = timed ? c->kir_ev[0] : nullptr, e1 = timed ? c->kir_ev[1] : nullptr;
-    if (K == 1)
-        hipExtLaunchKernelGGL(eikonal_fsm_kernel<1, 4>, dim3(nsrc), dim3(B), lds, c->stream, e0, e1, 0, va, vm, ny, nx,
-                              dx, dy, sx, sz, work, d_tt);
-    else if (K == 2)
-        hipExtLaunchKernelGGL(eikonal_fsm_kernel<2, 4>, dim3(nsrc), dim3(B), lds, c->stream, e0, e1, 0, va, vm, ny, nx,
-                              dx, dy, sx, sz, work, d_tt);
-    else if (K <= 4)
-        hipExtLaunchKernelGGL(eikonal_fsm_kernel<4, 2>, dim3(nsrc), dim3(B), lds, c->stream, e0, e1, 0, va, vm, ny, nx,
-                              dx, dy, sx, sz, work, d_tt);
-    else
-        hipExtLaunchKernelGGL(eikonal_fsm_kernel<8, 1>, dim3(nsrc), dim3(B), lds, c->stream, e0, e1, 0, va, vm, ny, nx,
-                              dx, dy, sx, sz, work, d_tt);
-    HIP_TRY(c, hipGetLastError());
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
-    if (timed) {
-        float ms;
-        HIP_TRY(c, hipEventElapsedTime(&ms, e0, e1));
-        c->kir_ms[0] += ms;
-        c->kir_n[0]++;
+    // batches of nb sources reuse one workspace; stream order serialises them
+    for (int b0 = 0; b0 < nsrc; b0 += nb) {
+        const int n = std::min(nb, nsrc - b0);
+        eikonal_launch(c, va, vm, ny, nx, dx, dy, n, sx + b0, sz + b0, work, d_tt + (size_t)b0 * ncell,
+                       timed && b0 == 0 ? c->kir_ev[0] : nullptr, timed && b0 + n == nsrc ? c->kir_ev[1] : nullptr);
+        HIP_TRY(c, hipGetLastError());
     }
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (timed)
+        if (int rc = eikonal_time(c, true)) return rc;
     return 0;
 }
 
@@ -381,15 +438,48 @@ extern "C" int psdm_rtm_eikonal_host(psdm_rtm_ctx *c, const float *h_vel, int ny
                                      int nsrc, const int *src_x, const int *src_z, float *h_tt) {
     if (!c) return PSDM_RTM_EINVAL;
     if (!h_vel || !h_tt) return fail(c, PSDM_RTM_EINVAL, "eikonal_host: null argument");
-    if (ny < 1 || nx < 1 || nsrc < 1) return fail(c, PSDM_RTM_EINVAL, "eikonal_host: need ny, nx, nsrc >= 1");
+    if (int rc = check_eikonal(c, ny, nx, dx, dy, nsrc, src_x, src_z)) return rc;
     if (int rc = set_device(c)) return rc;
+    // the device holds one batch of travel times at a time; each is copied out before the next
+    const int nb = eikonal_batch(c, ny, nx, nsrc);
     void *dv, *dt;
     const size_t n = (size_t)nx * ny;
     if (int rc = get_buf(c, "eikh_vel", n * sizeof(float), &dv)) return rc;
-    if (int rc = get_buf(c, "eikh_tt", n * (size_t)nsrc * sizeof(float), &dt)) return rc;
+    if (int rc = get_buf(c, "eikh_tt", n * (size_t)nb * sizeof(float), &dt)) return rc;
     HIP_TRY(c, hipMemcpyAsync(dv, h_vel, n * sizeof(float), hipMemcpyHostToDevice, c->stream));
-    if (int rc = psdm_rtm_eikonal(c, (const float *)dv, ny, nx, dx, dy, nsrc, src_x, src_z, (float *)dt)) return rc;
-    HIP_TRY(c, hipMemcpy(h_tt, dt, n * (size_t)nsrc * sizeof(float), hipMemcpyDeviceToHost));
+    const int *sx, *sz;
+    const float *va, *vm;
+    float *work;
+    if (int rc = eikonal_setup(c, (const float *)dv, ny, nx, nsrc, src_x, src_z, nb, &sx, &sz, &va, &vm, &work))
+        return rc;
+    const bool timed = c->timing != 0;
+    if (timed)
+        if (int rc = eikonal_events(c)) return rc;
+    for (int b0 = 0; b0 < nsrc; b0 += nb) {
+        const int m = std::min(nb, nsrc - b0);
+        eikonal_launch(c, va, vm, ny, nx, dx, dy, m, sx + b0, sz + b0, work, (float *)dt, timed ? c->kir_ev[0] : nullptr,
+                       timed ? c->kir_ev[1] : nullptr);
+        HIP_TRY(c, hipGetLastError());
+        HIP_TRY(c, hipMemcpyAsync(h_tt + (size_t)b0 * n, dt, n * (size_t)m * sizeof(float), hipMemcpyDeviceToHost,
+                                  c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        if (timed)
+            if (int rc = eikonal_time(c, b0 == 0)) return rc;
+    }
+    return 0;
+}
+
+extern "C" int psdm_rtm_set_eikonal_workspace(psdm_rtm_ctx *c, size_t max_bytes) {
+    if (!c) return PSDM_RTM_EINVAL;
+    if (max_bytes == 0) return fail(c, PSDM_RTM_EINVAL, "eikonal workspace limit must be > 0");
+    c->eik_work_cap = max_bytes;
+    return 0;
+}
+
+extern "C" int psdm_rtm_eikonal_batch(psdm_rtm_ctx *c, int ny, int nx, int nsrc, int *nb) {
+    if (!c) return PSDM_RTM_EINVAL;
+    if (!nb || ny < 1 || nx < 1 || nsrc < 1) return fail(c, PSDM_RTM_EINVAL, "eikonal_batch: bad argument");
+    *nb = eikonal_batch(c, ny, nx, nsrc);
     return 0;
 }
 

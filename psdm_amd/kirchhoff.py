@@ -37,6 +37,18 @@ def travel_times(ctx: Context, d_vel, dd: float, src_x, src_z, out=None):
     return out
 
 
+def set_eikonal_workspace(ctx: Context, max_bytes: int):
+    """Limit of the eikonal workspace: the sources of travel_times run in batches that fit."""
+    ctx.check(ctx._L.psdm_rtm_set_eikonal_workspace(ctx.h, int(max_bytes)), "psdm_rtm_set_eikonal_workspace")
+
+
+def eikonal_batch(ctx: Context, nx: int, ny: int, nsrc: int) -> int:
+    """Sources travel_times solves per launch for an (nx, ny) window under the limit."""
+    nb = C.c_int(0)
+    ctx.check(ctx._L.psdm_rtm_eikonal_batch(ctx.h, ny, nx, nsrc, C.byref(nb)), "psdm_rtm_eikonal_batch")
+    return nb.value
+
+
 def image(ctx: Context, d_tts, d_ttr, d_data, nt_prop: int, dd: float, dt: float, vp0: float = 1500.0,
           theta_max: float = 60.0, theta_taper: float = 10.0, wmax: float = 1.0, out=None):
     """(nx, ny) image of one shot: d_tts (nx, ny), d_ttr (nrec, nx, ny), d_data (nrec, ld)."""

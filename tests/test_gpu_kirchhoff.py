@@ -62,6 +62,49 @@ def test_eikonal_shot_receivers_bit_exact(ctx):
         np.testing.assert_array_equal(T[q].view(np.uint32), K.eikonal(v, 10.0, 10.0, int(xs[q]), 2).view(np.uint32))
 
 
+def test_eikonal_batches_under_workspace_limit(ctx):
+    """Many sources on a wide, shallow window with a workspace limit of three solves: the
+    sources run in batches 3 + 3 + 3 + 2 that reuse one workspace, in the device form and in
+    the host form (one batch of travel times on the device at a time); every field bit for
+    bit the oracle's, both forms identical."""
+    import ctypes as C
+
+    import torch
+
+    from psdm_amd.kirchhoff import eikonal_batch, set_eikonal_workspace, travel_times
+
+    nx, ny, nsrc = 5102, 8, 11
+    per = 4 * (2 * (nx + ny - 1) * nx + 1024)
+    v = model(nx, ny, 21)
+    rng = np.random.default_rng(4)
+    xs = rng.integers(1, nx + 1, nsrc).astype(np.int32)
+    xs[0], xs[-1] = 1, nx
+    zs = rng.integers(1, ny + 1, nsrc).astype(np.int32)
+    try:
+        set_eikonal_workspace(ctx, 3 * per + per // 2)
+        assert eikonal_batch(ctx, nx, ny, nsrc) == 3
+        assert eikonal_batch(ctx, nx, ny, 2) == 2
+        say(f"[gpu] eikonal batches of 3: nx={nx} ny={ny} nsrc={nsrc}")
+        T = travel_times(ctx, torch.from_numpy(v).cuda(), 10.0, xs, zs).cpu().numpy()
+        H = np.zeros((nsrc, nx, ny), np.float32)
+        ctx.check(ctx._L.psdm_rtm_eikonal_host(ctx.h, v.ctypes.data_as(C.c_void_p), ny, nx, 10.0, 10.0, nsrc,
+                                               xs.ctypes.data_as(C.c_void_p), zs.ctypes.data_as(C.c_void_p),
+                                               H.ctypes.data_as(C.c_void_p)), "psdm_rtm_eikonal_host")
+    finally:
+        set_eikonal_workspace(ctx, 16 << 30)
+    np.testing.assert_array_equal(H.view(np.uint32), T.view(np.uint32))
+    for q in range(nsrc):
+        np.testing.assert_array_equal(T[q].view(np.uint32), K.eikonal(v, 10.0, 10.0, int(xs[q]), int(zs[q])).view(np.uint32),
+                                      err_msg=f"source {q}")
+
+
+def test_eikonal_workspace_default_fits_a_wide_window(ctx):
+    from psdm_amd.kirchhoff import eikonal_batch
+
+    assert eikonal_batch(ctx, 715, 601, 481) == 481
+    assert eikonal_batch(ctx, 5102, 801, 481) == 71
+
+
 @pytest.mark.parametrize("nx,ny,nrec,nt,dt", [(120, 90, 24, 1500, 0.002), (301, 201, 60, 4000, 0.001)])
 def test_image_vs_oracle(ctx, nx, ny, nrec, nt, dt):
     import torch
