@@ -241,11 +241,13 @@ int capture_graph(psdm_rtm_engine *e, int compact) {
 size_t snap_data_bytes(const psdm_rtm_engine *e, int nx_w, int nsnap) {
     return (((size_t)nsnap * e->lds * nx_w + 64) * e->esize + 255) & ~(size_t)255;
 }
+// A shot's snapshot footprint in the arena, in the arena's 256-byte granules (what
+// Arena::alloc takes for it), so that a budget of S footprints holds S shots.
 size_t snap_bytes_of(const psdm_rtm_engine *e, int nx_w, int nt) {
     const int nsnap = (nt - 1) / e->cfg.store_snap;
     size_t b = snap_data_bytes(e, nx_w, nsnap);
     if (e->cfg.snap_format != kSnapF32) b += (size_t)nsnap * cdiv(e->cfg.ny, kStreamRows) * nx_w * sizeof(float);
-    return b;
+    return (b + 255) & ~(size_t)255;
 }
 
 // Builds one slot's device view for `job` starting at global step G.
@@ -762,7 +764,13 @@ int psdm_rtm_engine_run(psdm_rtm_engine *e, int max_chunks, int *n_active, int *
         }
         bool any = false;
         for (auto &sl : e->slots) any = any || sl.busy;
-        if (!any) break;
+        if (!any) {
+            // retire releases a shot's snapshots at once, so the arena is empty here: a
+            // queued shot that was not admitted never will be
+            if (!e->pending.empty())
+                return efail(e, PSDM_RTM_EINVAL, "a queued shot does not fit the empty snapshot arena");
+            break;
+        }
         if (int rc = upload_slots(e)) return rc;
         // bounded look-ahead: wait for the chunk issued `inflight` chunks ago
         if (e->chunks >= inflight) {

@@ -189,6 +189,42 @@ def test_engine_hbm_budget_admission(ctx):
     assert sa["steps_issued"] >= 3 * 2 * (SHOTS[1][2] - 1)
 
 
+def test_one_slot_arena_holds_a_full_size_16_bit_shot(ctx):
+    """A one-slot engine sized for exactly one shot of the largest window and record, with
+    FP16 snapshots every 4th step: the footprint (snapshots plus one FP32 scale per
+    248-row segment of each snapshot column, 65 x 57 x 4 bytes here) is not a multiple of
+    256 bytes, and the arena hands out 256-byte granules.  The default budget counts
+    whole granules, so the shot is admitted and gives the image it gives in a two-slot
+    engine (regression: it was never admitted and the drain did not return)."""
+    import torch
+
+    from psdm_amd.engine import Engine
+
+    vel = model(4)
+    nx_w, nt = 57, 261
+    assert ((nt - 1) // 4 * nx_w * 4) % 256 != 0
+    si = shot_inputs(vel, (21, nx_w, nt, 20), 9)
+    dev = torch.device("cuda:0")
+    d_vel = torch.from_numpy(np.ascontiguousarray(vel.T)).to(dev)
+    d_w = torch.from_numpy(si["S"]).to(dev)
+    d_d = torch.from_numpy(np.ascontiguousarray(si["data"].T)).to(dev)
+    imgs = []
+    for nslots in (1, 2):
+        say(f"[gpu] one full-size fp16 shot, {nslots} slot(s)")
+        eng = Engine(ctx, d_vel, NY, NXP, nslots=nslots, max_nx=nx_w, max_nt=nt, dPML=DPML, fs=0, dmodel=DX, dt=DT,
+                     snap_format="f16", store_snap=4, kahan=True, chunk_steps=16)
+        d_img = torch.zeros((nx_w, NY), device=dev)
+        torch.cuda.synchronize()
+        eng.submit(si["x_i"], nx_w, nt, si["nxSou"], si["nySou"], si["nxRec"], si["nyRec"], d_w, d_d, d_img)
+        eng.drain()
+        st = eng.stats()
+        assert st["hbm_budget"] % 256 == 0
+        imgs.append(d_img.cpu().numpy().T.copy(order="F"))
+        eng.close()
+    assert np.linalg.norm(imgs[0]) > 0
+    np.testing.assert_array_equal(imgs[0], imgs[1])
+
+
 def test_engine_rejects_oversized_and_out_of_model(ctx):
     import torch
 

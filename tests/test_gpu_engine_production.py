@@ -174,9 +174,6 @@ def test_uneven_windows_mixed_precision_vs_fp64(ctx, fmt, bar):
     assert max(errs) <= bar
 
 
-@pytest.mark.skip(reason="its one GPU run (FP32 every step, the 801 x 5102 window alone and with two 715-column "
-                         "windows, chunk 16) did not finish within 180 s and was stopped; no cause has been found by "
-                         "reading the engine, so the case stays off the GPU")
 def test_was_window_through_engine_alone_and_in_mix(ctx):
     """An 801 x 5102 WAS window (OBS as source, 1000 shots as receivers, Ricker_was delay
     1.5/f0): the FP32 image equals the oracle's; in FP32 and in FP16 every 4th step with
@@ -200,7 +197,9 @@ def test_was_window_through_engine_alone_and_in_mix(ctx):
     mcs = [shot_job(mw, _mcs_shot(q, x, DX, mw.nx, 480, 12.5, 150.0, 6.0, 8.0, 500.0), n, 20 + q)
            for q, (x, n) in enumerate(((9000.0, 49), (30000.0, 57)))]
     for kw in (dict(), dict(snap_format="f16", store_snap=4, kahan=True)):
+        say(f"  {kw or 'fp32'}: alone (1 slot)")
         alone, _, _ = run_engine(ctx, mw, [was], 1, chunk_steps=16, **kw)
+        say(f"  {kw or 'fp32'}: in a mix (3 slots)")
         mixed, _, _ = run_engine(ctx, mw, [mcs[0], was, mcs[1]], 3, chunk_steps=16, **kw)
         say(f"  {kw or 'fp32'}: |image| {np.linalg.norm(alone[0]):.3e}")
         np.testing.assert_array_equal(alone[0], mixed[1])
