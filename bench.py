@@ -181,9 +181,10 @@ def model_gathers(ctx, torch, dev, W, geos_model, d_true, d_wat, d_src, nshots):
     return obs
 
 
-def run_plain(ctx, torch, dev, W, geos, d_mig, d_src, obs, steps, warmup, timing_every):
-    """One shot per call: forward (snapshots to HBM) + backward + FP64 stack per shot."""
-    from psdm_amd.solvers import backward_device, forward_device
+def run_plain(ctx, torch, dev, W, geos, d_mig, d_src, obs, steps, warmup, timing_every, tune=True):
+    """One shot per call: forward (snapshots to HBM) + backward + FP64 stack per shot.
+    With `tune`, the launch shape is first chosen on a 300-step probe of shot 0's window."""
+    from psdm_amd.solvers import DeviceShot, backward_device, forward_device
 
     m, shots, nt = W["model"], W["shots"], W["nt"]
     ny = m.nz
@@ -194,6 +195,27 @@ def run_plain(ctx, torch, dev, W, geos, d_mig, d_src, obs, steps, warmup, timing
     fold = torch.zeros((m.nx, ny), dtype=torch.int32, device=dev)
     d_rec = torch.empty((nrec, nt), device=dev)
     torch.cuda.synchronize()
+    tuned = None
+    if tune:
+        from psdm_amd.autotune import tune_plain
+
+        npr, sh = 300, shots[0]
+        pgeo = DeviceShot(ny, sh.nx_w, npr, W["dPML"], W["fs"], m.dx, W["dt"], [sh.nxSou - sh.x_i + 1], [sh.nySou],
+                          sh.nxRec - sh.x_i + 1, sh.nyRec)
+        win = d_mig[sh.x_i - 1:sh.x_f]
+        p_src = d_src[:, :npr].contiguous()
+        p_rec = torch.empty((nrec, npr), device=dev)
+        p_obs = rev[0][:, :npr].contiguous()
+        torch.cuda.synchronize()
+
+        def probe():
+            forward_device(ctx, pgeo, win, p_src, p_rec, key=-22)
+            backward_device(ctx, pgeo, win, p_obs, d_kernel, key=-22)
+
+        best, trials = tune_plain(ctx, probe, log=log)
+        d_kernel.zero_()
+        torch.cuda.synchronize()
+        tuned = {"best": best, "trials": trials}
 
     def migrate(k):
         sh = shots[k % len(shots)]
@@ -221,10 +243,10 @@ def run_plain(ctx, torch, dev, W, geos, d_mig, d_src, obs, steps, warmup, timing
     n_ev, mean_ms = ctx.step_stats()
     ctx.set_timing(0)
     return dict(elapsed=elapsed, n_ev=n_ev, mean_ms=mean_ms, shots=[(k % len(shots)) for k in range(warmup, warmup + steps)],
-                psdm_sum=psdm_sum, fold=fold)
+                psdm_sum=psdm_sum, fold=fold, launch=tuned)
 
 
-def run_plain_wide(ctx, torch, dev, nt, timing_every):
+def run_plain_wide(ctx, torch, dev, nt, timing_every, tune=True):
     """The plain form on the widest window of the served stream: one WAS gather (801 x 5102
     window of the 801 x 6001 @ 10 m model, 1000 receivers, dt 1 ms, FP32 snapshots every
     step) over its first nt steps, forward + backward.  One step of this window is large
@@ -247,6 +269,25 @@ def run_plain_wide(ctx, torch, dev, nt, timing_every):
     d_rec = torch.empty((sh.nxRec.size, nt), device=dev)
     d_img = torch.zeros((nxw, ny), device=dev)
     torch.cuda.synchronize()
+    tuned = None
+    if tune:
+        from psdm_amd.autotune import tune_plain
+
+        npr = 200
+        pgeo = DeviceShot(ny, nxw, npr, dPML, 0, m.dx, dt, [sh.nxSou - sh.x_i + 1], [sh.nySou], sh.nxRec - sh.x_i + 1,
+                          sh.nyRec)
+        p_src = d_src[:, :npr].contiguous()
+        p_rec = torch.empty((sh.nxRec.size, npr), device=dev)
+        p_obs = d_obs[:, :npr].contiguous()
+        torch.cuda.synchronize()
+
+        def probe():
+            forward_device(ctx, pgeo, d_mig, p_src, p_rec, key=-12)
+            backward_device(ctx, pgeo, d_mig, p_obs, d_img, key=-12)
+
+        best, trials = tune_plain(ctx, probe, log=log)
+        d_img.zero_()
+        tuned = {"best": best, "trials": trials}
     forward_device(ctx, geo, d_mig, d_src, d_rec, key=-11)  # warm-up (allocations)
     backward_device(ctx, geo, d_mig, d_obs, d_img, key=-11)
     ctx.sync()
@@ -261,6 +302,7 @@ def run_plain_wide(ctx, torch, dev, nt, timing_every):
     b = 0.5 * (step_bytes(ny, nxw, dPML) + step_bytes(ny, nxw, dPML, backward=True))
     ach = b / (mean_ms * 1e-3) / 1e9
     return {"workload": f"WAS gather, {ny}x{nxw} window, first {nt} of 12000 steps (dt 1 ms), FP32 snapshots",
+            "launch": tuned,
             "gpt_updates_per_s": round(updates_per_shot(ny, nxw, nt, dPML) / el / 1e9, 3),
             "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": round(ach / HBM_PEAK_GBS, 4), "kernel": "rtm_step_kernel",
@@ -1001,7 +1043,7 @@ def main():
     ap.add_argument("--no-wide", action="store_true", help="skip the plain form on the 801 x 5102 window")
     ap.add_argument("--wide-steps", type=int, default=1500, help="plain form on the 801 x 5102 window: steps timed")
     ap.add_argument("--no-accuracy", action="store_true")
-    ap.add_argument("--no-tune", action="store_true", help="batched form with the default launch shape")
+    ap.add_argument("--no-tune", action="store_true", help="plain and batched forms with the default launch shapes")
     ap.add_argument("--engine-cfg", default=None,
                     help="batched form: launch shape as key=value pairs, e.g. nslots=20,tile_x=16,stream_w=4,groups=2 "
                          "(keys nslots, tile_x, stream_w, grid, groups, steps_per_launch), used instead of the "
@@ -1100,7 +1142,8 @@ def main():
 
     plain = batched = None
     if "plain" in forms:
-        P = run_plain(ctx, torch, dev, W, geos, d_mig, d_src, obs, args.steps, args.warmup, args.timing_every)
+        P = run_plain(ctx, torch, dev, W, geos, d_mig, d_src, obs, args.steps, args.warmup, args.timing_every,
+                      tune=not args.no_tune)
         ach = step_b / (P["mean_ms"] * 1e-3) / 1e9 if P["mean_ms"] > 0 else None
         plain = {"shots_per_s": round(args.steps / P["elapsed"], 4),
                  "gpt_updates_per_s": round(args.steps * upd_shot / P["elapsed"] / 1e9, 3),
@@ -1108,10 +1151,12 @@ def main():
                  "roofline": {"bound": "hbm", "achieved": round(ach, 1) if ach else None, "peak": HBM_PEAK_GBS,
                               "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4) if ach else None,
                               "kernel": "rtm_step_kernel", "mean_kernel_ms": round(P["mean_ms"], 5),
-                              "sampled_launches": P["n_ev"], "algorithmic_bytes_per_launch": round(step_b)}}
+                              "sampled_launches": P["n_ev"], "algorithmic_bytes_per_launch": round(step_b)},
+                 "launch": P["launch"]}
         log(f"[bench] plain {plain['shots_per_s']} shots/s")
         if not args.no_wide:
-            plain["wide_window"] = run_plain_wide(ctx, torch, dev, args.wide_steps, args.timing_every)
+            plain["wide_window"] = run_plain_wide(ctx, torch, dev, args.wide_steps, args.timing_every,
+                                                  tune=not args.no_tune)
             log(f"[bench] plain, wide window: roofline {plain['wide_window']['roofline']['frac']}")
 
     if "batched" in forms:

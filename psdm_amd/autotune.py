@@ -130,3 +130,29 @@ def tune(ctx, d_vel, ny, nx_prop, jobs, max_nx, nt, dPML, fs, dx, dt, snap_forma
     if best is None:
         raise RuntimeError("autotune: no slot count could be run")
     return best, trials
+
+
+def tune_plain(ctx, probe, tiles=(8, 16, 32), widths=(1, 2, 4, 8), timing_every=2, log=None):
+    """Launch shape of the one-shot solver (CPML tile width, interior columns per stream
+    item) for one window: `probe()` runs a short shot through the device entry points
+    (forward + backward on that window); each shape is scored by the mean duration of its
+    step kernel (start/stop stamps on sampled dispatches).  Leaves the best shape set on
+    the context and returns (best, trials)."""
+    trials = []
+    for tx, sw in itertools.product(tiles, widths):
+        ctx.set_launch(tile_x=tx)
+        ctx.set_stream_width(sw)
+        probe()  # allocations and first-touch of this shape's buffers
+        ctx.sync()
+        ctx.set_timing(timing_every)
+        probe()
+        ctx.sync()
+        n, ms = ctx.step_stats()
+        ctx.set_timing(0)
+        trials.append(dict(tile_x=tx, stream_w=sw, kernel_ms=round(ms, 5), sampled=n))
+        if log:
+            log(f"[tune plain] tile_x={tx} stream_w={sw}: {ms * 1e3:.2f} us per step kernel")
+    best = min(trials, key=lambda t: t["kernel_ms"])
+    ctx.set_launch(tile_x=best["tile_x"])
+    ctx.set_stream_width(best["stream_w"])
+    return best, trials
