@@ -181,9 +181,17 @@ def model_gathers(ctx, torch, dev, W, geos_model, d_true, d_wat, d_src, nshots):
     return obs
 
 
-def run_plain(ctx, torch, dev, W, geos, d_mig, d_src, obs, steps, warmup, timing_every, tune=True):
+def set_plain_shape(ctx, cfg):
+    """Launch shape of the one-shot solver from {"tile_x": .., "stream_w": ..}."""
+    ctx.set_launch(tile_x=int(cfg.get("tile_x", 8)))
+    ctx.set_stream_width(int(cfg.get("stream_w", 2)))
+    return {"tile_x": int(cfg.get("tile_x", 8)), "stream_w": int(cfg.get("stream_w", 2)), "given": True}
+
+
+def run_plain(ctx, torch, dev, W, geos, d_mig, d_src, obs, steps, warmup, timing_every, tune=True, shape=None):
     """One shot per call: forward (snapshots to HBM) + backward + FP64 stack per shot.
-    With `tune`, the launch shape is first chosen on a 300-step probe of shot 0's window."""
+    With `shape` ({"tile_x", "stream_w"}) that launch shape is used; else with `tune` it is
+    first chosen on a 300-step probe of shot 0's window."""
     from psdm_amd.solvers import DeviceShot, backward_device, forward_device
 
     m, shots, nt = W["model"], W["shots"], W["nt"]
@@ -196,7 +204,9 @@ def run_plain(ctx, torch, dev, W, geos, d_mig, d_src, obs, steps, warmup, timing
     d_rec = torch.empty((nrec, nt), device=dev)
     torch.cuda.synchronize()
     tuned = None
-    if tune:
+    if shape:
+        tuned = {"best": set_plain_shape(ctx, shape)}
+    elif tune:
         from psdm_amd.autotune import tune_plain
 
         npr, sh = 300, shots[0]
@@ -246,7 +256,7 @@ def run_plain(ctx, torch, dev, W, geos, d_mig, d_src, obs, steps, warmup, timing
                 psdm_sum=psdm_sum, fold=fold, launch=tuned)
 
 
-def run_plain_wide(ctx, torch, dev, nt, timing_every, tune=True):
+def run_plain_wide(ctx, torch, dev, nt, timing_every, tune=True, shape=None):
     """The plain form on the widest window of the served stream: one WAS gather (801 x 5102
     window of the 801 x 6001 @ 10 m model, 1000 receivers, dt 1 ms, FP32 snapshots every
     step) over its first nt steps, forward + backward.  One step of this window is large
@@ -270,7 +280,9 @@ def run_plain_wide(ctx, torch, dev, nt, timing_every, tune=True):
     d_img = torch.zeros((nxw, ny), device=dev)
     torch.cuda.synchronize()
     tuned = None
-    if tune:
+    if shape:
+        tuned = {"best": set_plain_shape(ctx, shape)}
+    elif tune:
         from psdm_amd.autotune import tune_plain
 
         npr = 200
@@ -1007,6 +1019,13 @@ def bench_served(args):
     ctx.close()
 
 
+def kv_cfg(text):
+    """{"key": int} from "key=value,key=value" (None for None)."""
+    if not text:
+        return None
+    return {k: int(v) for k, v in (kv.split("=") for kv in text.split(","))}
+
+
 def bench_copy(args):
     """Calibration of the HBM traffic counters (rocprofv3 FETCH_SIZE / WRITE_SIZE): four
     device-to-device copies of 256 MiB, whose bytes read and written are known."""
@@ -1044,6 +1063,11 @@ def main():
     ap.add_argument("--wide-steps", type=int, default=1500, help="plain form on the 801 x 5102 window: steps timed")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="plain and batched forms with the default launch shapes")
+    ap.add_argument("--plain-cfg", default=None,
+                    help="plain form on the benchmark's window: launch shape tile_x=..,stream_w=.. instead of the "
+                         "per-window tuning (to profile a tuned run's shape)")
+    ap.add_argument("--wide-cfg", default=None,
+                    help="plain form on the 801 x 5102 window: launch shape tile_x=..,stream_w=.. instead of tuning")
     ap.add_argument("--engine-cfg", default=None,
                     help="batched form: launch shape as key=value pairs, e.g. nslots=20,tile_x=16,stream_w=4,groups=2 "
                          "(keys nslots, tile_x, stream_w, grid, groups, steps_per_launch), used instead of the "
@@ -1143,7 +1167,7 @@ def main():
     plain = batched = None
     if "plain" in forms:
         P = run_plain(ctx, torch, dev, W, geos, d_mig, d_src, obs, args.steps, args.warmup, args.timing_every,
-                      tune=not args.no_tune)
+                      tune=not args.no_tune, shape=kv_cfg(args.plain_cfg))
         ach = step_b / (P["mean_ms"] * 1e-3) / 1e9 if P["mean_ms"] > 0 else None
         plain = {"shots_per_s": round(args.steps / P["elapsed"], 4),
                  "gpt_updates_per_s": round(args.steps * upd_shot / P["elapsed"] / 1e9, 3),
@@ -1156,7 +1180,7 @@ def main():
         log(f"[bench] plain {plain['shots_per_s']} shots/s")
         if not args.no_wide:
             plain["wide_window"] = run_plain_wide(ctx, torch, dev, args.wide_steps, args.timing_every,
-                                                  tune=not args.no_tune)
+                                                  tune=not args.no_tune, shape=kv_cfg(args.wide_cfg))
             log(f"[bench] plain, wide window: roofline {plain['wide_window']['roofline']['frac']}")
 
     if "batched" in forms:
