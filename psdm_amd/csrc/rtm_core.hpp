@@ -160,7 +160,8 @@ struct StepArgs {
     const float *fA, *fB;  // 1D damping tables (dn), index = distance into the CPML - 1
     const int *ev_ptr;     // per stream item [ev_ptr[u], ev_ptr[u+1]) injection events, by column
     const int *ev_cell;    // event cell j + i*nyt
-    const int *ev_inj;     // column of inj_val
+    const int *ev_inj;     // column of inj_val (the one-step lists are built with ev_inj[e] == e,
+                           // which stream_item relies on: it reads event e's value in column e)
     const float *inj_val;  // (nt-1) x ninj, row k-2 holds the values injected at step k
     int ninj;
     void *snaps;           // forward: snapshot base (format snap_fmt); backward: read

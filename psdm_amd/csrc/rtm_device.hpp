@@ -44,6 +44,13 @@ __device__ __forceinline__ gptr<T> gp(T *p) {
     return (gptr<T>)p;
 }
 
+// Wave-uniform read of memory that no launch in flight writes, through the scalar data
+// cache (s_load): its latency and counter are separate from the vector loads.
+template <class T>
+__device__ __forceinline__ T sload(const T *p) {
+    return *(const T __attribute__((address_space(4))) *)p;
+}
+
 // Native 4 x float / 2 x uint vectors (16- and 8-byte lane accesses).
 typedef float v4f __attribute__((ext_vector_type(4)));
 typedef unsigned v2u __attribute__((ext_vector_type(2)));

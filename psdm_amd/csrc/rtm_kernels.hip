@@ -67,18 +67,20 @@ __device__ __forceinline__ void stream_item(const StepArgs &a, int u, v4f *stage
     const size_t ld = a.ld;
     const v4f z4 = v4f{0.f, 0.f, 0.f, 0.f};
 
-    // injection events of this item, ordered by column then row: cell = iw << 16 | jw.
-    // The first 64 are loaded one per lane (cell and value) together with the window, so
-    // the column loop reads them with readlane instead of dependent scalar loads.
-    int e = gp(a.ev_ptr)[u];
-    const int ev_e = gp(a.ev_ptr)[u + 1];
+    // injection events of this item, ordered by column then row: cell = iw << 16 | jw; the
+    // one-step lists keep event e's values in column e of inj_val.  The item's range comes
+    // through the scalar cache; the first 64 events are loaded one per lane (cell and
+    // value, independent loads) together with the window, so the column loop reads them
+    // with readlane instead of dependent loads.
+    int e = sload(a.ev_ptr + u);
+    const int ev_e = sload(a.ev_ptr + u + 1);
     gptr<const float> inj = gp(a.inj_val) + (size_t)(a.k - 2) * a.ninj;
     const int e_first = e, e_reg = ev_e < e + 64 ? ev_e : e + 64;
     int ev_c = 0;
     float ev_v = 0.f;
     if (e + lane < e_reg) {
         ev_c = gp(a.ev_cell)[e + lane];
-        ev_v = inj[gp(a.ev_inj)[e + lane]];
+        ev_v = inj[e + lane];
     }
 
     const bool imaging = a.mode == 1 && a.snap_slot >= 0;
@@ -163,7 +165,7 @@ __device__ __forceinline__ void stream_item(const StepArgs &a, int u, v4f *stage
             } else {  // events past the first 64 of the item
                 cell = gp(a.ev_cell)[e];
                 if ((cell >> 16) != iw) break;
-                val = inj[gp(a.ev_inj)[e]];
+                val = inj[e];
             }
             const int r = (cell & 0xffff) - (o - 4);
             if (lane == (r >> 2)) {
