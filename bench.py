@@ -1098,6 +1098,8 @@ def main():
               f"(--gpus {args.gpus} is not supported; shots shard across GPUs with no per-step "
               f"communication)", file=sys.stderr)
         sys.exit(2)
+    if args.cpu_procs <= 0:  # every core this process may run on (at most 16), for every workload
+        args.cpu_procs = host_cores()[0]
     if args.workload in ("uneven", "mixed"):
         return bench_uneven(args, mixed=args.workload == "mixed")
     if args.workload == "served":
@@ -1123,7 +1125,7 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        procs = args.cpu_procs or host_cores()[0]
+        procs = args.cpu_procs
         log(f"[bench] cpu baseline: {procs} procs x {args.cpu_steps} steps")
         cpu = cpu_leg(ny, nxw, dPML, m.dx, dt, procs, args.cpu_steps, nt)
         log(f"[bench] cpu baseline {cpu['value']} Gupd/s ({cpu['kind']})")

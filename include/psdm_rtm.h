@@ -366,6 +366,12 @@ int psdm_rtm_eikonal_host(psdm_rtm_ctx *ctx, const float *h_vel, int ny, int nx,
 /* No counterpart in the reference (its solver keeps one source's arrays at a time): the
  * limit on the eikonal workspace of this ctx, in bytes (> 0; default 16 GiB). */
 int psdm_rtm_set_eikonal_workspace(psdm_rtm_ctx *ctx, size_t max_bytes);
+/* The eikonal's quotients by dx*dx, dy*dy and 2*aa (local_solver, solve_eikonal_FSM.f90:63-79)
+ * are correctly rounded either way; bit i of *fast_mask (i = 0, 1, 2 in that order) is set when
+ * that divisor uses a product and an FMA in place of the IEEE division, having matched it
+ * for every significand on the device (checked on first use of a (dx, dy) pair).  No
+ * counterpart in the reference. */
+int psdm_rtm_eikonal_division(psdm_rtm_ctx *ctx, float dx, float dy, int *fast_mask);
 /* *nb = the sources psdm_rtm_eikonal solves per launch for this shape under the ctx's
  * workspace limit (1 <= *nb <= nsrc). */
 int psdm_rtm_eikonal_batch(psdm_rtm_ctx *ctx, int ny, int nx, int nsrc, int *nb);

@@ -60,6 +60,7 @@ SIGNATURES = {
     "psdm_rtm_eikonal_host": ([_P, _P, _I, _I, _F, _F, _I, _P, _P, _P], _I),
     "psdm_rtm_set_eikonal_workspace": ([_P, C.c_size_t], _I),
     "psdm_rtm_eikonal_batch": ([_P, _I, _I, _I, C.POINTER(_I)], _I),
+    "psdm_rtm_eikonal_division": ([_P, _F, _F, C.POINTER(_I)], _I),
     "psdm_rtm_kirchhoff_image": ([_P, _P, _P, _P, _I, _I, _I, _I, _I, _F, _F, _F, _F, _F, _F, _P], _I),
     "psdm_rtm_kirchhoff_timing": ([_P, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(_LL),
                                    C.POINTER(_LL)], _I),

@@ -41,20 +41,79 @@ constexpr int kMaxBlock = 1024;  // eikonal workgroup size limit (its __launch_b
 constexpr int kTrash = 1024;     // per-solve sink for the stores of lanes without a cell, one word per thread
 static_assert(kTrash >= kMaxBlock, "every thread of a workgroup needs its own trash word");
 
+// Division by a divisor fixed for the launch (local_solver's dx*dx, dy*dy and 2*aa).
+// With zh = RN(1/y) and zl = RN(1/y - zh), RN(x*zh + RN(x*zl)) -- one product and one
+// FMA -- is the correctly rounded x / y for every x when a condition on y holds
+// (Brisebarre, Muller and Raina, "Accelerating correctly rounded floating-point division
+// when the divisor is known in advance", IEEE Trans. Computers 53(8), 2004).  The host
+// establishes it per divisor by comparing both forms on the device for every significand
+// (x in [1, 2), eikdiv_check_kernel); it then holds for every |x| in [2^-60, 2^64] and for
+// 0, because scaling x by a power of two scales every rounding of the form alike while no
+// product underflows or overflows (the host enables it only for 2^-30 <= y <= 2^30 and
+// zl = 0 or |zl| >= 2^-60).  local_solver's numerators are a, b, a*a, b*b and
+// -bb + sqrt(disc) >= 2*(a/(dx*dx) + b/(dy*dy)), with 0 <= a, b <= 1e9: all of them are 0 or
+// lie in that range when a and b are each 0 or >= 2^-30, which is checked once per cell;
+// any other cell, and a divisor that fails the check, take the IEEE division.
+struct EikConsts {
+    float dx, dy, aa;     // aa = 1/(dx*dx) + 1/(dy*dy) in the reference's order
+    float zh[3], zl[3];   // per divisor dx*dx, dy*dy, 2*aa
+    int fast;             // bit i: divisor i takes the two-operation form
+};
+
+template <bool kAllow>
+__device__ __forceinline__ float divc(float x, float y, float zh, float zl, int fast, int i) {
+    if (kAllow && ((fast >> i) & 1)) return __builtin_fmaf(x, zh, x * zl);
+    return x / y;
+}
+
 // local_solver (solve_eikonal_FSM.f90:63-79), the reference's FP32 operations in order
-__device__ __forceinline__ float local_solver(float a, float b, float dx, float dy, float v_inv) {
-    const float aa = 1.0f / (dx * dx) + 1.0f / (dy * dy);
-    const float bb = -2.0f * (a / (dx * dx) + b / (dy * dy));
-    const float cc = (a * a) / (dx * dx) + (b * b) / (dy * dy) - v_inv * v_inv;
+// (every quotient correctly rounded, by divc)
+template <bool kAllow>
+__device__ __forceinline__ float local_solver_q(float a, float b, const EikConsts &k, float v_inv) {
+    const float d0 = k.dx * k.dx, d1 = k.dy * k.dy, aa = k.aa;
+    const float bb = -2.0f * (divc<kAllow>(a, d0, k.zh[0], k.zl[0], k.fast, 0) +
+                              divc<kAllow>(b, d1, k.zh[1], k.zl[1], k.fast, 1));
+    const float cc = divc<kAllow>(a * a, d0, k.zh[0], k.zl[0], k.fast, 0) +
+                     divc<kAllow>(b * b, d1, k.zh[1], k.zl[1], k.fast, 1) - v_inv * v_inv;
     const float disc = bb * bb - 4.0f * aa * cc;
+    const float dmin = fminf(k.dx, k.dy);
     float t;
     if (disc >= 0.0f) {
-        t = (-bb + sqrtf(disc)) / (2.0f * aa);
-        if (t < fmaxf(a, b)) t = fmaxf(a, b) + v_inv * fminf(dx, dy);
+        t = divc<kAllow>(-bb + sqrtf(disc), 2.0f * aa, k.zh[2], k.zl[2], k.fast, 2);
+        if (t < fmaxf(a, b)) t = fmaxf(a, b) + v_inv * dmin;
     } else {
-        t = fminf(a, b) + v_inv * fminf(dx, dy);
+        t = fminf(a, b) + v_inv * dmin;
     }
     return t;
+}
+
+__device__ __forceinline__ float local_solver(float a, float b, const EikConsts &k, float v_inv) {
+    const bool ok = (a == 0.0f || a >= 0x1p-30f) && (b == 0.0f || b >= 0x1p-30f);
+    return ok ? local_solver_q<true>(a, b, k, v_inv) : local_solver_q<false>(a, b, k, v_inv);
+}
+
+// Per divisor of `k`: whether RN(x*zh + RN(x*zl)) equals x / y for every x in [1, 2).
+// Thread t of block b tests the 32 significands from 32 * (b * blockDim + t); bad[3b + i]
+// = 1 when block b found a mismatch for divisor i.
+constexpr int kDivCheckPer = 32, kDivCheckBlock = 256;
+constexpr int kDivCheckBlocks = (1 << 23) / (kDivCheckPer * kDivCheckBlock);
+__global__ __launch_bounds__(kDivCheckBlock) void eikdiv_check_kernel(EikConsts k, int *bad) {
+    const unsigned base = (blockIdx.x * blockDim.x + threadIdx.x) * (unsigned)kDivCheckPer;
+    int miss[3] = {0, 0, 0};
+    for (int q = 0; q < kDivCheckPer; ++q) {
+        const float x = __uint_as_float(0x3f800000u | (base + (unsigned)q));
+        const float ys[3] = {k.dx * k.dx, k.dy * k.dy, 2.0f * k.aa};
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const float f = __builtin_fmaf(x, k.zh[i], x * k.zl[i]);
+            if (__float_as_uint(f) != __float_as_uint(x / ys[i])) miss[i] = 1;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const int m = __syncthreads_or(miss[i]);
+        if (threadIdx.x == 0) bad[blockIdx.x * 3 + i] = m;
+    }
 }
 
 // Workgroup barrier for the wavefront steps: LDS writes complete, then s_barrier.  Global
@@ -136,7 +195,7 @@ __device__ __forceinline__ unsigned ring_addr(bool fwd, int D, int c, int nx, in
 // the last sweep M -> the model's column-major output.
 template <int K, int P>
 __global__ __launch_bounds__(kMaxBlock) void eikonal_fsm_kernel(const float *__restrict__ va, const float *__restrict__ vm,
-                                                           int ny, int nx, float dx, float dy,
+                                                           int ny, int nx, EikConsts kc,
                                                            const int *__restrict__ src_x,
                                                            const int *__restrict__ src_z, float *__restrict__ work,
                                                            float *__restrict__ tt) {
@@ -214,7 +273,7 @@ __global__ __launch_bounds__(kMaxBlock) void eikonal_fsm_kernel(const float *__r
                         // a = min(T(j,i-1), T(j,i+1)), b = min(T(j-1,i), T(j+1,i)): min is
                         // symmetric, so the sweep-order neighbours give the same values
                         const float a = fminf(cp, cn), b = fminf(rp, rn);
-                        const float t = local_solver(a, b, dx, dy, vq[k][u]);
+                        const float t = local_solver(a, b, kc, vq[k][u]);
                         if (t < self) tnew = t;
                     }
                     prv[k] = tnew;
@@ -369,24 +428,71 @@ int eikonal_setup(psdm_rtm_ctx *c, const float *d_vel, int ny, int nx, int nsrc,
     return 0;
 }
 
+// local_solver's constants for the grid spacings: aa = 1/(dx*dx) + 1/(dy*dy) in the
+// reference's order, and per divisor (dx*dx, dy*dy, 2*aa) zh = RN(1/y), zl ~ 1/y - zh and
+// whether divc may use them, established once per (dx, dy) by eikdiv_check_kernel and
+// kept by the ctx.
+int eikonal_consts(psdm_rtm_ctx *c, float dx, float dy, EikConsts *k, int *fast_mask) {
+    const float d0 = dx * dx, d1 = dy * dy;
+    const float aa = 1.0f / d0 + 1.0f / d1;
+    k->dx = dx;
+    k->dy = dy;
+    k->aa = aa;
+    k->fast = 0;
+    const float ys[3] = {d0, d1, 2.0f * aa};
+    int eligible = 0;
+    for (int i = 0; i < 3; ++i) {
+        const float zh = 1.0f / ys[i];
+        const float zl = (float)(1.0 / (double)ys[i] - (double)zh);
+        k->zh[i] = zh;
+        k->zl[i] = zl;
+        if (ys[i] >= 0x1p-30f && ys[i] <= 0x1p30f && (zl == 0.0f || std::fabs(zl) >= 0x1p-60f)) eligible |= 1 << i;
+    }
+    unsigned bx, by;
+    std::memcpy(&bx, &dx, 4);
+    std::memcpy(&by, &dy, 4);
+    const unsigned long long key = (unsigned long long)bx << 32 | by;
+    auto it = c->eik_fast.find(key);
+    int mask;
+    if (it != c->eik_fast.end()) {
+        mask = it->second;
+    } else {
+        void *v;
+        if (int rc = get_buf(c, "eik_divchk", (size_t)kDivCheckBlocks * 3 * sizeof(int), &v)) return rc;
+        hipLaunchKernelGGL(eikdiv_check_kernel, dim3(kDivCheckBlocks), dim3(kDivCheckBlock), 0, c->stream, *k,
+                           (int *)v);
+        HIP_TRY(c, hipGetLastError());
+        std::vector<int> bad((size_t)kDivCheckBlocks * 3);
+        HIP_TRY(c, hipMemcpyAsync(bad.data(), v, bad.size() * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        mask = eligible;
+        for (size_t q = 0; q < bad.size(); ++q)
+            if (bad[q]) mask &= ~(1 << (int)(q % 3));
+        c->eik_fast[key] = mask;
+    }
+    k->fast = mask;
+    if (fast_mask) *fast_mask = mask;
+    return 0;
+}
+
 // One launch: sources sx[0..n) into tt (ny, nx, n); e0 / e1 stamp its start / end (or null).
-void eikonal_launch(psdm_rtm_ctx *c, const float *va, const float *vm, int ny, int nx, float dx, float dy, int n,
+void eikonal_launch(psdm_rtm_ctx *c, const float *va, const float *vm, int ny, int nx, const EikConsts &kc, int n,
                     const int *sx, const int *sz, float *work, float *tt, hipEvent_t e0, hipEvent_t e1) {
     const int B = nx >= kMaxBlock ? kMaxBlock : (nx + 63) / 64 * 64;
     const int K = (nx + B - 1) / B;
     const unsigned lds = (4u * (unsigned)nx + (unsigned)(kTile * kTileLd)) * (unsigned)sizeof(float);
     if (K == 1)
-        hipExtLaunchKernelGGL(eikonal_fsm_kernel<1, 4>, dim3(n), dim3(B), lds, c->stream, e0, e1, 0, va, vm, ny, nx, dx,
-                              dy, sx, sz, work, tt);
+        hipExtLaunchKernelGGL(eikonal_fsm_kernel<1, 4>, dim3(n), dim3(B), lds, c->stream, e0, e1, 0, va, vm, ny, nx, kc,
+                              sx, sz, work, tt);
     else if (K == 2)
-        hipExtLaunchKernelGGL(eikonal_fsm_kernel<2, 4>, dim3(n), dim3(B), lds, c->stream, e0, e1, 0, va, vm, ny, nx, dx,
-                              dy, sx, sz, work, tt);
+        hipExtLaunchKernelGGL(eikonal_fsm_kernel<2, 4>, dim3(n), dim3(B), lds, c->stream, e0, e1, 0, va, vm, ny, nx, kc,
+                              sx, sz, work, tt);
     else if (K <= 4)
-        hipExtLaunchKernelGGL(eikonal_fsm_kernel<4, 2>, dim3(n), dim3(B), lds, c->stream, e0, e1, 0, va, vm, ny, nx, dx,
-                              dy, sx, sz, work, tt);
+        hipExtLaunchKernelGGL(eikonal_fsm_kernel<4, 2>, dim3(n), dim3(B), lds, c->stream, e0, e1, 0, va, vm, ny, nx, kc,
+                              sx, sz, work, tt);
     else
-        hipExtLaunchKernelGGL(eikonal_fsm_kernel<8, 1>, dim3(n), dim3(B), lds, c->stream, e0, e1, 0, va, vm, ny, nx, dx,
-                              dy, sx, sz, work, tt);
+        hipExtLaunchKernelGGL(eikonal_fsm_kernel<8, 1>, dim3(n), dim3(B), lds, c->stream, e0, e1, 0, va, vm, ny, nx, kc,
+                              sx, sz, work, tt);
 }
 
 int eikonal_events(psdm_rtm_ctx *c) {
@@ -413,6 +519,8 @@ extern "C" int psdm_rtm_eikonal(psdm_rtm_ctx *c, const float *d_vel, int ny, int
     if (int rc = check_eikonal(c, ny, nx, dx, dy, nsrc, src_x, src_z)) return rc;
     if (int rc = set_device(c)) return rc;
     const int nb = eikonal_batch(c, ny, nx, nsrc);
+    EikConsts kc;
+    if (int rc = eikonal_consts(c, dx, dy, &kc, nullptr)) return rc;
     const int *sx, *sz;
     const float *va, *vm;
     float *work;
@@ -424,7 +532,7 @@ extern "C" int psdm_rtm_eikonal(psdm_rtm_ctx *c, const float *d_vel, int ny, int
     // batches of nb sources reuse one workspace; stream order serialises them
     for (int b0 = 0; b0 < nsrc; b0 += nb) {
         const int n = std::min(nb, nsrc - b0);
-        eikonal_launch(c, va, vm, ny, nx, dx, dy, n, sx + b0, sz + b0, work, d_tt + (size_t)b0 * ncell,
+        eikonal_launch(c, va, vm, ny, nx, kc, n, sx + b0, sz + b0, work, d_tt + (size_t)b0 * ncell,
                        timed && b0 == 0 ? c->kir_ev[0] : nullptr, timed && b0 + n == nsrc ? c->kir_ev[1] : nullptr);
         HIP_TRY(c, hipGetLastError());
     }
@@ -447,6 +555,8 @@ extern "C" int psdm_rtm_eikonal_host(psdm_rtm_ctx *c, const float *h_vel, int ny
     if (int rc = get_buf(c, "eikh_vel", n * sizeof(float), &dv)) return rc;
     if (int rc = get_buf(c, "eikh_tt", n * (size_t)nb * sizeof(float), &dt)) return rc;
     HIP_TRY(c, hipMemcpyAsync(dv, h_vel, n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    EikConsts kc;
+    if (int rc = eikonal_consts(c, dx, dy, &kc, nullptr)) return rc;
     const int *sx, *sz;
     const float *va, *vm;
     float *work;
@@ -457,7 +567,7 @@ extern "C" int psdm_rtm_eikonal_host(psdm_rtm_ctx *c, const float *h_vel, int ny
         if (int rc = eikonal_events(c)) return rc;
     for (int b0 = 0; b0 < nsrc; b0 += nb) {
         const int m = std::min(nb, nsrc - b0);
-        eikonal_launch(c, va, vm, ny, nx, dx, dy, m, sx + b0, sz + b0, work, (float *)dt, timed ? c->kir_ev[0] : nullptr,
+        eikonal_launch(c, va, vm, ny, nx, kc, m, sx + b0, sz + b0, work, (float *)dt, timed ? c->kir_ev[0] : nullptr,
                        timed ? c->kir_ev[1] : nullptr);
         HIP_TRY(c, hipGetLastError());
         HIP_TRY(c, hipMemcpyAsync(h_tt + (size_t)b0 * n, dt, n * (size_t)m * sizeof(float), hipMemcpyDeviceToHost,
@@ -474,6 +584,14 @@ extern "C" int psdm_rtm_set_eikonal_workspace(psdm_rtm_ctx *c, size_t max_bytes)
     if (max_bytes == 0) return fail(c, PSDM_RTM_EINVAL, "eikonal workspace limit must be > 0");
     c->eik_work_cap = max_bytes;
     return 0;
+}
+
+extern "C" int psdm_rtm_eikonal_division(psdm_rtm_ctx *c, float dx, float dy, int *fast_mask) {
+    if (!c) return PSDM_RTM_EINVAL;
+    if (!fast_mask || !(dx > 0.f) || !(dy > 0.f)) return fail(c, PSDM_RTM_EINVAL, "eikonal_division: bad argument");
+    if (int rc = set_device(c)) return rc;
+    EikConsts kc;
+    return eikonal_consts(c, dx, dy, &kc, fast_mask);
 }
 
 extern "C" int psdm_rtm_eikonal_batch(psdm_rtm_ctx *c, int ny, int nx, int nsrc, int *nb) {

@@ -49,6 +49,14 @@ def eikonal_batch(ctx: Context, nx: int, ny: int, nsrc: int) -> int:
     return nb.value
 
 
+def eikonal_division(ctx: Context, dx: float, dy: float) -> int:
+    """Bit mask of the eikonal divisors (dx*dx, dy*dy, 2*aa) that use the two-operation
+    division for these spacings (both forms are correctly rounded)."""
+    m = C.c_int(0)
+    ctx.check(ctx._L.psdm_rtm_eikonal_division(ctx.h, dx, dy, C.byref(m)), "psdm_rtm_eikonal_division")
+    return m.value
+
+
 def image(ctx: Context, d_tts, d_ttr, d_data, nt_prop: int, dd: float, dt: float, vp0: float = 1500.0,
           theta_max: float = 60.0, theta_taper: float = 10.0, wmax: float = 1.0, out=None):
     """(nx, ny) image of one shot: d_tts (nx, ny), d_ttr (nrec, nx, ny), d_data (nrec, ld)."""

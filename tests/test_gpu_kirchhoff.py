@@ -98,6 +98,42 @@ def test_eikonal_batches_under_workspace_limit(ctx):
                                       err_msg=f"source {q}")
 
 
+@pytest.mark.parametrize("dx,dy", [(10.0, 10.0), (12.5, 7.0), (31.6821442, 31.6821442), (31.6821442, 5.0)])
+def test_eikonal_constant_divisor_division_bit_exact(ctx, dx, dy):
+    """local_solver's quotients by dx*dx, dy*dy and 2*aa: a divisor that matched the IEEE
+    division for every significand on the device uses a product and an FMA, any other keeps
+    the IEEE division; either way the travel times are the oracle's bit for bit.  dx = 10
+    (all three divisors pass the check) and f32(31.6821442)**2 = 1003.75824 (fails it: the
+    two-operation form is off by one ulp for some significand) cover both paths."""
+    import ctypes as C
+
+    import torch
+
+    from psdm_amd.kirchhoff import eikonal_division
+
+    mask = eikonal_division(ctx, dx, dy)
+    if dx == dy == 10.0:
+        assert mask == 7
+    if dx == 31.6821442:
+        assert not mask & 1
+    nx, ny = 150, 90
+    v = model(nx, ny, 17)
+    srcs = [(1, 1), (75, 2), (150, 90), (40, 61)]
+    xs = np.array([q[0] for q in srcs], np.int32)
+    zs = np.array([q[1] for q in srcs], np.int32)
+    say(f"[gpu] eikonal dx={dx} dy={dy} divisor mask {mask}")
+    d_v = torch.from_numpy(v).cuda()
+    out = torch.empty((len(srcs), nx, ny), dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    ctx.check(ctx._L.psdm_rtm_eikonal(ctx.h, C.c_void_p(d_v.data_ptr()), ny, nx, dx, dy, len(srcs),
+                                      xs.ctypes.data_as(C.c_void_p), zs.ctypes.data_as(C.c_void_p),
+                                      C.c_void_p(out.data_ptr())), "psdm_rtm_eikonal")
+    T = out.cpu().numpy()
+    for q, (sx, sz) in enumerate(srcs):
+        np.testing.assert_array_equal(T[q].view(np.uint32), K.eikonal(v, dx, dy, sx, sz).view(np.uint32),
+                                      err_msg=f"source {q}")
+
+
 def test_eikonal_workspace_default_fits_a_wide_window(ctx):
     from psdm_amd.kirchhoff import eikonal_batch
 
