@@ -13,8 +13,13 @@ import ctypes as C
 from ._native import Context
 
 
-def psdm2twt(ctx: Context, d_vel, d_psdm, dd: float, dt: float):
-    """(vel_twt, psdm_twt) as (nx, nt) tensors from (nx, nz) d_vel, d_psdm (float32, same device)."""
+def psdm2twt(ctx: Context, d_vel, d_psdm, dd: float, dt: float, out=None):
+    """(vel_twt, psdm_twt) as (nx, nt) tensors from (nx, nz) d_vel, d_psdm (float32, same device).
+
+    out: optional (vel_buf, psdm_buf), contiguous float32 tensors of nx * nt_cap elements
+    each on the same device; then one call computes nt (which must not exceed nt_cap) and
+    fills their first nx * nt elements, returned as (nx, nt) views.  Without it a length
+    query runs first and the outputs are allocated to fit."""
     import torch
 
     for t in (d_vel, d_psdm):
@@ -25,6 +30,16 @@ def psdm2twt(ctx: Context, d_vel, d_psdm, dd: float, dt: float):
     nx, nz = d_vel.shape
     nt = C.c_int()
     args = (ctx.h, nx, nz, dd, dt, C.c_void_p(d_vel.data_ptr()), C.c_void_p(d_psdm.data_ptr()))
+    if out is not None:
+        vb, pb = out
+        for t in (vb, pb):
+            if t.dtype != torch.float32 or not t.is_contiguous() or t.device != d_vel.device:
+                raise ValueError("out buffers must be contiguous float32 tensors on the inputs' device")
+        cap = min(vb.numel(), pb.numel()) // nx
+        ctx.check(ctx._L.psdm_rtm_psdm2twt(*args, cap, C.c_void_p(vb.data_ptr()), C.c_void_p(pb.data_ptr()),
+                                           C.byref(nt)), "psdm_rtm_psdm2twt")
+        n = nx * nt.value
+        return vb.view(-1)[:n].view(nx, nt.value), pb.view(-1)[:n].view(nx, nt.value)
     ctx.check(ctx._L.psdm_rtm_psdm2twt(*args, 0, None, None, C.byref(nt)), "psdm_rtm_psdm2twt")
     vt = torch.empty((nx, nt.value), dtype=torch.float32, device=d_vel.device)
     pt = torch.empty_like(vt)

@@ -87,6 +87,33 @@ def test_psdm2twt_length_query_and_errors(ctx):
                                            C.byref(nt)), "psdm2twt")
 
 
+def test_psdm2twt_single_call_into_given_buffers(ctx):
+    """With output buffers of nt_cap >= nt rows one call fills the packed (nt, nx) result at
+    their front: equal to the query-then-convert path, and to the oracle; a buffer too small
+    for nt is rejected."""
+    import torch
+
+    from psdm_amd import poststack as PS
+    from psdm_amd._native import RtmError
+
+    nz, nx, dd, dt = 200, 23, 10.0, 0.004
+    vel, psdm = model(nz, nx, dd, 5)
+    dv = torch.from_numpy(np.ascontiguousarray(vel.T)).to("cuda:0")
+    dp = torch.from_numpy(np.ascontiguousarray(psdm.T)).to("cuda:0")
+    say("[gpu] psdm2twt into given buffers")
+    vt, pt = PS.psdm2twt(ctx, dv, dp, dd, dt)
+    nt = vt.shape[1]
+    bufs = (torch.full((nx * (nt + 7),), -1.0, device="cuda:0"), torch.full((nx * (nt + 7),), -1.0, device="cuda:0"))
+    v2, p2 = PS.psdm2twt(ctx, dv, dp, dd, dt, out=bufs)
+    assert v2.shape == (nx, nt) and torch.equal(v2, vt) and torch.equal(p2, pt)
+    ov, op = P.psdm2twt(vel, psdm, dd, dt)
+    np.testing.assert_array_equal(v2.cpu().numpy().T, ov)
+    np.testing.assert_array_equal(p2.cpu().numpy().T, op)
+    small = (torch.empty(nx * (nt - 1), device="cuda:0"), torch.empty(nx * (nt - 1), device="cuda:0"))
+    with pytest.raises(RtmError, match="nt_cap"):
+        PS.psdm2twt(ctx, dv, dp, dd, dt, out=small)
+
+
 @pytest.mark.parametrize("ny,nx,seed", [(601, 37, 0), (1, 1, 1), (255, 2001, 2)])
 def test_normalize_bit_exact_vs_oracle(ctx, ny, nx, seed):
     """psdm_rtm_normalize (normalize_psdm, auxiliar_PSDM.f90:1-20) against the oracle's FP64
