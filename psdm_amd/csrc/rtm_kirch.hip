@@ -47,13 +47,15 @@ static_assert(kTrash >= kMaxBlock, "every thread of a workgroup needs its own tr
 // (Brisebarre, Muller and Raina, "Accelerating correctly rounded floating-point division
 // when the divisor is known in advance", IEEE Trans. Computers 53(8), 2004).  The host
 // establishes it per divisor by comparing both forms on the device for every significand
-// (x in [1, 2), eikdiv_check_kernel); it then holds for every |x| in [2^-60, 2^64] and for
+// (x in [1, 2), eikdiv_check_kernel); it then holds for every |x| in [2^-60, 2^96] and for
 // 0, because scaling x by a power of two scales every rounding of the form alike while no
 // product underflows or overflows (the host enables it only for 2^-30 <= y <= 2^30 and
-// zl = 0 or |zl| >= 2^-60).  local_solver's numerators are a, b, a*a, b*b and
-// -bb + sqrt(disc) >= 2*(a/(dx*dx) + b/(dy*dy)), with 0 <= a, b <= 1e9: all of them are 0 or
-// lie in that range when a and b are each 0 or >= 2^-30, which is checked once per cell;
-// any other cell, and a divisor that fails the check, take the IEEE division.
+// zl = 0 or |zl| >= 2^-60, so |x*zl| >= 2^-120 and |x*zh| <= 2^126).  local_solver's
+// numerators are a, b, a*a, b*b (a, b <= BIG = 1e9: travel times only ever decrease from
+// it) and -bb + sqrt(disc) >= 2*(a/(dx*dx) + b/(dy*dy)): all of them are 0 or lie in that
+// range when a and b are each 0 or >= 2^-30, checked once per cell, and the last one is
+// <= 2^96 (checked: a near-zero velocity can make it larger, or infinite); any other
+// numerator, and a divisor that fails the check, take the IEEE division.
 struct EikConsts {
     float dx, dy, aa;     // aa = 1/(dx*dx) + 1/(dy*dy) in the reference's order
     float zh[3], zl[3];   // per divisor dx*dx, dy*dy, 2*aa
@@ -61,8 +63,8 @@ struct EikConsts {
 };
 
 template <bool kAllow>
-__device__ __forceinline__ float divc(float x, float y, float zh, float zl, int fast, int i) {
-    if (kAllow && ((fast >> i) & 1)) return __builtin_fmaf(x, zh, x * zl);
+__device__ __forceinline__ float divc(float x, float y, float zh, float zl, int fast, int i, bool in_range = true) {
+    if (kAllow && ((fast >> i) & 1) && in_range) return __builtin_fmaf(x, zh, x * zl);
     return x / y;
 }
 
@@ -79,7 +81,8 @@ __device__ __forceinline__ float local_solver_q(float a, float b, const EikConst
     const float dmin = fminf(k.dx, k.dy);
     float t;
     if (disc >= 0.0f) {
-        t = divc<kAllow>(-bb + sqrtf(disc), 2.0f * aa, k.zh[2], k.zl[2], k.fast, 2);
+        const float x3 = -bb + sqrtf(disc);
+        t = divc<kAllow>(x3, 2.0f * aa, k.zh[2], k.zl[2], k.fast, 2, x3 <= 0x1p96f);
         if (t < fmaxf(a, b)) t = fmaxf(a, b) + v_inv * dmin;
     } else {
         t = fminf(a, b) + v_inv * dmin;

@@ -98,13 +98,16 @@ def test_eikonal_batches_under_workspace_limit(ctx):
                                       err_msg=f"source {q}")
 
 
-@pytest.mark.parametrize("dx,dy", [(10.0, 10.0), (12.5, 7.0), (31.6821442, 31.6821442), (31.6821442, 5.0)])
-def test_eikonal_constant_divisor_division_bit_exact(ctx, dx, dy):
+@pytest.mark.parametrize("dx,dy,odd", [(10.0, 10.0, False), (12.5, 7.0, False), (31.6821442, 31.6821442, False),
+                                       (31.6821442, 5.0, False), (10.0, 10.0, True)])
+def test_eikonal_constant_divisor_division_bit_exact(ctx, dx, dy, odd):
     """local_solver's quotients by dx*dx, dy*dy and 2*aa: a divisor that matched the IEEE
     division for every significand on the device uses a product and an FMA, any other keeps
     the IEEE division; either way the travel times are the oracle's bit for bit.  dx = 10
     (all three divisors pass the check) and f32(31.6821442)**2 = 1003.75824 (fails it: the
-    two-operation form is off by one ulp for some significand) cover both paths."""
+    two-operation form is off by one ulp for some significand) cover both paths.  `odd`:
+    zero and near-zero velocities in the model (1/v infinite or huge: the numerators leave
+    the range the check covers and take the IEEE division)."""
     import ctypes as C
 
     import torch
@@ -118,6 +121,10 @@ def test_eikonal_constant_divisor_division_bit_exact(ctx, dx, dy):
         assert not mask & 1
     nx, ny = 150, 90
     v = model(nx, ny, 17)
+    if odd:
+        v[60:64, 30:33] = 0.0
+        v[100, 10:20] = 1e-30
+        v[20:25, 70] = 1e-19
     srcs = [(1, 1), (75, 2), (150, 90), (40, 61)]
     xs = np.array([q[0] for q in srcs], np.int32)
     zs = np.array([q[1] for q in srcs], np.int32)
