@@ -369,8 +369,9 @@ int psdm_rtm_set_eikonal_workspace(psdm_rtm_ctx *ctx, size_t max_bytes);
 /* The eikonal's quotients by dx*dx, dy*dy and 2*aa (local_solver, solve_eikonal_FSM.f90:63-79)
  * are correctly rounded either way; bit i of *fast_mask (i = 0, 1, 2 in that order) is set when
  * that divisor uses a product and an FMA in place of the IEEE division, having matched it
- * for every significand on the device (checked on first use of a (dx, dy) pair).  No
- * counterpart in the reference. */
+ * for every significand on the device (checked on the first use of each divisor value and
+ * kept by the ctx; psdm_rtm_kirchhoff_image does the same for its divisions by dt and
+ * 2*dt).  No counterpart in the reference. */
 int psdm_rtm_eikonal_division(psdm_rtm_ctx *ctx, float dx, float dy, int *fast_mask);
 /* *nb = the sources psdm_rtm_eikonal solves per launch for this shape under the ctx's
  * workspace limit (1 <= *nb <= nsrc). */

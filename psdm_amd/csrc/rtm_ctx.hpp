@@ -63,7 +63,7 @@ struct psdm_rtm_ctx {
     double kir_ms[2] = {0.0, 0.0};
     long long kir_n[2] = {0, 0};
     size_t eik_work_cap = (size_t)16 << 30;  // eikonal workspace limit (sources run in batches)
-    std::map<unsigned long long, int> eik_fast;  // (dx, dy) bits -> divisors with the two-operation division
+    std::map<unsigned, bool> div_ok;  // divisor bits -> its two-operation division matched the IEEE one
 };
 
 namespace psdm {

@@ -47,7 +47,7 @@ static_assert(kTrash >= kMaxBlock, "every thread of a workgroup needs its own tr
 // (Brisebarre, Muller and Raina, "Accelerating correctly rounded floating-point division
 // when the divisor is known in advance", IEEE Trans. Computers 53(8), 2004).  The host
 // establishes it per divisor by comparing both forms on the device for every significand
-// (x in [1, 2), eikdiv_check_kernel); it then holds for every |x| in [2^-60, 2^96] and for
+// (x in [1, 2), divcheck_kernel); it then holds for every |x| in [2^-60, 2^96] and for
 // 0, because scaling x by a power of two scales every rounding of the form alike while no
 // product underflows or overflows (the host enables it only for 2^-30 <= y <= 2^30 and
 // zl = 0 or |zl| >= 2^-60, so |x*zl| >= 2^-120 and |x*zh| <= 2^126).  local_solver's
@@ -95,21 +95,25 @@ __device__ __forceinline__ float local_solver(float a, float b, const EikConsts 
     return ok ? local_solver_q<true>(a, b, k, v_inv) : local_solver_q<false>(a, b, k, v_inv);
 }
 
+// Three divisors fixed for a launch and their two-operation constants (divc).
+struct DivSet {
+    float y[3], zh[3], zl[3];
+};
+
 // Per divisor of `k`: whether RN(x*zh + RN(x*zl)) equals x / y for every x in [1, 2).
 // Thread t of block b tests the 32 significands from 32 * (b * blockDim + t); bad[3b + i]
 // = 1 when block b found a mismatch for divisor i.
 constexpr int kDivCheckPer = 32, kDivCheckBlock = 256;
 constexpr int kDivCheckBlocks = (1 << 23) / (kDivCheckPer * kDivCheckBlock);
-__global__ __launch_bounds__(kDivCheckBlock) void eikdiv_check_kernel(EikConsts k, int *bad) {
+__global__ __launch_bounds__(kDivCheckBlock) void divcheck_kernel(DivSet k, int *bad) {
     const unsigned base = (blockIdx.x * blockDim.x + threadIdx.x) * (unsigned)kDivCheckPer;
     int miss[3] = {0, 0, 0};
     for (int q = 0; q < kDivCheckPer; ++q) {
         const float x = __uint_as_float(0x3f800000u | (base + (unsigned)q));
-        const float ys[3] = {k.dx * k.dx, k.dy * k.dy, 2.0f * k.aa};
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
             const float f = __builtin_fmaf(x, k.zh[i], x * k.zl[i]);
-            if (__float_as_uint(f) != __float_as_uint(x / ys[i])) miss[i] = 1;
+            if (__float_as_uint(f) != __float_as_uint(x / k.y[i])) miss[i] = 1;
         }
     }
 #pragma unroll
@@ -117,6 +121,13 @@ __global__ __launch_bounds__(kDivCheckBlock) void eikdiv_check_kernel(EikConsts 
         const int m = __syncthreads_or(miss[i]);
         if (threadIdx.x == 0) bad[blockIdx.x * 3 + i] = m;
     }
+}
+
+// Whether x is 0 or in [2^-60, 2^96], the range over which a divisor that passed the check
+// gives correctly rounded quotients (see divc).
+__device__ __forceinline__ bool div_range(float x) {
+    const float ax = fabsf(x);
+    return x == 0.0f || (ax >= 0x1p-60f && ax <= 0x1p96f);
 }
 
 // Workgroup barrier for the wavefront steps: LDS writes complete, then s_barrier.  Global
@@ -339,11 +350,23 @@ __global__ __launch_bounds__(kMaxBlock) void eikonal_fsm_kernel(const float *__r
     }
 }
 
+// The imaging loop's launch constants beside its arguments: the two-operation division
+// (divc) by dt and by 2*dt where it passed the check (bit 0 / bit 1 of `fast`), and
+// cos_in = cos(theta_max) + 1e-4: a pair with cos_theta >= cos_in has theta below
+// theta_max by at least 1e-4 / sin(theta_max) radians, far beyond the error of acosf and
+// of the degree conversion, so its aperture weight is 1 without evaluating acosf.  Both
+// leave every result as the IEEE form gives it.
+struct ImgConsts {
+    float zh_t, zl_t, zh_2t, zl_2t;
+    int fast;
+    float cos_in;
+};
+
 __global__ __launch_bounds__(256) void kirchhoff_image_kernel(const float *__restrict__ tts, const float *__restrict__ ttr,
                                                               const float *__restrict__ data, int ld, int nrec,
                                                               int nt_prop, int ny, int nx, float dd, float dt,
                                                               float vp0, float theta_max, float theta_taper,
-                                                              float wmax, float *__restrict__ img) {
+                                                              float wmax, ImgConsts kc, float *__restrict__ img) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;  // pixel (j, i), j fastest
     if (p >= nx * ny) return;
     const int j = p % ny;
@@ -356,21 +379,25 @@ __global__ __launch_bounds__(256) void kirchhoff_image_kernel(const float *__res
     for (int k = 0; k < nrec; ++k) {
         const float tr = ttr[(size_t)k * n + p];
         const float tt_tot = ts + tr;
-        const float q = tt_tot / dt;
+        const float q = (kc.fast & 1) && div_range(tt_tot) ? __builtin_fmaf(tt_tot, kc.zh_t, tt_tot * kc.zl_t)
+                                                          : tt_tot / dt;
         const int it = 1 + (int)floorf(q);
         if (it <= 1 || it >= nt_prop) continue;
         const float rr = vp0 * fmaxf(tr, tmin);
         float cos_theta = z_img / (rs + rr + 1.e-6f);
         cos_theta = fmaxf(fminf(cos_theta, 1.0f), 0.0f);
-        const float theta = acosf(cos_theta) * 180.0f / pi;
         float w_theta = 1.0f;
-        if (!(theta <= theta_max)) {
-            const float u = (theta - theta_max) / theta_taper;
-            w_theta = expf(-(u * u));
+        if (!(cos_theta >= kc.cos_in)) {
+            const float theta = acosf(cos_theta) * 180.0f / pi;
+            if (!(theta <= theta_max)) {
+                const float u = (theta - theta_max) / theta_taper;
+                w_theta = expf(-(u * u));
+            }
         }
         const float weight = w_theta * fminf(cos_theta / sqrtf(rs * rr + eps), wmax);
         const float *trace = data + (size_t)k * ld;
-        const float amp = (trace[it] - trace[it - 2]) / (2.0f * dt);  // Data_target(it+1) - Data_target(it-1)
+        const float d2 = trace[it] - trace[it - 2];  // Data_target(it+1) - Data_target(it-1)
+        const float amp = (kc.fast & 2) && div_range(d2) ? __builtin_fmaf(d2, kc.zh_2t, d2 * kc.zl_2t) : d2 / (2.0f * dt);
         acc = acc + weight * amp;
     }
     img[p] = acc;
@@ -431,47 +458,67 @@ int eikonal_setup(psdm_rtm_ctx *c, const float *d_vel, int ny, int nx, int nsrc,
     return 0;
 }
 
+// zh, zl of up to three divisors and the bit mask of those that take the two-operation
+// form of divc: divisors in [2^-30, 2^30] with zl = 0 or |zl| >= 2^-60 whose form matched
+// the IEEE division for every significand on the device (divcheck_kernel, run once per
+// divisor value and kept by the ctx).
+int fast_divisors(psdm_rtm_ctx *c, const float ys[3], DivSet *k, int *mask_out) {
+    int mask = 0, todo = 0;
+    for (int i = 0; i < 3; ++i) {
+        const float zh = 1.0f / ys[i];
+        const float zl = (float)(1.0 / (double)ys[i] - (double)zh);
+        k->y[i] = ys[i];
+        k->zh[i] = zh;
+        k->zl[i] = zl;
+        if (!(ys[i] >= 0x1p-30f && ys[i] <= 0x1p30f && (zl == 0.0f || std::fabs(zl) >= 0x1p-60f))) continue;
+        unsigned bits;
+        std::memcpy(&bits, &ys[i], 4);
+        auto it = c->div_ok.find(bits);
+        if (it == c->div_ok.end())
+            todo |= 1 << i;
+        else if (it->second)
+            mask |= 1 << i;
+    }
+    if (todo) {
+        void *v;
+        if (int rc = get_buf(c, "divchk", (size_t)kDivCheckBlocks * 3 * sizeof(int), &v)) return rc;
+        hipLaunchKernelGGL(divcheck_kernel, dim3(kDivCheckBlocks), dim3(kDivCheckBlock), 0, c->stream, *k, (int *)v);
+        HIP_TRY(c, hipGetLastError());
+        std::vector<int> bad((size_t)kDivCheckBlocks * 3);
+        HIP_TRY(c, hipMemcpyAsync(bad.data(), v, bad.size() * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        int fail_bits = 0;
+        for (size_t q = 0; q < bad.size(); ++q)
+            if (bad[q]) fail_bits |= 1 << (int)(q % 3);
+        for (int i = 0; i < 3; ++i) {
+            if (!((todo >> i) & 1)) continue;
+            unsigned bits;
+            std::memcpy(&bits, &ys[i], 4);
+            const bool ok = !((fail_bits >> i) & 1);
+            c->div_ok[bits] = ok;
+            if (ok) mask |= 1 << i;
+        }
+    }
+    *mask_out = mask;
+    return 0;
+}
+
 // local_solver's constants for the grid spacings: aa = 1/(dx*dx) + 1/(dy*dy) in the
-// reference's order, and per divisor (dx*dx, dy*dy, 2*aa) zh = RN(1/y), zl ~ 1/y - zh and
-// whether divc may use them, established once per (dx, dy) by eikdiv_check_kernel and
-// kept by the ctx.
+// reference's order, and per divisor (dx*dx, dy*dy, 2*aa) zh, zl and whether divc may use
+// them.
 int eikonal_consts(psdm_rtm_ctx *c, float dx, float dy, EikConsts *k, int *fast_mask) {
     const float d0 = dx * dx, d1 = dy * dy;
     const float aa = 1.0f / d0 + 1.0f / d1;
     k->dx = dx;
     k->dy = dy;
     k->aa = aa;
-    k->fast = 0;
     const float ys[3] = {d0, d1, 2.0f * aa};
-    int eligible = 0;
-    for (int i = 0; i < 3; ++i) {
-        const float zh = 1.0f / ys[i];
-        const float zl = (float)(1.0 / (double)ys[i] - (double)zh);
-        k->zh[i] = zh;
-        k->zl[i] = zl;
-        if (ys[i] >= 0x1p-30f && ys[i] <= 0x1p30f && (zl == 0.0f || std::fabs(zl) >= 0x1p-60f)) eligible |= 1 << i;
-    }
-    unsigned bx, by;
-    std::memcpy(&bx, &dx, 4);
-    std::memcpy(&by, &dy, 4);
-    const unsigned long long key = (unsigned long long)bx << 32 | by;
-    auto it = c->eik_fast.find(key);
+    DivSet ds;
     int mask;
-    if (it != c->eik_fast.end()) {
-        mask = it->second;
-    } else {
-        void *v;
-        if (int rc = get_buf(c, "eik_divchk", (size_t)kDivCheckBlocks * 3 * sizeof(int), &v)) return rc;
-        hipLaunchKernelGGL(eikdiv_check_kernel, dim3(kDivCheckBlocks), dim3(kDivCheckBlock), 0, c->stream, *k,
-                           (int *)v);
-        HIP_TRY(c, hipGetLastError());
-        std::vector<int> bad((size_t)kDivCheckBlocks * 3);
-        HIP_TRY(c, hipMemcpyAsync(bad.data(), v, bad.size() * sizeof(int), hipMemcpyDeviceToHost, c->stream));
-        HIP_TRY(c, hipStreamSynchronize(c->stream));
-        mask = eligible;
-        for (size_t q = 0; q < bad.size(); ++q)
-            if (bad[q]) mask &= ~(1 << (int)(q % 3));
-        c->eik_fast[key] = mask;
+    if (int rc = fast_divisors(c, ys, &ds, &mask)) return rc;
+    for (int i = 0; i < 3; ++i) {
+        k->zh[i] = ds.zh[i];
+        k->zl[i] = ds.zl[i];
     }
     k->fast = mask;
     if (fast_mask) *fast_mask = mask;
@@ -618,10 +665,23 @@ extern "C" int psdm_rtm_kirchhoff_image(psdm_rtm_ctx *c, const float *d_tts, con
     if (timed)
         for (int q = 2; q < 4; ++q)
             if (!c->kir_ev[q]) HIP_TRY(c, hipEventCreate(&c->kir_ev[q]));
+    ImgConsts kc;
+    {
+        const float ys[3] = {dt, 2.0f * dt, dt};
+        DivSet ds;
+        int mask;
+        if (int rc = fast_divisors(c, ys, &ds, &mask)) return rc;
+        kc.zh_t = ds.zh[0];
+        kc.zl_t = ds.zl[0];
+        kc.zh_2t = ds.zh[1];
+        kc.zl_2t = ds.zl[1];
+        kc.fast = mask & 3;
+        kc.cos_in = (float)(std::cos((double)theta_max * 3.14159265358979323846 / 180.0) + 1e-4);
+    }
     const int np = nx * ny;
     hipExtLaunchKernelGGL(kirchhoff_image_kernel, dim3((np + 255) / 256), dim3(256), 0, c->stream,
                           timed ? c->kir_ev[2] : nullptr, timed ? c->kir_ev[3] : nullptr, 0, d_tts, d_ttr, d_data, ld,
-                          nrec, nt_prop, ny, nx, dd, dt, vp0, theta_max, theta_taper, wmax, d_img);
+                          nrec, nt_prop, ny, nx, dd, dt, vp0, theta_max, theta_taper, wmax, kc, d_img);
     HIP_TRY(c, hipGetLastError());
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     if (timed) {
