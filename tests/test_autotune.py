@@ -38,3 +38,42 @@ def test_tune_prefers_full_rounds(monkeypatch):
     assert best["score"] == pytest.approx(2.51)
     assert (16, 4, 16, 4) in calls  # the partial round of 16 slots was timed at 4 busy
     assert not any(t.get("nslots") == 32 for t in trials)
+
+
+def test_tune_plain_picks_the_fastest_shape_and_leaves_it_set():
+    """tune_plain on a stand-in context whose step kernel time depends on the launch shape:
+    every (tile, width) is probed twice (once untimed), the fastest is chosen, and it is the
+    shape left on the context."""
+
+    class Ctx:
+        def __init__(self):
+            self.shape = [8, 2]
+            self.timing = 0
+            self.runs = []
+
+        def set_launch(self, tile_x=8):
+            self.shape[0] = tile_x
+
+        def set_stream_width(self, w):
+            self.shape[1] = w
+
+        def set_timing(self, n):
+            self.timing = n
+
+        def sync(self):
+            pass
+
+        def step_stats(self):
+            tx, sw = self.shape
+            return 100, 0.01 + 0.001 * abs(tx - 16) / 8 + 0.002 * abs(sw - 2)
+
+    ctx = Ctx()
+
+    def probe():
+        ctx.runs.append((tuple(ctx.shape), ctx.timing))
+
+    best, trials = A.tune_plain(ctx, probe)
+    assert (best["tile_x"], best["stream_w"]) == (16, 2)
+    assert ctx.shape == [16, 2] and ctx.timing == 0
+    assert len(trials) == 12 and len(ctx.runs) == 24
+    assert all(((t["tile_x"], t["stream_w"]), 0) in ctx.runs for t in trials)
