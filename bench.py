@@ -150,6 +150,36 @@ def accuracy_line(ctx, torch, dev, W, geos, d_mig, d_src, obs, cfg):
     ctx.sync()
     img_batched = d_img2.cpu().numpy().T
     e_plain, e_batched = rel_l2(img_plain, fx["image_fp64"]), rel_l2(img_batched, fx["image_fp64"])
+    # the opt-in zero-sum weights (psdm_rtm_set_weights): the same shot, plain and batched
+    ctx.set_weights("zero_sum")
+    try:
+        d_rec_z = torch.empty((sh.nxRec.size, nt), device=dev)
+        d_img_z = torch.zeros((sh.nx_w, ny), device=dev)
+        forward_device(ctx, geos[0], win, d_src, d_rec_z, key=-7)
+        backward_device(ctx, geos[0], win, obs[0].flip(1).contiguous(), d_img_z, key=-7)
+        ctx.sync()
+    finally:
+        ctx.set_weights("reference")
+    d_img_zb = torch.zeros((sh.nx_w, ny), device=dev)
+    eng = Engine(ctx, d_mig, ny, m.nx, nslots=1, max_nx=sh.nx_w, max_nt=nt, dPML=W["dPML"], fs=W["fs"], dmodel=m.dx,
+                 dt=W["dt"], tile_x=cfg["tile_x"], stream_w=cfg.get("stream_w", 0), groups=1,
+                 chunk_steps=cfg.get("chunk_steps", 64), weights="zero_sum")
+    eng.submit(**job_of(W, 0, d_src, obs), d_image_out=d_img_zb)
+    eng.drain()
+    eng.close()
+    ctx.sync()
+    img_z, img_zb = d_img_z.cpu().numpy().T, d_img_zb.cpu().numpy().T
+    e_z, e_zb = rel_l2(img_z, fx["image_fp64"]), rel_l2(img_zb, fx["image_fp64"])
+    zero_sum = {"plain": {"image_rel_l2_vs_fp64": e_z,
+                          "traces_rel_l2_vs_fp64": rel_l2(d_rec_z.cpu().numpy().T[:, fx["trace_channels"]],
+                                                          fx["traces_fp64"])},
+                "batched": {"image_rel_l2_vs_fp64": e_zb},
+                "batched_image_equals_plain": bool(np.array_equal(img_z, img_zb)),
+                "image_rel_l2_vs_reference_form": rel_l2(img_z, img_plain),
+                "within_bar": bool(max(e_z, e_zb) <= ACC_BAR),
+                "note": "opt-in zero-sum second-derivative weights: the FP32 weights of the reference form sum to "
+                        "-5.9e-8 instead of 0 (a spurious mass term whose phase error grows with the record); the "
+                        "zero-sum form removes it, and no longer reproduces the reference's FP32 output bit for bit"}
     return {"workload": f"mcs2001 shot 0, {ny}x{sh.nx_w} window, nt {nt} (full record)",
             "reference": "FP64 oracle fixture (tests/golden/accuracy_mcs2001_fp64.npz)",
             "gather_matches_fixture": bool(same_gather),
@@ -157,8 +187,10 @@ def accuracy_line(ctx, torch, dev, W, geos, d_mig, d_src, obs, cfg):
             "batched": {"image_rel_l2_vs_fp64": e_batched},
             "batched_image_equals_plain": bool(np.array_equal(img_plain, img_batched)),
             "bar": ACC_BAR, "within_bar": bool(max(e_plain, e_batched) <= ACC_BAR),
-            "note": "both forms reproduce the reference's FP32 arithmetic bit for bit; the distance to FP64 is the "
-                    "FP32 rounding the reference's own solver accumulates over the record"}
+            "note": "both forms reproduce the reference's FP32 arithmetic bit for bit; their distance to FP64 is set by "
+                    "the FP32 rounding of the reference's FD weights (an FP64 run with FP32-rounded weights is 1.3e-4 "
+                    "away), not by rounding that accumulates in the time loop",
+            "zero_sum": zero_sum}
 
 
 def model_gathers(ctx, torch, dev, W, geos_model, d_true, d_wat, d_src, nshots):
@@ -337,7 +369,7 @@ def run_batched(ctx, torch, dev, W, d_mig, d_src, obs, steps, warmup, cfg):
                  snap_format=cfg.get("snap_format", "f32"),
                  store_snap=cfg.get("store_snap", 1), kahan=cfg.get("kahan", False),
                  chunk_steps=cfg.get("chunk_steps", 64), d_psdm_sum=psdm_sum, d_cov_sum=fold,
-                 steps_per_launch=cfg.get("steps_per_launch", 1))
+                 steps_per_launch=cfg.get("steps_per_launch", 1), weights=cfg.get("weights", "reference"))
     for k in range(warmup, warmup + steps):  # the shots the plain form times
         eng.submit(**job_of(W, k % len(shots), d_src, obs))
     ctx.sync()
@@ -1069,6 +1101,7 @@ def main():
     ap.add_argument("--no-wide", action="store_true", help="skip the plain form on the 801 x 5102 window")
     ap.add_argument("--wide-steps", type=int, default=1500, help="plain form on the 801 x 5102 window: steps timed")
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--no-zero-sum", action="store_true", help="skip the batched form with zero-sum weights")
     ap.add_argument("--no-tune", action="store_true", help="plain and batched forms with the default launch shapes")
     ap.add_argument("--plain-cfg", default=None,
                     help="plain form on the benchmark's window: launch shape tile_x=..,stream_w=.. instead of the "
@@ -1239,6 +1272,23 @@ def main():
             batched["stack_rel_diff_vs_plain"] = float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
             batched["fold_equal"] = bool(torch.equal(B["fold"], P["fold"]))
         log(f"[bench] batched {batched['shots_per_s']} shots/s")
+        if not args.no_zero_sum:
+            # the same shots and launch shape with the opt-in zero-sum weights (one step per launch)
+            zcfg = dict(cfg, weights="zero_sum", steps_per_launch=1)
+            BZ = run_batched(ctx, torch, dev, W, d_mig, d_src, obs, args.steps, args.warmup, zcfg)
+            stz = BZ["stats"]
+            zb = 0.5 * (step_bytes(ny, nxw, dPML, 4.0, 1, backward=False) +
+                        step_bytes(ny, nxw, dPML, 4.0, 1, backward=True)) * stz["slot_steps"] / max(stz["steps_issued"], 1)
+            achz = zb / (BZ["launch_ms"] * 1e-3) / 1e9
+            batched["zero_sum"] = {
+                "shots_per_s": round(args.steps / BZ["elapsed"], 4),
+                "gpt_updates_per_s": round(args.steps * upd_shot / BZ["elapsed"] / 1e9, 3),
+                "mean_kernel_ms": round(BZ["launch_ms"], 5), "roofline_frac": round(achz / HBM_PEAK_GBS, 4),
+                "config": {k: v for k, v in zcfg.items()},
+                "stack_rel_l2_vs_reference_weights": float(
+                    np.linalg.norm((BZ["psdm_sum"] - B["psdm_sum"]).cpu().numpy()) /
+                    max(np.linalg.norm(B["psdm_sum"].cpu().numpy()), 1e-300))}
+            log(f"[bench] batched, zero-sum weights {batched['zero_sum']['shots_per_s']} shots/s")
 
     def traffic_of(name, key):
         """HBM bytes measured with rocprofv3 counters (profiles/<name>), per `key` unit."""
@@ -1258,7 +1308,8 @@ def main():
             acc["full"] = accuracy_line(ctx, torch, dev, W, geos, d_mig, d_src, obs,
                                         batched["config"] if batched else dict(tile_x=16, stream_w=4))
             log(f"[bench] full-record accuracy: plain {acc['full']['plain']['image_rel_l2_vs_fp64']:.3g}, "
-                f"batched {acc['full']['batched']['image_rel_l2_vs_fp64']:.3g} (bar {ACC_BAR})")
+                f"batched {acc['full']['batched']['image_rel_l2_vs_fp64']:.3g}, zero-sum weights "
+                f"{acc['full']['zero_sum']['batched']['image_rel_l2_vs_fp64']:.3g} (bar {ACC_BAR})")
 
     head = batched or plain
     form = "batched" if batched else "plain"
@@ -1300,6 +1351,15 @@ def main():
         "cpu_baseline": cpu,
         "accuracy": acc,
     }
+    if batched is not None and "zero_sum" in batched and acc is not None and "full" in acc:
+        z = acc["full"]["zero_sum"]
+        out["within_accuracy_bar"] = {
+            "bar": ACC_BAR, "reference_weights": acc["full"]["within_bar"], "zero_sum_weights": z["within_bar"],
+            "shots_per_s_zero_sum_batched": batched["zero_sum"]["shots_per_s"],
+            "image_rel_l2_vs_fp64": {"reference_weights": acc["full"]["batched"]["image_rel_l2_vs_fp64"],
+                                     "zero_sum_weights": z["batched"]["image_rel_l2_vs_fp64"]},
+            "note": "value is the reference-weight form (bit-exact with the reference's FP32 solver); the opt-in "
+                    "zero-sum form is the one within the FP64 bar over the full record"}
     print(json.dumps(out), flush=True)
     ctx.close()
 

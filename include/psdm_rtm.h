@@ -131,6 +131,19 @@ int psdm_rtm_set_launch(psdm_rtm_ctx *ctx, int tile_x, int use_graph, int graph_
 /* Interior columns per wavefront work item of the one-shot solver (default 2; shorter
  * items spread one shot over more wavefronts). */
 int psdm_rtm_set_stream_width(psdm_rtm_ctx *ctx, int stream_w);
+/* Second-derivative weights of the device solver (psdm_rtm_*_device and the engine's
+ * own cfg.weights), an opt-in that has no counterpart in the reference:
+ *   0 (default): the reference's form, sum over k = 0..3 of (p(i+k) + p(i-k)) * w_k
+ *     (solver_space_derivs_clean.f90:1-69), bit-exact with the reference's FP32 results;
+ *   1: the zero-sum form, sum over k = 1..3 of ((p(i+k) - p) + (p(i-k) - p)) * w_k.
+ *     The reference's FP32 weights sum to -5.9e-8 instead of 0, a spurious mass term
+ *     whose phase error grows with the record; the zero-sum form removes it (601 x 715
+ *     window, 6000 steps: image 2.1e-5 rel-L2 from the FP64 form of the same algorithm
+ *     instead of 1.1e-4), at the cost of no longer reproducing the reference's FP32
+ *     output bit for bit.
+ * Applies to every later one-shot solve on the context; the Fortran shim never sets it,
+ * so the drop-in keeps the reference form. */
+int psdm_rtm_set_weights(psdm_rtm_ctx *ctx, int weights);
 int psdm_rtm_set_timing(psdm_rtm_ctx *ctx, int timing);
 int psdm_rtm_step_stats(psdm_rtm_ctx *ctx, long long *n_steps, double *mean_ms);
 
@@ -172,6 +185,9 @@ typedef struct {
                                 computed redundantly over a halo, so each launch reads and writes a shot's
                                 wavefield once per two steps; FP32 snapshots only, ny and windows >= 16);
                                 results are bit-identical to 1 */
+    int weights;             /* second-derivative weights: 0 the reference's form (bit-exact with the
+                                reference), 1 the zero-sum form (see psdm_rtm_set_weights); 1 needs
+                                steps_per_launch <= 1 */
 } psdm_rtm_engine_cfg;
 
 typedef struct {

@@ -55,6 +55,7 @@ def lib():
         L.rtm_oracle_fd_coef_f64.argtypes = [_I, _P]
         L.rtm_oracle_fd_coef_f32.argtypes = [_I, _P]
         L.rtm_oracle_deriv_f64.argtypes = [_I, _I, _I, _P, _P]
+        L.rtm_oracle_deriv_f32.argtypes = [_I, _I, _I, _P, _P]
         _oracle = L
     return _oracle
 
@@ -157,12 +158,15 @@ def fd_coef(num: int, precision: str = "f64"):
     return c
 
 
-def deriv(which: int, p: np.ndarray) -> np.ndarray:
-    """One full-grid derivative pass in FP64 (solver_space_derivs_clean.f90):
-    which 0 = d2/dx2, 1 = d2/dy2, 2 = d/dx, 3 = d/dy; p indexed [row, col]."""
-    pf = np.asfortranarray(p, dtype=np.float64)
+def deriv(which: int, p: np.ndarray, precision: str = "f64") -> np.ndarray:
+    """One full-grid derivative pass (solver_space_derivs_clean.f90): which 0 = d2/dx2,
+    1 = d2/dy2, 2 = d/dx, 3 = d/dy, 4 / 5 = the zero-sum forms of 0 / 1 (the opt-in
+    zero-sum weights mode); p indexed [row, col]."""
+    dt_ = np.float64 if precision == "f64" else np.float32
+    pf = np.asfortranarray(p, dtype=dt_)
     d = np.zeros_like(pf, order="F")
-    lib().rtm_oracle_deriv_f64(which, pf.shape[0], pf.shape[1], _ptr(pf), _ptr(d))
+    fn = lib().rtm_oracle_deriv_f64 if precision == "f64" else lib().rtm_oracle_deriv_f32
+    fn(which, pf.shape[0], pf.shape[1], _ptr(pf), _ptr(d))
     return d
 
 

@@ -168,13 +168,23 @@ void rtm_oracle_pml_tables_f64(int mode, int dPML, float dt, float d, double *fA
 void rtm_oracle_fd_coef_f64(int num, double *c) { fd_coef_f64(num, c); }
 void rtm_oracle_fd_coef_f32(int num, float *c) { fd_coef_f32(num, c); }
 
-/* One full-grid derivative pass: which = 0 d2/dx2, 1 d2/dy2, 2 d/dx, 3 d/dy (grid (nyt,nxt)). */
+/* One full-grid derivative pass: which = 0 d2/dx2, 1 d2/dy2, 2 d/dx, 3 d/dy, 4 / 5 the
+ * zero-sum forms of 0 / 1 (grid (nyt,nxt)). */
 void rtm_oracle_deriv_f64(int which, int nyt, int nxt, const double *p, double *d) {
     double c1[4], c2[4];
     fd_coef_f64(1, c1);
     fd_coef_f64(2, c2);
-    if (which == 0) deriv2x_f64(3, c2, nyt, nxt, p, d);
-    else if (which == 1) deriv2y_f64(3, c2, nyt, nxt, p, d);
+    if (which == 0 || which == 4) deriv2x_f64(3, c2, nyt, nxt, p, d, which == 4);
+    else if (which == 1 || which == 5) deriv2y_f64(3, c2, nyt, nxt, p, d, which == 5);
     else if (which == 2) derivx_f64(3, c1, nyt, nxt, p, d);
     else derivy_f64(3, c1, nyt, nxt, p, d);
+}
+void rtm_oracle_deriv_f32(int which, int nyt, int nxt, const float *p, float *d) {
+    float c1[4], c2[4];
+    fd_coef_f32(1, c1);
+    fd_coef_f32(2, c2);
+    if (which == 0 || which == 4) deriv2x_f32(3, c2, nyt, nxt, p, d, which == 4);
+    else if (which == 1 || which == 5) deriv2y_f32(3, c2, nyt, nxt, p, d, which == 5);
+    else if (which == 2) derivx_f32(3, c1, nyt, nxt, p, d);
+    else derivy_f32(3, c1, nyt, nxt, p, d);
 }

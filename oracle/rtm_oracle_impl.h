@@ -31,10 +31,22 @@
 
 #define AT(a, j, i, ld) (a)[(size_t)(i) * (size_t)(ld) + (size_t)(j)]
 
-/* space_deriv2x, solver_space_derivs_clean.f90:1-37.  k outer, i, j inner. */
-static void FN(deriv2x)(int acc, const R *coef, int nyt, int nxt, const R *p, R *d) {
+/* space_deriv2x, solver_space_derivs_clean.f90:1-37.  k outer, i, j inner.
+ * zs = 1: the zero-sum form of the same stencil (not the reference's arithmetic; the
+ * opt-in "zero-sum weights" mode of this repository): sum over k = 1..acc of
+ * ((p(i+k) - p(i)) + (p(i-k) - p(i))) * coef[k], whose implied centre weight is exactly
+ * -2 * (coef[1] + ... + coef[acc]), so a constant field has a zero derivative in any
+ * precision.  With FP32 weights the reference form's weights sum to -5.9e-8 instead. */
+static void FN(deriv2x)(int acc, const R *coef, int nyt, int nxt, const R *p, R *d, int zs) {
     int i1 = acc, i2 = nxt - 1 - acc;
     memset(d, 0, sizeof(R) * (size_t)nyt * nxt);
+    if (zs)
+        for (int k = 1; k <= acc; k++)
+            for (int i = i1; i <= i2; i++)
+                for (int j = 0; j < nyt; j++)
+                    AT(d, j, i, nyt) = AT(d, j, i, nyt) + ((AT(p, j, i + k, nyt) - AT(p, j, i, nyt)) +
+                                                           (AT(p, j, i - k, nyt) - AT(p, j, i, nyt))) * coef[k];
+    else
     for (int k = 0; k <= acc; k++)
         for (int i = i1; i <= i2; i++)
             for (int j = 0; j < nyt; j++)
@@ -45,12 +57,18 @@ static void FN(deriv2x)(int acc, const R *coef, int nyt, int nxt, const R *p, R 
         for (int j = 0; j < nyt; j++) AT(d, j, i, nyt) = AT(d, j, nxt - 1 - acc, nyt);
 }
 
-/* space_deriv2y, solver_space_derivs_clean.f90:39-69.  i outer, j, k inner. */
-static void FN(deriv2y)(int acc, const R *coef, int nyt, int nxt, const R *p, R *d) {
+/* space_deriv2y, solver_space_derivs_clean.f90:39-69.  i outer, j, k inner.
+ * zs = 1: the zero-sum form, as in deriv2x. */
+static void FN(deriv2y)(int acc, const R *coef, int nyt, int nxt, const R *p, R *d, int zs) {
     int j1 = acc, j2 = nyt - 1 - acc;
     memset(d, 0, sizeof(R) * (size_t)nyt * nxt);
     for (int i = 0; i < nxt; i++) {
         for (int j = j1; j <= j2; j++)
+            if (zs)
+                for (int k = 1; k <= acc; k++)
+                    AT(d, j, i, nyt) = AT(d, j, i, nyt) + ((AT(p, j + k, i, nyt) - AT(p, j, i, nyt)) +
+                                                           (AT(p, j - k, i, nyt) - AT(p, j, i, nyt))) * coef[k];
+            else
             for (int k = 0; k <= acc; k++)
                 AT(d, j, i, nyt) = AT(d, j, i, nyt) + (AT(p, j + k, i, nyt) + AT(p, j - k, i, nyt)) * coef[k];
         for (int j = 0; j < j1; j++) AT(d, j, i, nyt) = AT(d, acc, i, nyt);
@@ -210,6 +228,8 @@ static int FN(solver)(int iprop, int variant, int table_mode, int dPML, int fs, 
      * reference's bookkeeping (which, with the drivers' nt_snap = nt, reads unfilled
      * slots); it is the pairing the batched engine uses, checked here in FP64. */
     const int aligned = (variant & 2) != 0;
+    /* variant bit 4: zero-sum weights in the second-derivative passes (deriv2x above) */
+    const int zsum = (variant & 4) != 0;
     variant &= 1;
     int dn = dPML + 2 * acc;
     int nxt = nx + 2 * dn, nyt = dn + ny + dn;
@@ -311,8 +331,8 @@ static int FN(solver)(int iprop, int variant, int table_mode, int dPML, int fs, 
             if (jj >= 0 && jj < nyt && ii >= 0 && ii < nxt)
                 AT(source, jj, ii, nyt) = (R)S[(size_t)s * nt + (k - 2)];
         }
-        FN(deriv2x)(acc, coef2, nyt, nxt, p, d2x);
-        FN(deriv2y)(acc, coef2, nyt, nxt, p, d2y);
+        FN(deriv2x)(acc, coef2, nyt, nxt, p, d2x, zsum);
+        FN(deriv2y)(acc, coef2, nyt, nxt, p, d2y, zsum);
         FN(derivx)(acc, coef1, nyt, nxt, p, d1x);
         FN(derivy)(acc, coef1, nyt, nxt, p, d1y);
 

@@ -33,6 +33,7 @@ SIGNATURES = {
     "psdm_rtm_set_launch": ([_P, _I, _I, _I], _I),
     "psdm_rtm_set_stream_width": ([_P, _I], _I),
     "psdm_rtm_set_timing": ([_P, _I], _I),
+    "psdm_rtm_set_weights": ([_P, _I], _I),
     "psdm_rtm_step_stats": ([_P, C.POINTER(_LL), C.POINTER(C.c_double)], _I),
     "psdm_rtm_engine_create": ([_P, _P, C.POINTER(_P)], _I),
     "psdm_rtm_engine_destroy": ([_P], _I),
@@ -150,6 +151,12 @@ class Context:
 
     def set_stream_width(self, stream_w: int):
         self.check(self._L.psdm_rtm_set_stream_width(self.h, stream_w), "set_stream_width")
+
+    def set_weights(self, weights: str):
+        """Second-derivative weights of the device solver: "reference" (the default,
+        bit-exact with the reference) or "zero_sum" (psdm_rtm_set_weights)."""
+        code = {"reference": 0, "zero_sum": 1}[weights]
+        self.check(self._L.psdm_rtm_set_weights(self.h, code), "set_weights")
 
     def set_timing(self, every: int):
         self.check(self._L.psdm_rtm_set_timing(self.h, every), "set_timing")

@@ -276,6 +276,7 @@ int prepare(psdm_rtm_ctx *c, const Layout &L, const float *d_ctmp, const float *
     a.ram = L.ram;
     a.dt = L.dt;
     a.ss_f = (float)L.ss;
+    a.zsum = c->weights == 1;
     a.lds = L.lds;
     a.snap_stride = (size_t)L.lds * L.nx;
     a.scale_stride = L.T.ri * L.nx;
@@ -702,6 +703,13 @@ int psdm_rtm_set_stream_width(psdm_rtm_ctx *c, int stream_w) {
     if (!c) return PSDM_RTM_EINVAL;
     if (stream_w < 1 || stream_w > 256) return fail(c, PSDM_RTM_EINVAL, "stream_w must be in 1..256");
     c->stream_w = stream_w;
+    return 0;
+}
+
+int psdm_rtm_set_weights(psdm_rtm_ctx *c, int weights) {
+    if (!c) return PSDM_RTM_EINVAL;
+    if (weights != 0 && weights != 1) return fail(c, PSDM_RTM_EINVAL, "weights must be 0 (reference) or 1 (zero-sum)");
+    c->weights = weights;
     return 0;
 }
 

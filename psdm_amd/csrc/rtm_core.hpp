@@ -30,6 +30,8 @@ enum SnapFmt { kSnapF32 = 0, kSnapF16 = 1, kSnapBF16 = 2 };
 // Snapshot / imaging variant a step kernel is compiled for (the interior carries only the
 // registers its variant needs): FP32 or 16-bit snapshots, plain or Kahan imaging.
 enum SnapVariant { kSvF32 = 0, kSvF32Kahan = 1, kSv16 = 2, kSv16Kahan = 3 };
+// Added to a SnapVariant: the kernel's second derivatives use the zero-sum weights (d2z).
+constexpr int kSvZeroSum = 4;
 constexpr int snap_variant(int snap_fmt, int kahan) { return (snap_fmt != kSnapF32 ? 2 : 0) + (kahan ? 1 : 0); }
 
 // Everything one time step of one shot needs.  Layout in HBM:
@@ -146,6 +148,7 @@ struct StepArgs {
     int zc, zp, zm;   // state not yet written this phase reads as zero: p(k-1), p(k-2), CPML memory
     int snap_fmt;     // SnapFmt
     int kahan;        // compensated image accumulation
+    int zsum;         // host side: launch the zero-sum-weight kernels (d2z) instead of the reference's (d2)
     float dt, ss_f;   // dt; float(store_snap)
     const float *pc;  // p at step k-1
     float *pn;        // in: p at step k-2; out: p at step k

@@ -36,6 +36,7 @@ struct psdm_rtm_ctx {
     std::vector<std::pair<void *, size_t>> spare;  // released snapshot buffers
     int tile_x = 8;
     int stream_w = 2;  // interior columns per stream item of the one-shot solver
+    int weights = 0;   // second-derivative weights of the device solver: 0 reference form, 1 zero-sum form
     int use_graph = 0, graph_steps = 64;
     int timing = 0;                                 // sample every `timing`-th step kernel (0 = off)
     std::vector<hipEvent_t> ev_free;                // recycled events

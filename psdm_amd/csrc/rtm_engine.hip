@@ -173,7 +173,7 @@ BatchArgs group_args(const psdm_rtm_engine *e, int q, int goff, int compact, int
 // One launch of slot group q at chunk offset g: one time step, or two with temporal
 // blocking (the edge-tile kernel, then the core kernel, on the same stream).
 void launch_group(const psdm_rtm_engine *e, int q, int g, int compact, hipStream_t st) {
-    const int sv = snap_variant(e->cfg.snap_format, e->cfg.kahan);
+    const int sv = snap_variant(e->cfg.snap_format, e->cfg.kahan) + (e->cfg.weights == 1 ? kSvZeroSum : 0);
     int grid;
     const BatchArgs b = group_args(e, q, g, compact, &grid);
     if (e->spl == 2) {
@@ -344,6 +344,7 @@ int admit(psdm_rtm_engine *e, int s, Job &job, size_t arena_off) {
     a.ram = e->cfg.variant == 0;
     a.snap_fmt = e->cfg.snap_format;
     a.kahan = e->cfg.kahan;
+    a.zsum = e->cfg.weights == 1;
     a.dt = e->cfg.dt;
     a.ss_f = (float)e->cfg.store_snap;
     a.c2 = c2;
@@ -566,6 +567,10 @@ int psdm_rtm_engine_create(psdm_rtm_ctx *c, const psdm_rtm_engine_cfg *cfg, psdm
         return fail(c, PSDM_RTM_EINVAL, "steps_per_launch 2 needs ny >= 16");
     if (cfg->steps_per_launch == 2 && cfg->dPML + 6 > 256)
         return fail(c, PSDM_RTM_EINVAL, "steps_per_launch 2 needs dPML <= 250");
+    if (cfg->weights != 0 && cfg->weights != 1)
+        return fail(c, PSDM_RTM_EINVAL, "weights must be 0 (reference) or 1 (zero-sum)");
+    if (cfg->weights == 1 && cfg->steps_per_launch == 2)
+        return fail(c, PSDM_RTM_EINVAL, "zero-sum weights need steps_per_launch <= 1");
     if (!cfg->d_vel) return fail(c, PSDM_RTM_EINVAL, "null velocity model");
     if (!(cfg->dt > 0.f) || !(cfg->dmodel > 0.f)) return fail(c, PSDM_RTM_EINVAL, "dt and dmodel must be > 0");
     auto *e = new psdm_rtm_engine();

@@ -45,8 +45,10 @@ __device__ __forceinline__ void stage_p(gptr<const float> pc, int ld, int nyt, i
 // solver_ac_backward_ram.f90:455-481).  Only the rows the item owns are stored.
 template <int SV>
 __device__ __forceinline__ void stream_item(const StepArgs &a, int u, v4f *stage) {
-    constexpr bool kF32 = SV == kSvF32 || SV == kSvF32Kahan;  // else 16-bit (FP16 or BF16)
-    constexpr bool kKahan = SV == kSvF32Kahan || SV == kSv16Kahan;
+    constexpr int SB = SV & 3;                                // the SnapVariant proper
+    constexpr bool kF32 = SB == kSvF32 || SB == kSvF32Kahan;  // else 16-bit (FP16 or BF16)
+    constexpr bool kKahan = SB == kSvF32Kahan || SB == kSv16Kahan;
+    constexpr bool kZS = (SV & kSvZeroSum) != 0;
     const int lane = threadIdx.x & 63;
     const int dn = a.dn;
     const int CS = fdiv(u, a.fd_ri), R = u - CS * a.ri;
@@ -150,10 +152,17 @@ __device__ __forceinline__ void stream_item(const StepArgs &a, int u, v4f *stage
         const float b1 = from_prev_lane(w3.y), b2 = from_prev_lane(w3.z), b3 = from_prev_lane(w3.w);
         const float n0 = from_next_lane(w3.x), n1 = from_next_lane(w3.y), n2 = from_next_lane(w3.z);
         v4f lap;
-        lap.x = d2(w3.x, w4.x, w2.x, w5.x, w1.x, w6.x, w0.x) + d2(w3.x, w3.y, b3, w3.z, b2, w3.w, b1);
-        lap.y = d2(w3.y, w4.y, w2.y, w5.y, w1.y, w6.y, w0.y) + d2(w3.y, w3.z, w3.x, w3.w, b3, n0, b2);
-        lap.z = d2(w3.z, w4.z, w2.z, w5.z, w1.z, w6.z, w0.z) + d2(w3.z, w3.w, w3.y, n0, w3.x, n1, b3);
-        lap.w = d2(w3.w, w4.w, w2.w, w5.w, w1.w, w6.w, w0.w) + d2(w3.w, n0, w3.z, n1, w3.y, n2, w3.x);
+        if (kZS) {
+            lap.x = d2z(w3.x, w4.x, w2.x, w5.x, w1.x, w6.x, w0.x) + d2z(w3.x, w3.y, b3, w3.z, b2, w3.w, b1);
+            lap.y = d2z(w3.y, w4.y, w2.y, w5.y, w1.y, w6.y, w0.y) + d2z(w3.y, w3.z, w3.x, w3.w, b3, n0, b2);
+            lap.z = d2z(w3.z, w4.z, w2.z, w5.z, w1.z, w6.z, w0.z) + d2z(w3.z, w3.w, w3.y, n0, w3.x, n1, b3);
+            lap.w = d2z(w3.w, w4.w, w2.w, w5.w, w1.w, w6.w, w0.w) + d2z(w3.w, n0, w3.z, n1, w3.y, n2, w3.x);
+        } else {
+            lap.x = d2(w3.x, w4.x, w2.x, w5.x, w1.x, w6.x, w0.x) + d2(w3.x, w3.y, b3, w3.z, b2, w3.w, b1);
+            lap.y = d2(w3.y, w4.y, w2.y, w5.y, w1.y, w6.y, w0.y) + d2(w3.y, w3.z, w3.x, w3.w, b3, n0, b2);
+            lap.z = d2(w3.z, w4.z, w2.z, w5.z, w1.z, w6.z, w0.z) + d2(w3.z, w3.w, w3.y, n0, w3.x, n1, b3);
+            lap.w = d2(w3.w, w4.w, w2.w, w5.w, w1.w, w6.w, w0.w) + d2(w3.w, n0, w3.z, n1, w3.y, n2, w3.x);
+        }
         // (d2x + d2y + source) at the injection cells of this column (:443)
         while (e < ev_e) {
             int cell;
@@ -311,7 +320,7 @@ __device__ __forceinline__ void stream_item(const StepArgs &a, int u, v4f *stage
 // 1 low side, 2 high side.  Every tile spans >= 4 cells along each CPML axis, so all
 // clamped-edge stencil points (solver_space_derivs_clean.f90:24-34) and x-memory
 // neighbours lie inside the LDS-staged ranges.
-template <int TX>
+template <int TX, bool ZS>
 __device__ __forceinline__ void cpml_tile(const StepArgs &a, int xs, int ys, int j0, int j1, int i0, int i1,
                                           float *tile, float *phs) {
     constexpr int TY = kTileY, HY = TY + 6, HX = TX + 6, CPW = TX / 4;
@@ -408,14 +417,8 @@ __device__ __forceinline__ void cpml_tile(const StepArgs &a, int xs, int ys, int
         const float xm3 = Tx[-3 * HY], xp3 = Tx[3 * HY];
         const float ym1 = Ty[-1], yp1 = Ty[1], ym2 = Ty[-2], yp2 = Ty[2];
         const float ym3 = Ty[-3], yp3 = Ty[3];
-        float d2x = (px + px) * kC2_0;
-        d2x = d2x + (xp1 + xm1) * kC2_1;
-        d2x = d2x + (xp2 + xm2) * kC2_2;
-        d2x = d2x + (xp3 + xm3) * kC2_3;
-        float d2y = (py + py) * kC2_0;
-        d2y = d2y + (yp1 + ym1) * kC2_1;
-        d2y = d2y + (yp2 + ym2) * kC2_2;
-        d2y = d2y + (yp3 + ym3) * kC2_3;
+        const float d2x = ZS ? d2z(px, xp1, xm1, xp2, xm2, xp3, xm3) : d2(px, xp1, xm1, xp2, xm2, xp3, xm3);
+        const float d2y = ZS ? d2z(py, yp1, ym1, yp2, ym2, yp3, ym3) : d2(py, yp1, ym1, yp2, ym2, yp3, ym3);
         const float p = tile[(i - i0 + 3) * HY + (lane + 3)];
         const float A = Av[q], B = Bv[q];
         // x-direction memory (strips xl/xr and the corners), pml_matrices_x/_xycorner(1)
@@ -509,10 +512,10 @@ __device__ __forceinline__ void step_item(const StepArgs &a, int t, float *lds) 
     int j0, j1, i0, i1;
     const int ys = axis_tile(R, a.rt_pml, a.rt_int, a.dn, a.ny, kTileY, a.fd_rpml, &j0, &j1);
     const int xs = axis_tile(C, a.ct_pml, a.ct_int, a.dn, a.nx, TX, a.fd_cpml, &i0, &i1);
-    cpml_tile<TX>(a, xs, ys, j0, j1, i0, i1, tile, aux);
+    cpml_tile<TX, (SV & kSvZeroSum) != 0>(a, xs, ys, j0, j1, i0, i1, tile, aux);
 }
 
-template <int TX>
+template <int TX, int SV>
 __global__ __launch_bounds__(kBlock) void rtm_step_kernel(StepArgs a) {
     __shared__ __attribute__((aligned(16))) float lds[lds_floats(TX)];
     const int b = blockIdx.x;
@@ -526,7 +529,7 @@ __global__ __launch_bounds__(kBlock) void rtm_step_kernel(StepArgs a) {
     // contiguous tile range [x*per_xcd, (x+1)*per_xcd) and shares halos in its own L2.
     const int t = (b & 7) * a.per_xcd + (b >> 3);
     if (t >= a.nitems) return;
-    step_item<TX, kSvF32>(a, t, lds);  // the one-shot solver: FP32 snapshots, plain imaging
+    step_item<TX, SV>(a, t, lds);  // the one-shot solver: FP32 snapshots, plain imaging
 }
 
 // Per-step fields of a slot at global step G (forward steps k = 2..nt, then backward
@@ -711,7 +714,8 @@ __global__ void rtm_normalize_kernel(const double *sum, const int *fold, float *
 // step reports the kernel's own duration (as a profiler's kernel trace does).
 template <int TX>
 void launch_step_tx(const StepArgs &a, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
-    hipExtLaunchKernelGGL(rtm_step_kernel<TX>, dim3(8 * a.per_xcd), dim3(kBlock), 0, s, e0, e1, 0, a);
+    auto k = a.zsum ? rtm_step_kernel<TX, kSvF32 + kSvZeroSum> : rtm_step_kernel<TX, kSvF32>;
+    hipExtLaunchKernelGGL(k, dim3(8 * a.per_xcd), dim3(kBlock), 0, s, e0, e1, 0, a);
 }
 
 void launch_step(const StepArgs &a, int tile_x, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
@@ -744,10 +748,15 @@ void launch_batch_sv(const BatchArgs &b, int tile_x, int grid, hipStream_t s, hi
 
 void launch_batch(const BatchArgs &b, int tile_x, int sv, int grid, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
     grid = (grid + 7) / 8 * 8;
+    constexpr int Z = kSvZeroSum;
     switch (sv) {
     case kSvF32Kahan: launch_batch_sv<kSvF32Kahan>(b, tile_x, grid, s, e0, e1); break;
     case kSv16: launch_batch_sv<kSv16>(b, tile_x, grid, s, e0, e1); break;
     case kSv16Kahan: launch_batch_sv<kSv16Kahan>(b, tile_x, grid, s, e0, e1); break;
+    case kSvF32 + Z: launch_batch_sv<kSvF32 + Z>(b, tile_x, grid, s, e0, e1); break;
+    case kSvF32Kahan + Z: launch_batch_sv<kSvF32Kahan + Z>(b, tile_x, grid, s, e0, e1); break;
+    case kSv16 + Z: launch_batch_sv<kSv16 + Z>(b, tile_x, grid, s, e0, e1); break;
+    case kSv16Kahan + Z: launch_batch_sv<kSv16Kahan + Z>(b, tile_x, grid, s, e0, e1); break;
     default: launch_batch_sv<kSvF32>(b, tile_x, grid, s, e0, e1); break;
     }
 }

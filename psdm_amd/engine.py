@@ -28,8 +28,10 @@ class EngineCfg(C.Structure):
         ("snap_format", _I), ("kahan", _I), ("chunk_steps", _I), ("tile_x", _I), ("stream_w", _I), ("grid", _I),
         ("groups", _I), ("use_graph", _I),
         ("max_inflight", _I), ("hbm_budget", _LL), ("d_psdm_sum", _P), ("d_cov_sum", _P),
-        ("steps_per_launch", _I),
+        ("steps_per_launch", _I), ("weights", _I),
     ]
+
+WEIGHTS = {"reference": 0, "zero_sum": 1}
 
 
 class Job(C.Structure):
@@ -54,7 +56,7 @@ class Engine:
                  dPML: int, fs: int, dmodel: float, dt: float, variant: int = 0, store_snap: int = 1,
                  snap_format: str = "f32", kahan: bool = False, chunk_steps: int = 64, tile_x: int = 32,
                  stream_w: int = 0, grid: int = 0, groups: int = 0, use_graph: bool = True, max_inflight: int = 2, hbm_budget: int = 0,
-                 d_psdm_sum=None, d_cov_sum=None, steps_per_launch: int = 1):
+                 d_psdm_sum=None, d_cov_sum=None, steps_per_launch: int = 1, weights: str = "reference"):
         self.ctx = ctx
         self._L = ctx._L
         self._keep = [d_vel, d_psdm_sum, d_cov_sum]
@@ -67,6 +69,7 @@ class Engine:
         cfg.stream_w, cfg.groups = stream_w, groups
         cfg.max_inflight, cfg.hbm_budget = max_inflight, hbm_budget
         cfg.steps_per_launch = steps_per_launch
+        cfg.weights = WEIGHTS[weights]
         cfg.d_psdm_sum = d_psdm_sum.data_ptr() if d_psdm_sum is not None else None
         cfg.d_cov_sum = d_cov_sum.data_ptr() if d_cov_sum is not None else None
         self.cfg = cfg

@@ -168,6 +168,71 @@ def test_derivatives_exact_on_polynomials():
     np.testing.assert_allclose(d1y[3:ny - 3, :], exact_d1y[3:ny - 3, :], rtol=1e-9, atol=1e-7)
 
 
+def test_zero_sum_weights_brute_force_and_constant_field():
+    """The zero-sum form of the second-derivative passes (which 4 / 5): brute force of
+    sum_k ((p(i+k) - p(i)) + (p(i-k) - p(i))) * w_k with the same clamp copies, the same
+    operator as the reference form in exact arithmetic (FP64 agreement to rounding), and
+    an exact zero on a constant field in FP32, where the reference form's FP32 weights
+    leave -5.9e-8 per unit value (their sum is not zero)."""
+    t = np.load(os.path.join(GOLD, "tables.npz"))
+    p = t["deriv_in"].astype(np.float64)
+    ny, nx = p.shape
+    c2 = O.fd_coef(2)
+    bf = np.zeros((2, ny, nx))
+    for j in range(ny):
+        for i in range(3, nx - 3):
+            bf[0, j, i] = sum(c2[k] * ((p[j, i + k] - p[j, i]) + (p[j, i - k] - p[j, i])) for k in range(1, 4))
+    for j in range(3, ny - 3):
+        for i in range(nx):
+            bf[1, j, i] = sum(c2[k] * ((p[j + k, i] - p[j, i]) + (p[j - k, i] - p[j, i])) for k in range(1, 4))
+    bf[0, :, :3] = bf[0, :, 3:4]
+    bf[0, :, nx - 3:] = bf[0, :, nx - 4:nx - 3]
+    bf[1, :3, :] = bf[1, 3:4, :]
+    bf[1, ny - 3:, :] = bf[1, ny - 4:ny - 3, :]
+    for w in (0, 1):
+        dz = O.deriv(4 + w, p)
+        np.testing.assert_allclose(dz, bf[w], rtol=1e-12, atol=1e-12)
+        # same operator as the reference form (centre weight -2 (w1 + w2 + w3) = 2 w0)
+        np.testing.assert_allclose(dz, O.deriv(w, p), rtol=1e-12, atol=1e-12 * np.abs(p).max())
+    const = np.full((ny, nx), 1.7, np.float32)
+    for w in (4, 5):
+        assert not O.deriv(w, const, precision="f32").any()
+    assert np.abs(O.deriv(0, const, precision="f32")).max() > 0  # the reference form's residual
+    w32 = O.fd_coef(2, precision="f32").astype(np.float64)
+    assert abs(w32[0] + w32[1] + w32[2] + w32[3]) > 1e-8  # its weights do not sum to zero
+
+
+def test_zero_sum_weights_exact_on_polynomials():
+    ny, nx = 20, 24
+    y = np.arange(ny, dtype=np.float64)[:, None] * np.ones((1, nx))
+    x = np.ones((ny, 1)) * np.arange(nx, dtype=np.float64)[None, :]
+    p = 0.01 * x ** 6 - 0.3 * x ** 2 + 2 * y ** 4 + x * y + 5.0
+    np.testing.assert_allclose(O.deriv(4, p)[:, 3:nx - 3], (0.3 * x ** 4 - 0.6)[:, 3:nx - 3], rtol=1e-9, atol=1e-7)
+    np.testing.assert_allclose(O.deriv(5, p)[3:ny - 3, :], (24 * y ** 2)[3:ny - 3, :], rtol=1e-9, atol=1e-7)
+
+
+@pytest.mark.parametrize("name", ["ram_fs0", "disk_fs1", "ram_ss2"])
+def test_zero_sum_solver_fp64_equals_reference_form_fp64(name):
+    """In FP64 the zero-sum solver (variant bit 4) and the reference form agree to
+    rounding; in FP32 they differ (the zero-sum form is not the reference's arithmetic)."""
+    c = case(name)
+    geo = (c["nxSou"], c["nySou"], c["nxRec"], c["nyRec"])
+    out = {}
+    for prec in ("f64", "f32"):
+        for zs in (0, 4):
+            v = c["variant"] | zs
+            Rec, F = O.forward(*geo, c["S"], c["dx"], c["dt"], c["ctmp"], c["store_snap"], c["nt_snap"], c["nt"],
+                               c["dPML"], c["fs"], variant=v, table_mode=0, precision=prec)
+            k = O.backward(*geo, np.asfortranarray(Rec[::-1, :].astype(np.float32)), c["dx"], c["dt"], c["ctmp"],
+                           c["store_snap"], c["nt_snap"], c["nt"], c["dPML"], c["fs"], Fields=F, variant=v,
+                           table_mode=0, precision=prec)
+            out[prec, zs] = (Rec, k)
+    for q in (0, 1):
+        assert rel_l2(out["f64", 4][q], out["f64", 0][q]) <= 1e-12
+        assert rel_l2(out["f32", 4][q], out["f64", 0][q]) <= 1e-5
+    assert not np.array_equal(out["f32", 4][0], out["f32", 0][0])
+
+
 def _run(c, impl="oracle", table_mode=1, precision="f32", scale=1.0):
     geo = (c["nxSou"], c["nySou"], c["nxRec"], c["nyRec"])
     Rec, Fields = O.forward(*geo, c["S"] * scale, c["dx"], c["dt"], c["ctmp"], c["store_snap"], c["nt_snap"], c["nt"],
