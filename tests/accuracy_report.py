@@ -7,7 +7,8 @@ oracle on the true and on the water-only model (direct wave removed) and migrate
 FP64 oracle with snapshots every 4th step (backward step k paired with the snapshot of
 forward step nt + 2 - k, the engine's decimated bookkeeping).  The same gathers are
 migrated on the GPU by the engine with FP32, FP16 and BF16 snapshots every 4th step and
-Kahan-compensated image accumulation.  Reported per format and shot: image rel-L2 vs
+Kahan-compensated image accumulation, each with the reference's second-derivative
+weights and with the opt-in zero-sum weights (`<fmt>_zero_sum`).  Reported per format and shot: image rel-L2 vs
 FP64 and the RMS amplitude error along each known reflector (peak |image| of the FP64
 image within +-5 samples of the reflector depth, compared at the same sample).
 
@@ -134,10 +135,10 @@ def main():
     d_mig = torch.from_numpy(np.ascontiguousarray(m.migration.T)).to(dev)
     report = {"workload": "uneven line sample, 601-deep @10 m, dt 0.8 ms, dPML 40, snapshots every 4th step",
               "shots": [{"index": k, "nx_w": shots[k].nx_w, "nt": args.nt} for k in idx], "forms": {}}
-    for fmt in ("f32", "f16", "bf16"):
+    for fmt, weights in [(f, w) for w in ("reference", "zero_sum") for f in ("f32", "f16", "bf16")]:
         eng = Engine(ctx, d_mig, m.nz, m.nx, nslots=len(idx), max_nx=max(shots[k].nx_w for k in idx),
                      max_nt=args.nt, dPML=W["dPML"], fs=W["fs"], dmodel=m.dx, dt=W["dt"], tile_x=32,
-                     snap_format=fmt, store_snap=4, kahan=True)
+                     snap_format=fmt, store_snap=4, kahan=True, weights=weights)
         imgs, keep = [], []
         for k in idx:
             sh, p = shots[k], payloads[idx.index(k)]
@@ -155,8 +156,9 @@ def main():
             res.append({"index": k, "image_rel_l2_vs_fp64": rel_l2(img, ref[k]["image64"]),
                         "reflectors": reflector_amplitude_errors(img, ref[k]["image64"], m.reflectors, shots[k].x_i,
                                                                  m.dx)})
-        report["forms"][fmt] = res
-        log(f"[accuracy] {fmt}: image rel-L2 {[round(r['image_rel_l2_vs_fp64'], 7) for r in res]}")
+        name = fmt if weights == "reference" else f"{fmt}_zero_sum"
+        report["forms"][name] = res
+        log(f"[accuracy] {name}: image rel-L2 {[round(r['image_rel_l2_vs_fp64'], 7) for r in res]}")
     print(json.dumps(report), flush=True)
     ctx.close()
 
