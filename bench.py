@@ -1134,7 +1134,7 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: this build runs the RTM path on a single MI355X "
+        print(f"bench.py measures one GPU: the library runs the RTM path on a single MI355X "
               f"(--gpus {args.gpus} is not supported; shots shard across GPUs with no per-step "
               f"communication)", file=sys.stderr)
         sys.exit(2)
