@@ -51,8 +51,10 @@ constexpr int snap_variant(int snap_fmt, int kahan) { return (snap_fmt != kSnapF
 //               top CPML [0,dn), the interior [dn,dn+ny) and the bottom CPML apart,
 //               columns likewise, so each tile is x strip, y strip or corner.  Only
 //               the non-interior tiles of that grid are used: first the top/bottom
-//               strips (2*rt_pml row tiles x nct column tiles, c = C*2rt_pml + r), then
-//               the x strips over the interior rows (2*ct_pml x rt_int, c = cc*rt_int + r).
+//               strips (2*rt_pml row tiles x the listed column tiles, c = C*2rt_pml + r;
+//               under the _ram rule the interior column tiles from column dn on are
+//               left out, ctb_skip of them), then the x strips over the interior rows
+//               (2*ct_pml x rt_int, c = cc*rt_int + r).
 //   stream items  the interior in wavefront-sized pieces: kStreamRows rows (4 per lane,
 //               one 4-row halo lane at each end) x sw columns, walked column by column
 //               with a 7-column register window (ri row tiles x cs column segments,
@@ -60,6 +62,7 @@ constexpr int snap_variant(int snap_fmt, int kahan) { return (snap_fmt != kSnapF
 struct Tiling {
     int rt_pml, rt_int, ct_pml, ct_int, nrt, nct;  // the region-aligned 64 x TX grid
     int ncpml_a, ncpml;                            // CPML tiles: top/bottom strips, all
+    int ctb_skip;                                  // interior column tiles left out of the top/bottom strips
     int ri, cs, sw, nstream, nsblk;                // stream items and their workgroups
     int nitems;                                    // workgroups of one step: ncpml + nsblk
 };
@@ -87,7 +90,12 @@ __host__ __device__ inline int stream_origin(int R, int ny) {
     return R * kStreamRows < last ? R * kStreamRows : last;
 }
 
-__host__ __device__ inline Tiling make_tiling(int ny, int nx, int dn, int tx, int sw) {
+// ram: the _ram padding rule, under which the top and bottom pads past interior column dn
+// have c2 = 0 and p stays exactly 0 (solver_ac_forward_ram.f90:288-302); their tiles are
+// left out of the work list (the caller zeroes both wavefield buffers before a shot's
+// first step), and the top/bottom strip tiles list the corner columns and the interior
+// column tiles that start before column dn only.
+__host__ __device__ inline Tiling make_tiling(int ny, int nx, int dn, int tx, int sw, bool ram) {
     Tiling T;
     T.rt_pml = cdiv(dn, kTileY);
     T.rt_int = cdiv(ny, kTileY);
@@ -95,7 +103,9 @@ __host__ __device__ inline Tiling make_tiling(int ny, int nx, int dn, int tx, in
     T.ct_int = cdiv(nx, tx);
     T.nrt = 2 * T.rt_pml + T.rt_int;
     T.nct = 2 * T.ct_pml + T.ct_int;
-    T.ncpml_a = 2 * T.rt_pml * T.nct;
+    const int live = ram ? (T.ct_int < cdiv(dn, tx) ? T.ct_int : cdiv(dn, tx)) : T.ct_int;
+    T.ctb_skip = T.ct_int - live;
+    T.ncpml_a = 2 * T.rt_pml * (T.nct - T.ctb_skip);
     T.ncpml = T.ncpml_a + 2 * T.ct_pml * T.rt_int;
     T.ri = cdiv(ny, kStreamRows);
     T.sw = sw;
@@ -140,6 +150,7 @@ struct StepArgs {
     int nitems, per_xcd;  // workgroups of one step (Tiling::nitems); per XCD
     int rt_pml, rt_int, ct_pml, ct_int;
     int ncpml_a, ncpml, ri, sw, nstream;
+    int ctb_skip;  // Tiling::ctb_skip
     FastDiv fd_2rp, fd_rint, fd_ri, fd_rpml, fd_cpml;  // division by 2*rt_pml, rt_int, ri, rt_pml, ct_pml
     int fs, ram;      // free surface flag; ram = 1 -> _ram semantics, 0 -> _disk
     int k;            // time step index of the reference loop, 2..nt
@@ -189,6 +200,7 @@ inline void set_tiling(StepArgs &a, const Tiling &T) {
     a.ct_pml = T.ct_pml;
     a.ct_int = T.ct_int;
     a.ncpml_a = T.ncpml_a;
+    a.ctb_skip = T.ctb_skip;
     a.ncpml = T.ncpml;
     a.ri = T.ri;
     a.sw = T.sw;

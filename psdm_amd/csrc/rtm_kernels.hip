@@ -502,9 +502,12 @@ __device__ __forceinline__ void step_item(const StepArgs &a, int t, float *lds) 
     }
     int R, C;
     if (t < a.ncpml_a) {  // top / bottom strips
-        C = fdiv(t, a.fd_2rp);
-        const int r = t - C * 2 * a.rt_pml;
+        const int cl = fdiv(t, a.fd_2rp);
+        const int r = t - cl * 2 * a.rt_pml;
         R = r < a.rt_pml ? r : r + a.rt_int;
+        // the listed columns: left corners, the interior column tiles before the skipped
+        // ones (_ram pads past column dn, left out), right corners
+        C = cl < a.ct_pml + a.ct_int - a.ctb_skip ? cl : cl + a.ctb_skip;
     } else {  // x strips over the interior rows
         const int t2 = t - a.ncpml_a;
         const int cc = fdiv(t2, a.fd_rint);
