@@ -438,7 +438,7 @@ def run_engine_jobs(ctx, torch, dev, d_mig, ny, nx_prop, jobs, W, cfg, budget, i
                  groups=cfg.get("groups", 0),
                  snap_format=cfg.get("snap_format", "f32"),
                  store_snap=cfg.get("store_snap", 1), kahan=cfg.get("kahan", False),
-                 chunk_steps=cfg.get("chunk_steps", 64), hbm_budget=budget)
+                 chunk_steps=cfg.get("chunk_steps", 64), hbm_budget=budget, weights=cfg.get("weights", "reference"))
     for q, j in enumerate(jobs):
         eng.submit(**j, d_image_out=images[q] if images is not None else None)
     ctx.sync()
@@ -484,21 +484,26 @@ def bench_uneven(args, mixed=False):
         f"{max(s.nx_w for s in shots)}, {upd / 1e12:.2f} T updates, snapshot budget {args.hbm_gb} GB")
 
     cfg = dict(nslots=args.slots, tile_x=16, stream_w=4, grid=0, groups=2, chunk_steps=64)
-    forms = [("f32", 1, False)] + ([("f32", 4, True), ("f16", 4, True), ("bf16", 4, True)] if mixed else [])
+    # mixed: the 16-bit forms, and FP32 / FP16 every 4th step with the zero-sum weights (the
+    # forms that stay within 1e-4 of the FP64 oracle at nt 8000, tests/accuracy_report.py)
+    forms = [("f32", 1, False, "reference")] + ([("f32", 4, True, "reference"), ("f16", 4, True, "reference"),
+                                                 ("bf16", 4, True, "reference"), ("f32", 4, True, "zero_sum"),
+                                                 ("f16", 4, True, "zero_sum")] if mixed else [])
     results = {}
-    for fmt, ss, kahan in forms:
-        c = dict(cfg, snap_format=fmt, store_snap=ss, kahan=kahan)
-        log(f"[bench] batched {fmt} every {ss}: {c}")
+    for fmt, ss, kahan, weights in forms:
+        c = dict(cfg, snap_format=fmt, store_snap=ss, kahan=kahan, weights=weights)
+        log(f"[bench] batched {fmt} every {ss}, {weights} weights: {c}")
         el, st = run_engine_jobs(ctx, torch, dev, d_mig, ny, m.nx, jobs, W, c, budget)
         esize = 4 if fmt == "f32" else 2
         sb = sum(snap_store_bytes(ny, sh.nx_w, nts[k], ss, esize) for k, sh in enumerate(shots))
         byts = sum(0.5 * (step_bytes(ny, sh.nx_w, dPML, esize, ss) + step_bytes(ny, sh.nx_w, dPML, esize, ss, True))
                    * 2 * (nts[k] - 1) for k, sh in enumerate(shots))
-        r = {"snap_format": fmt, "store_snap": ss, "kahan": kahan, "shots_per_s": round(len(shots) / el, 4),
+        r = {"snap_format": fmt, "store_snap": ss, "kahan": kahan, "weights": weights,
+             "shots_per_s": round(len(shots) / el, 4),
              "gpt_updates_per_s": round(upd / el / 1e9, 3), "elapsed_s": round(el, 3),
              "snapshot_bytes_total": int(sb), "hbm_roofline_frac": round(byts / el / (HBM_PEAK_GBS * 1e9), 4),
              "steps_issued": st["steps_issued"], "slot_steps": st["slot_steps"]}
-        results[f"{fmt}_ss{ss}"] = r
+        results[f"{fmt}_ss{ss}" + ("" if weights == "reference" else "_zero_sum")] = r
         log(f"[bench]   {r['shots_per_s']} shots/s, {r['gpt_updates_per_s']} Gupd/s, snapshots "
             f"{sb / 2 ** 30:.1f} GiB")
 
