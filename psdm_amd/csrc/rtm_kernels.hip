@@ -332,9 +332,11 @@ __device__ __forceinline__ void cpml_tile(const StepArgs &a, int xs, int ys, int
 
     // the top CPML is never updated with a free surface (:391); the _ram top and bottom
     // pads past column dn have c2 = 0 (velocity padding :288-302), so p stays exactly 0
-    // there and their CPML memory never reaches p.  Such a tile stores its zeros in the
-    // first two steps of a phase only (zp: k <= 3), one into each ping-pong buffer; nothing
-    // else writes those cells, so from step 4 on both buffers already hold the zeros.
+    // there and their CPML memory never reaches p.  The _ram pad tiles are not in the work
+    // list (make_tiling; the solver zeroes both wavefield buffers before a shot); a listed
+    // zero tile (the top strip under a free surface) stores its zeros in the first two
+    // steps of a phase only (zp: k <= 3), one into each ping-pong buffer; nothing else
+    // writes those cells, so from step 4 on both buffers already hold the zeros.
     const bool zero_tile = (a.fs && ys == 1) || (a.ram && ys != 0 && xs == 0 && i0 - dn >= dn);
     if (zero_tile) {
         if (rowok && a.zp)
