@@ -438,7 +438,8 @@ def run_engine_jobs(ctx, torch, dev, d_mig, ny, nx_prop, jobs, W, cfg, budget, i
                  groups=cfg.get("groups", 0),
                  snap_format=cfg.get("snap_format", "f32"),
                  store_snap=cfg.get("store_snap", 1), kahan=cfg.get("kahan", False),
-                 chunk_steps=cfg.get("chunk_steps", 64), hbm_budget=budget, weights=cfg.get("weights", "reference"))
+                 chunk_steps=cfg.get("chunk_steps", 64), hbm_budget=budget, weights=cfg.get("weights", "reference"),
+                 steps_per_launch=cfg.get("steps_per_launch", 1))
     for q, j in enumerate(jobs):
         eng.submit(**j, d_image_out=images[q] if images is not None else None)
     ctx.sync()
@@ -484,6 +485,17 @@ def bench_uneven(args, mixed=False):
         f"{max(s.nx_w for s in shots)}, {upd / 1e12:.2f} T updates, snapshot budget {args.hbm_gb} GB")
 
     cfg = dict(nslots=args.slots, tile_x=16, stream_w=4, grid=0, groups=2, chunk_steps=64)
+    tuned = None
+    if not args.no_tune:
+        # launch shape and steps per launch (temporal blocking) chosen by the autotuner on the
+        # line's own windows, at the given slot count and snapshot budget
+        from psdm_amd.autotune import tune
+
+        best, trials = tune(ctx, d_mig, ny, m.nx, jobs, max(s.nx_w for s in shots), max(nts), dPML, W["fs"], m.dx,
+                            W["dt"], slots=[args.slots], steps_per_launch=(1, 2), hbm_budget=budget, log=log)
+        cfg.update(tile_x=best["tile_x"], stream_w=best["stream_w"], grid=best["grid"], groups=best["groups"],
+                   steps_per_launch=best["steps_per_launch"])
+        tuned = {"best": best, "trials": trials}
     # mixed: the 16-bit forms, and FP32 / FP16 every 4th step with the zero-sum weights (the
     # forms that stay within 1e-4 of the FP64 oracle at nt 8000, tests/accuracy_report.py)
     forms = [("f32", 1, False, "reference")] + ([("f32", 4, True, "reference"), ("f16", 4, True, "reference"),
@@ -492,6 +504,8 @@ def bench_uneven(args, mixed=False):
     results = {}
     for fmt, ss, kahan, weights in forms:
         c = dict(cfg, snap_format=fmt, store_snap=ss, kahan=kahan, weights=weights)
+        if fmt != "f32" or weights != "reference":
+            c["steps_per_launch"] = 1  # two steps per launch take FP32 snapshots and the reference weights only
         log(f"[bench] batched {fmt} every {ss}, {weights} weights: {c}")
         el, st = run_engine_jobs(ctx, torch, dev, d_mig, ny, m.nx, jobs, W, c, budget)
         esize = 4 if fmt == "f32" else 2
@@ -552,7 +566,9 @@ def bench_uneven(args, mixed=False):
         "config": {"workload": f"uneven line: {len(shots)} shots on a 40 km 601-deep model @10 m, windows "
                                f"600-2000 columns, nt in {{4000,6000,8000}}, dt 0.8 ms, dPML 40",
                    "form": "batched" + (" FP16 snapshots every 4th step" if mixed else ""),
-                   "slots": args.slots, "tile_x": 16, "stream_w": 4, "snapshot_budget_gb": args.hbm_gb},
+                   "slots": args.slots, "launch": {k: cfg[k] for k in ("tile_x", "stream_w", "grid", "groups")},
+                   "steps_per_launch": cfg.get("steps_per_launch", 1), "snapshot_budget_gb": args.hbm_gb,
+                   "autotune": tuned},
         "gpt_updates_per_s": head["gpt_updates_per_s"],
         "forms": results, "plain": plain,
         "speedup_batched_vs_plain": round(base["shots_per_s"] / plain["shots_per_s"], 3) if plain else None,

@@ -23,7 +23,8 @@ import time
 from .engine import Engine
 
 
-def _trial(ctx, d_vel, ny, nx_prop, jobs, nslots, busy, max_nx, nt, dPML, fs, dx, dt, cfg, chunk_steps, chunks):
+def _trial(ctx, d_vel, ny, nx_prop, jobs, nslots, busy, max_nx, nt, dPML, fs, dx, dt, cfg, chunk_steps, chunks,
+           hbm_budget=0):
     """Seconds per engine step with `busy` of `nslots` slots occupied."""
     import torch
 
@@ -33,7 +34,7 @@ def _trial(ctx, d_vel, ny, nx_prop, jobs, nslots, busy, max_nx, nt, dPML, fs, dx
         cfg["steps_per_launch"] = spl
     nt_trial = min(nt, (chunks + 1) * chunk_steps * spl // 2 + 1)  # forward and backward inside the trial
     eng = Engine(ctx, d_vel, ny, nx_prop, nslots=nslots, max_nx=max_nx, max_nt=nt, dPML=dPML, fs=fs, dmodel=dx, dt=dt,
-                 chunk_steps=chunk_steps, **cfg)
+                 chunk_steps=chunk_steps, hbm_budget=hbm_budget, **cfg)
     try:
         for q in range(busy):
             eng.submit(**dict(jobs[q % len(jobs)], nt=nt_trial))
@@ -68,8 +69,10 @@ def round_time(t_step, nshots, nslots, t_part=None):
 
 def tune(ctx, d_vel, ny, nx_prop, jobs, max_nx, nt, dPML, fs, dx, dt, snap_format="f32", store_snap=1, kahan=False,
          slots=None, tiles=(16, 32), stream_ws=(4, 8), grids=(0,), groups=(2,), steps_per_launch=(1,),
-         tb_stream_ws=(8, 16), chunk_steps=32, chunks=5, nshots=None, log=None):
-    """jobs: list of Engine.submit keyword dicts.  Returns (best config dict, trials)."""
+         tb_stream_ws=(8, 16), chunk_steps=32, chunks=5, nshots=None, hbm_budget=0, log=None):
+    """jobs: list of Engine.submit keyword dicts.  hbm_budget: the trial engines' snapshot
+    arena (0 = the engine's default, nslots x the largest shot at `nt`; give the workload's
+    budget where that would not fit in HBM).  Returns (best config dict, trials)."""
     slots = list(slots) if slots else slot_candidates(nshots)
     say = log or (lambda *a: None)
     trials = []
@@ -78,7 +81,7 @@ def tune(ctx, d_vel, ny, nx_prop, jobs, max_nx, nt, dPML, fs, dx, dt, snap_forma
     def run(nslots, busy, cfg):
         try:
             return _trial(ctx, d_vel, ny, nx_prop, jobs, nslots, busy, max_nx, nt, dPML, fs, dx, dt, dict(base, **cfg),
-                          chunk_steps, chunks)
+                          chunk_steps, chunks, hbm_budget)
         except Exception as exc:  # e.g. a slot count whose snapshot stores exceed HBM
             say(f"[autotune] slots={nslots} busy={busy} {cfg}: skipped ({exc})")
             return None

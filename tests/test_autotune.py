@@ -22,8 +22,7 @@ def test_round_time_counts_the_partial_round_at_its_own_cost():
 def test_tune_prefers_full_rounds(monkeypatch):
     calls = []
 
-    def fake_trial(ctx, d_vel, ny, nx_prop, jobs, nslots, busy, *rest):
-        cfg = rest[-3]
+    def fake_trial(ctx, d_vel, ny, nx_prop, jobs, nslots, busy, max_nx, nt, dPML, fs, dx, dt, cfg, *rest):
         calls.append((nslots, busy, cfg["tile_x"], cfg["stream_w"]))
         if nslots == 32:
             raise RuntimeError("out of memory")
