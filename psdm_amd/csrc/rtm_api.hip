@@ -164,7 +164,7 @@ Layout make_layout(const psdm_rtm_geom *g, int tile_x, int stream_w) {
     L.dx = g->dmodel;
     L.dt = g->dt;
     L.tile_x = tile_x;
-    L.T = psdm::make_tiling(L.ny, L.nx, L.dn, tile_x, stream_w, L.ram != 0);  // state zeroed per call
+    L.T = psdm::make_tiling(L.ny, L.nx, L.dn, tile_x, stream_w, L.ram != 0, L.fs != 0);  // state zeroed per call
     L.per_xcd = (L.T.nitems + 7) / 8;
     return L;
 }

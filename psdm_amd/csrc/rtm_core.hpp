@@ -51,9 +51,10 @@ constexpr int snap_variant(int snap_fmt, int kahan) { return (snap_fmt != kSnapF
 //               top CPML [0,dn), the interior [dn,dn+ny) and the bottom CPML apart,
 //               columns likewise, so each tile is x strip, y strip or corner.  Only
 //               the non-interior tiles of that grid are used: first the top/bottom
-//               strips (2*rt_pml row tiles x the listed column tiles, c = C*2rt_pml + r;
+//               strips (tb_rows row tiles x the listed column tiles, c = C*tb_rows + r;
 //               under the _ram rule the interior column tiles from column dn on are
-//               left out, ctb_skip of them), then the x strips over the interior rows
+//               left out, ctb_skip of them, and with a free surface the top row tiles),
+//               then the x strips over the interior rows
 //               (2*ct_pml x rt_int, c = cc*rt_int + r).
 //   stream items  the interior in wavefront-sized pieces: kStreamRows rows (4 per lane,
 //               one 4-row halo lane at each end) x sw columns, walked column by column
@@ -63,6 +64,7 @@ struct Tiling {
     int rt_pml, rt_int, ct_pml, ct_int, nrt, nct;  // the region-aligned 64 x TX grid
     int ncpml_a, ncpml;                            // CPML tiles: top/bottom strips, all
     int ctb_skip;                                  // interior column tiles left out of the top/bottom strips
+    int tb_rows;                                   // row tiles listed per top/bottom column (bottom only with fs)
     int ri, cs, sw, nstream, nsblk;                // stream items and their workgroups
     int nitems;                                    // workgroups of one step: ncpml + nsblk
 };
@@ -92,10 +94,12 @@ __host__ __device__ inline int stream_origin(int R, int ny) {
 
 // ram: the _ram padding rule, under which the top and bottom pads past interior column dn
 // have c2 = 0 and p stays exactly 0 (solver_ac_forward_ram.f90:288-302); their tiles are
-// left out of the work list (the caller zeroes both wavefield buffers before a shot's
-// first step), and the top/bottom strip tiles list the corner columns and the interior
-// column tiles that start before column dn only.
-__host__ __device__ inline Tiling make_tiling(int ny, int nx, int dn, int tx, int sw, bool ram) {
+// left out of the work list, and the top/bottom strip tiles list the corner columns and
+// the interior column tiles that start before column dn only.  fs: with a free surface
+// the top strip and its corners are never updated (:391, :400-416) and p stays 0 there,
+// so only the bottom row tiles are listed.  Either way the caller zeroes both wavefield
+// buffers before a shot's first step.
+__host__ __device__ inline Tiling make_tiling(int ny, int nx, int dn, int tx, int sw, bool ram, bool fs) {
     Tiling T;
     T.rt_pml = cdiv(dn, kTileY);
     T.rt_int = cdiv(ny, kTileY);
@@ -105,7 +109,8 @@ __host__ __device__ inline Tiling make_tiling(int ny, int nx, int dn, int tx, in
     T.nct = 2 * T.ct_pml + T.ct_int;
     const int live = ram ? (T.ct_int < cdiv(dn, tx) ? T.ct_int : cdiv(dn, tx)) : T.ct_int;
     T.ctb_skip = T.ct_int - live;
-    T.ncpml_a = 2 * T.rt_pml * (T.nct - T.ctb_skip);
+    T.tb_rows = (fs ? 1 : 2) * T.rt_pml;
+    T.ncpml_a = T.tb_rows * (T.nct - T.ctb_skip);
     T.ncpml = T.ncpml_a + 2 * T.ct_pml * T.rt_int;
     T.ri = cdiv(ny, kStreamRows);
     T.sw = sw;
@@ -151,7 +156,8 @@ struct StepArgs {
     int rt_pml, rt_int, ct_pml, ct_int;
     int ncpml_a, ncpml, ri, sw, nstream;
     int ctb_skip;  // Tiling::ctb_skip
-    FastDiv fd_2rp, fd_rint, fd_ri, fd_rpml, fd_cpml;  // division by 2*rt_pml, rt_int, ri, rt_pml, ct_pml
+    int tb_rows;   // Tiling::tb_rows
+    FastDiv fd_tbr, fd_rint, fd_ri, fd_rpml, fd_cpml;  // division by tb_rows, rt_int, ri, rt_pml, ct_pml
     int fs, ram;      // free surface flag; ram = 1 -> _ram semantics, 0 -> _disk
     int k;            // time step index of the reference loop, 2..nt
     int mode;         // 0 forward (snapshots), 1 backward (imaging)
@@ -201,11 +207,12 @@ inline void set_tiling(StepArgs &a, const Tiling &T) {
     a.ct_int = T.ct_int;
     a.ncpml_a = T.ncpml_a;
     a.ctb_skip = T.ctb_skip;
+    a.tb_rows = T.tb_rows;
     a.ncpml = T.ncpml;
     a.ri = T.ri;
     a.sw = T.sw;
     a.nstream = T.nstream;
-    a.fd_2rp = make_fastdiv(2 * T.rt_pml);
+    a.fd_tbr = make_fastdiv(T.tb_rows);
     a.fd_rint = make_fastdiv(T.rt_int);
     a.fd_ri = make_fastdiv(T.ri);
     a.fd_rpml = make_fastdiv(T.rt_pml);

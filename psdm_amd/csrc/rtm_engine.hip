@@ -256,7 +256,7 @@ int admit(psdm_rtm_engine *e, int s, Job &job, size_t arena_off) {
     const psdm_rtm_job &J = job.j;
     const int ny = e->cfg.ny, nx = J.nx_w, nt = J.nt, dn = e->dn, TX = e->cfg.tile_x;
     const int nxt = nx + 2 * dn;
-    const Tiling T = make_tiling(ny, nx, dn, TX, e->sw, e->cfg.variant == 0);
+    const Tiling T = make_tiling(ny, nx, dn, TX, e->sw, e->cfg.variant == 0, e->cfg.fs != 0);
     Slot &sl = e->slots[s];
 
     // injection events: forward source (1 point), backward receivers (last writer wins)
@@ -362,8 +362,9 @@ int admit(psdm_rtm_engine *e, int s, Job &job, size_t arena_off) {
     a.snap_stride = (size_t)e->lds * nx;
     a.rec_row = -1;
     float *st = e->d_state + (size_t)s * e->state_stride;
-    // both wavefield buffers start at zero: the _ram top/bottom pad tiles are not in the
-    // work list (make_tiling), so nothing else writes their zeros (stream order: the
+    // both wavefield buffers start at zero: the _ram top/bottom pad tiles and, with a free
+    // surface, the top strip are not in the work list (make_tiling), so nothing writes
+    // their zeros (stream order: the
     // slot's previous shot has finished; admission is never inside a graph capture)
     HIP_TRY(c, hipMemsetAsync(st, 0, 2 * e->n_p * sizeof(float), c->stream));
     d.pA = st + e->off;
@@ -596,7 +597,7 @@ int psdm_rtm_engine_create(psdm_rtm_ctx *c, const psdm_rtm_engine_cfg *cfg, psdm
         e->cap_tiles = T.ne;
         e->cap_core = T.ncblk;
     } else {
-        e->cap_tiles = make_tiling(cfg->ny, cfg->max_nx, e->dn, cfg->tile_x, e->sw, cfg->variant == 0).nitems;
+        e->cap_tiles = make_tiling(cfg->ny, cfg->max_nx, e->dn, cfg->tile_x, e->sw, cfg->variant == 0, cfg->fs != 0).nitems;
     }
     e->groups = cfg->groups > 0 ? cfg->groups : 2;
     if (!cfg->use_graph) e->groups = 1;

@@ -332,11 +332,10 @@ __device__ __forceinline__ void cpml_tile(const StepArgs &a, int xs, int ys, int
 
     // the top CPML is never updated with a free surface (:391); the _ram top and bottom
     // pads past column dn have c2 = 0 (velocity padding :288-302), so p stays exactly 0
-    // there and their CPML memory never reaches p.  The _ram pad tiles are not in the work
-    // list (make_tiling; the solver zeroes both wavefield buffers before a shot); a listed
-    // zero tile (the top strip under a free surface) stores its zeros in the first two
-    // steps of a phase only (zp: k <= 3), one into each ping-pong buffer; nothing else
-    // writes those cells, so from step 4 on both buffers already hold the zeros.
+    // there and their CPML memory never reaches p.  make_tiling leaves both kinds out of
+    // the work list (the solver zeroes both wavefield buffers before a shot); a caller with
+    // its own work list that does list such a tile gets its zeros stored in the first two
+    // steps of a phase (zp: k <= 3), one into each ping-pong buffer.
     const bool zero_tile = (a.fs && ys == 1) || (a.ram && ys != 0 && xs == 0 && i0 - dn >= dn);
     if (zero_tile) {
         if (rowok && a.zp)
@@ -504,9 +503,10 @@ __device__ __forceinline__ void step_item(const StepArgs &a, int t, float *lds) 
     }
     int R, C;
     if (t < a.ncpml_a) {  // top / bottom strips
-        const int cl = fdiv(t, a.fd_2rp);
-        const int r = t - cl * 2 * a.rt_pml;
-        R = r < a.rt_pml ? r : r + a.rt_int;
+        const int cl = fdiv(t, a.fd_tbr);
+        const int r = t - cl * a.tb_rows;
+        // both strips' row tiles, or with a free surface the bottom strip's only
+        R = a.tb_rows == 2 * a.rt_pml ? (r < a.rt_pml ? r : r + a.rt_int) : r + a.rt_pml + a.rt_int;
         // the listed columns: left corners, the interior column tiles before the skipped
         // ones (_ram pads past column dn, left out), right corners
         C = cl < a.ct_pml + a.ct_int - a.ctb_skip ? cl : cl + a.ctb_skip;
